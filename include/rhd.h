@@ -1,0 +1,213 @@
+/*
+ * rhd — MI355X-native sensor development for rohditor (C ABI).
+ *
+ * The stage is normalize -> highlight (Off / Clip / Opposed) -> demosaic
+ * (bilinear / MHC / RCD / AMaZE) -> capture sharpening (masked
+ * Richardson-Lucy), producing f32 linear camera RGB kept resident on the GPU
+ * as three planes. It replaces the reference's wgpu executor
+ * `GpuSensorProcessor` (crates/gpu/src/sensor, crates/gpu/src/capture)
+ * behind the same stage API; every entry point below cites the reference
+ * interface it replaces.
+ *
+ * Conventions: plain pointers and sizes only. Every call that launches GPU
+ * work runs on the context's own HIP stream. Handles are move-only in the
+ * reference (each stage consumes its input); here a stage that consumes a
+ * handle says so and releases it, even on error. All functions are safe to
+ * call from one thread per context; contexts are independent.
+ */
+#ifndef RHD_H
+#define RHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHD_ABI_VERSION 1
+
+/* crates/gpu/src/lib.rs:55-98 (GpuPreviewError). */
+typedef enum rhd_status {
+    RHD_OK = 0,
+    RHD_CANCELLED = 1,
+    RHD_UNSUPPORTED = 2,        /* device cannot run the stage */
+    RHD_OVER_BUDGET = 3,        /* routing: run on the CPU, keep the device */
+    RHD_INVALID_DIMENSIONS = 4,
+    RHD_BASE_MISMATCH = 5,
+    RHD_UNSUPPORTED_EDITS = 6,  /* routing: run on the CPU, keep the device */
+    RHD_SYNCHRONIZATION = 7,    /* a HIP call or kernel failed */
+    RHD_INVALID_INPUT = 8,      /* invalid metadata, non-finite samples, bad handles */
+    RHD_READBACK = 9
+} rhd_status;
+
+/* crates/image/src/lib.rs:52-58 (BayerPattern); codes as in
+ * crates/gpu/src/sensor/normalize.rs:695-702. */
+typedef enum rhd_cfa { RHD_RGGB = 0, RHD_BGGR = 1, RHD_GRBG = 2, RHD_GBRG = 3 } rhd_cfa;
+
+/* crates/highlight/src/lib.rs:137-141 (HighlightExecution). */
+typedef enum rhd_highlight_method {
+    RHD_HIGHLIGHT_OFF = 0,
+    RHD_HIGHLIGHT_CLIP = 1,
+    RHD_HIGHLIGHT_OPPOSED = 2
+} rhd_highlight_method;
+
+/* crates/demosaic/src/lib.rs:125-141 (DemosaicAlgorithm; Neural is not offered). */
+typedef enum rhd_demosaic_algorithm {
+    RHD_DEMOSAIC_BILINEAR = 0,
+    RHD_DEMOSAIC_MHC = 1,
+    RHD_DEMOSAIC_RCD = 2,
+    RHD_DEMOSAIC_AMAZE = 3
+} rhd_demosaic_algorithm;
+
+/*
+ * Resolved sensor-development description: crates/core/src/pipeline/
+ * sensor.rs:282-338 (SensorDevelopmentDescription) restricted to what the
+ * stage reads. Everything is resolved on the host before the call, as
+ * `from_frame` does: crop, level patterns, highlight levels, demosaic balance.
+ */
+typedef struct rhd_sensor_desc {
+    uint32_t sensor_width, sensor_height; /* RawFileInfo width/height */
+    uint64_t row_stride;                  /* RawFrame.row_stride, in samples */
+    int32_t sensor_pattern;               /* rhd_cfa of the full sensor */
+    uint32_t crop_x, crop_y, crop_width, crop_height; /* resolved crop */
+    const float* black_levels;            /* LevelPattern.values (host memory) */
+    uint32_t black_repeat_width, black_repeat_height;
+    const float* white_levels;            /* 1, 3 or repeat_w * repeat_h values */
+    uint32_t white_count;
+    int32_t highlight;                    /* rhd_highlight_method */
+    float clip_levels[3];                 /* ChannelClipLevels (Clip) */
+    float detection_levels[3];            /* ChannelDetectionLevels (Opposed) */
+    int32_t has_neutral_ceiling;          /* OpposedOptions.neutral_ceiling is Some */
+    float neutral_ceiling[3];
+    int32_t demosaic;                     /* rhd_demosaic_algorithm */
+    float demosaic_balance[3];            /* as-shot WhiteBalanceGains */
+} rhd_sensor_desc;
+
+/*
+ * Detail contract with capture sharpening only: crates/core/src/
+ * sharpening.rs:19-69 (CaptureSharpeningContract) and crates/edit/src/
+ * sharpening.rs:22-81 (CaptureSharpening). `iterations` is 8 in the
+ * reference (sharpening.rs:13); other counts are an extension. False colour
+ * and noise reduction are not built: a non-zero flag is UnsupportedEdits.
+ */
+typedef struct rhd_detail_desc {
+    int32_t capture_enabled;
+    float amount;           /* [0, 1] */
+    float sigma;            /* "radius": Gaussian sigma in [0.3, 1.2] */
+    float noise_protection; /* [0, 1] */
+    float ceilings[3];      /* crates/core/src/pipeline/detail.rs:323-337 */
+    int32_t iterations;
+    int32_t false_colour;
+    int32_t noise_reduction;
+} rhd_detail_desc;
+
+/* crates/highlight/src/lib.rs:253-258 (ClipStats). */
+typedef struct rhd_clip_stats {
+    uint64_t affected_sites, changed_sites, nominal_over_white_sites;
+    uint64_t affected_by_channel[3];
+} rhd_clip_stats;
+
+/* crates/highlight/src/lib.rs:212-224 (OpposedStats). */
+typedef struct rhd_opposed_stats {
+    uint64_t suspected_clipped_sites, reconstructed_sites, changed_sites, fallback_sites;
+    uint64_t fully_unsupported_sites, neutralised_sites;
+    uint64_t suspected_by_channel[3];
+    uint32_t regions; /* clipped regions found (0 when nothing was suspected) */
+} rhd_opposed_stats;
+
+/* SensorMetrics / HighlightMetrics / DemosaicMetrics / CaptureMetrics
+ * (normalize.rs:16-28, highlight.rs:22-35, demosaic.rs:48-58,
+ * capture/mod.rs:30-45), with GPU stage times from HIP events. */
+typedef struct rhd_metrics {
+    float normalize_ms, highlight_ms, demosaic_ms, capture_ms, total_ms;
+    uint64_t uploaded_bytes, readback_bytes, estimated_gpu_bytes;
+    uint32_t launches;
+    int32_t highlight_method;
+    rhd_clip_stats clip;
+    rhd_opposed_stats opposed;
+} rhd_metrics;
+
+typedef struct rhd_context rhd_context; /* GpuSensorProcessor */
+typedef struct rhd_mosaic rhd_mosaic;   /* resident u16 RawFrame mosaic */
+typedef struct rhd_plane rhd_plane;     /* GpuNormalizedMosaic / GpuHighlightedMosaic */
+typedef struct rhd_source rhd_source;   /* GpuSensorCameraSource: 3 resident planes */
+
+/* GpuSensorProcessor::new (crates/gpu/src/sensor/normalize.rs:91) plus
+ * set_budget (:157): `budget_bytes` caps the logical bytes held by live
+ * handles and scratch (crates/gpu/src/memory.rs:42-81); 0 selects 64 GiB
+ * (the reference defaults to 768 MiB, resources.rs:3, sized for desktop GPUs). */
+rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out);
+void rhd_destroy(rhd_context* ctx);
+rhd_status rhd_set_budget(rhd_context* ctx, uint64_t budget_bytes);
+uint64_t rhd_reserved_bytes(const rhd_context* ctx);
+/* Message of the last failing call on this context (never NULL). */
+const char* rhd_last_error(const rhd_context* ctx);
+const char* rhd_status_name(rhd_status status);
+/* GpuPreviewError::is_routing (crates/gpu/src/lib.rs:104-109). */
+int rhd_status_is_routing(rhd_status status);
+
+/* GpuSensorProcessor::supports (crates/gpu/src/sensor/develop.rs:55-66),
+ * extended with Opposed; `detail` may be NULL. Returns 1 or 0. */
+int rhd_supports(const rhd_sensor_desc* desc, const rhd_detail_desc* detail);
+
+/* Upload of a decoded RawFrame mosaic (normalize.rs:308-351 packs and
+ * uploads per tile; here the whole frame goes up once and stays resident so
+ * re-develops reuse it). `host` holds row_stride * height samples. */
+rhd_status rhd_mosaic_upload(rhd_context* ctx, const uint16_t* host, uint64_t row_stride,
+                             uint32_t width, uint32_t height, rhd_mosaic** out);
+void rhd_mosaic_release(rhd_mosaic* mosaic);
+
+/* GpuSensorProcessor::normalize (crates/gpu/src/sensor/normalize.rs:170). */
+rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_sensor_desc* desc,
+                         const volatile uint32_t* cancel, rhd_plane** out, rhd_metrics* metrics);
+/* GpuSensorProcessor::apply_highlight (crates/gpu/src/sensor/highlight.rs:87),
+ * in place on `plane`, which stays owned by the caller. */
+rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* desc,
+                         const volatile uint32_t* cancel, rhd_metrics* metrics);
+/* GpuSensorProcessor::demosaic (crates/gpu/src/sensor/demosaic.rs:74),
+ * including capture sharpening when `detail` is active. Consumes `plane`. */
+rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* desc,
+                        const rhd_detail_desc* detail, const volatile uint32_t* cancel,
+                        rhd_source** out, rhd_metrics* metrics);
+/* GpuCaptureProcessor::process_resident_from_resident
+ * (crates/gpu/src/capture/mod.rs:54), in place on `source`. */
+rhd_status rhd_capture(rhd_context* ctx, rhd_source* source, const rhd_detail_desc* detail,
+                       const volatile uint32_t* cancel, rhd_metrics* metrics);
+
+/* GpuSensorProcessor::develop (crates/gpu/src/sensor/develop.rs:68-142):
+ * normalize -> highlight -> demosaic (+ capture) from a resident mosaic. */
+rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
+                                const rhd_sensor_desc* desc, const rhd_detail_desc* detail,
+                                const volatile uint32_t* cancel, rhd_source** out,
+                                rhd_metrics* metrics);
+/* The same from a host RawFrame mosaic (upload included). */
+rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t row_stride,
+                       const rhd_sensor_desc* desc, const rhd_detail_desc* detail,
+                       const volatile uint32_t* cancel, rhd_source** out, rhd_metrics* metrics);
+
+/* Test and CPU-handoff readbacks (the reference's test readbacks). */
+rhd_status rhd_plane_read(rhd_context* ctx, const rhd_plane* plane, float* dst, uint64_t count);
+rhd_status rhd_plane_write(rhd_context* ctx, rhd_plane* plane, const float* src, uint64_t count);
+/* A resident f32 mosaic made from host samples, for staged tests. */
+rhd_status rhd_plane_upload(rhd_context* ctx, const float* host, uint32_t width, uint32_t height,
+                            int32_t pattern, rhd_plane** out);
+void rhd_plane_dims(const rhd_plane* plane, uint32_t* width, uint32_t* height, int32_t* pattern);
+void rhd_plane_release(rhd_plane* plane);
+
+/* Interleaved W x H x 3 readback matching LinearRgbImage. */
+rhd_status rhd_source_read_rgb(rhd_context* ctx, const rhd_source* source, float* dst,
+                               uint64_t count);
+/* Device pointers of the three planes (R, G, B; W*H floats each). */
+rhd_status rhd_source_planes(const rhd_source* source, float** red, float** green, float** blue);
+void rhd_source_dims(const rhd_source* source, uint32_t* width, uint32_t* height);
+void rhd_source_release(rhd_source* source);
+
+/* Blocks until the context's stream is idle. */
+rhd_status rhd_synchronize(rhd_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RHD_H */

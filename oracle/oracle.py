@@ -152,6 +152,15 @@ def opposed(mosaic: np.ndarray, pattern: int, detection, neutral_ceiling=None):
     return st, m, stats, labels, int(count.value)
 
 
+def rcd_probe(mosaic: np.ndarray, pattern: int, red: int, green: int) -> np.ndarray:
+    """Stage values of the first RCD tile (crates/demosaic/src/rcd.rs:355-466)."""
+    m = np.ascontiguousarray(mosaic, dtype=np.float32)
+    out = np.zeros(9, dtype=np.float32)
+    lib().rhdo_rcd_probe(_p(m), _sz(m.shape[1]), _sz(m.shape[0]), ctypes.c_int(pattern),
+                         _sz(red), _sz(green), _p(out))
+    return out
+
+
 def label_components(mask: np.ndarray) -> tuple[np.ndarray, int]:
     """Compact raster-order 8-connected labels (crates/highlight/src/regions.rs:188-271)."""
     lab = np.ascontiguousarray(mask != 0, dtype=np.uint32)

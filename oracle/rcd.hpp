@@ -208,6 +208,40 @@ template <class T> void at_green(Scratch<T>& s, int pattern) {
     }
 }
 
+// Stage probes of crates/demosaic/src/rcd.rs:355-466: values after each
+// stage of the first tile at tile indices `red` and `green`.
+template <class T> void probe_first_tile(const Mosaic<T>& m, size_t red, size_t green, T out[9]) {
+    Scratch<T> s;
+    s.clear();
+    int pattern = m.pattern;
+    for (int row = 0; row < TS; ++row) {
+        size_t sy = mirror_coordinate((ptrdiff_t)row - BORDER, m.height);
+        int c0 = color_at(pattern, 0, row), c1 = color_at(pattern, 1, row);
+        for (int col = 0; col < TS; ++col) {
+            T v = m.at(mirror_coordinate((ptrdiff_t)col - BORDER, m.width), sy);
+            s.cfa[row * TS + col] = v;
+            s.rgb(c0)[row * TS + col] = v;
+            s.rgb(c1)[row * TS + col] = v;
+        }
+    }
+    find_directions(s);
+    out[0] = s.vh[red];
+    low_pass(s, pattern);
+    out[1] = s.pq[red / 2];
+    interpolate_green(s, pattern);
+    out[2] = s.rgb1[red];
+    diagonal_high_pass(s);
+    out[3] = s.p[green / 2];
+    out[4] = s.q[green / 2];
+    diagonal_directions(s, pattern);
+    out[5] = s.pq[red / 2];
+    opposite_color(s, pattern);
+    out[6] = s.rgb2[red];
+    at_green(s, pattern);
+    out[7] = s.rgb0[green];
+    out[8] = s.rgb2[green];
+}
+
 }  // namespace rcd
 
 // rcd.rs:36-113 with populate (:210-257) and write_tile (:261-299).

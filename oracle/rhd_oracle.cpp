@@ -127,6 +127,13 @@ int rhdo_opposed(float* mosaic, size_t width, size_t height, int pattern,
     return status;
 }
 
+int rhdo_rcd_probe(const float* mosaic, size_t width, size_t height, int pattern, size_t red,
+                   size_t green, float out[9]) {
+    Mosaic<float> m{mosaic, width, height, width, pattern};
+    rcd::probe_first_tile(m, red, green, out);
+    return 0;
+}
+
 int rhdo_label_components(uint32_t* labels, size_t width, size_t height, uint32_t* count) {
     *count = opposed::label_components(labels, width, height);
     return 0;

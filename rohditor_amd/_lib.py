@@ -1,0 +1,115 @@
+"""ctypes binding of the librhd C ABI (include/rhd.h).
+
+The library is loaded from this package directory only; a missing library is
+an error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int32, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "librhd.so"
+
+OK, CANCELLED, UNSUPPORTED, OVER_BUDGET, INVALID_DIMENSIONS, BASE_MISMATCH, \
+    UNSUPPORTED_EDITS, SYNCHRONIZATION, INVALID_INPUT, READBACK = range(10)
+STATUS_NAMES = ["Ok", "Cancelled", "Unsupported", "OverBudget", "InvalidDimensions",
+                "BaseMismatch", "UnsupportedEdits", "Synchronization", "InvalidInput",
+                "Readback"]
+
+
+class SensorDesc(ctypes.Structure):
+    _fields_ = [
+        ("sensor_width", c_uint32), ("sensor_height", c_uint32), ("row_stride", c_uint64),
+        ("sensor_pattern", c_int32), ("crop_x", c_uint32), ("crop_y", c_uint32),
+        ("crop_width", c_uint32), ("crop_height", c_uint32),
+        ("black_levels", POINTER(c_float)), ("black_repeat_width", c_uint32),
+        ("black_repeat_height", c_uint32), ("white_levels", POINTER(c_float)),
+        ("white_count", c_uint32), ("highlight", c_int32), ("clip_levels", c_float * 3),
+        ("detection_levels", c_float * 3), ("has_neutral_ceiling", c_int32),
+        ("neutral_ceiling", c_float * 3), ("demosaic", c_int32),
+        ("demosaic_balance", c_float * 3),
+    ]
+
+
+class DetailDesc(ctypes.Structure):
+    _fields_ = [
+        ("capture_enabled", c_int32), ("amount", c_float), ("sigma", c_float),
+        ("noise_protection", c_float), ("ceilings", c_float * 3), ("iterations", c_int32),
+        ("false_colour", c_int32), ("noise_reduction", c_int32),
+    ]
+
+
+class ClipStats(ctypes.Structure):
+    _fields_ = [("affected_sites", c_uint64), ("changed_sites", c_uint64),
+                ("nominal_over_white_sites", c_uint64), ("affected_by_channel", c_uint64 * 3)]
+
+
+class OpposedStats(ctypes.Structure):
+    _fields_ = [("suspected_clipped_sites", c_uint64), ("reconstructed_sites", c_uint64),
+                ("changed_sites", c_uint64), ("fallback_sites", c_uint64),
+                ("fully_unsupported_sites", c_uint64), ("neutralised_sites", c_uint64),
+                ("suspected_by_channel", c_uint64 * 3), ("regions", c_uint32)]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [
+        ("normalize_ms", c_float), ("highlight_ms", c_float), ("demosaic_ms", c_float),
+        ("capture_ms", c_float), ("total_ms", c_float), ("uploaded_bytes", c_uint64),
+        ("readback_bytes", c_uint64), ("estimated_gpu_bytes", c_uint64), ("launches", c_uint32),
+        ("highlight_method", c_int32), ("clip", ClipStats), ("opposed", OpposedStats),
+    ]
+
+
+# Every symbol include/rhd.h declares, with (restype, argtypes).
+_P = c_void_p
+SIGNATURES = {
+    "rhd_create": (c_int, [c_int, c_uint64, POINTER(_P)]),
+    "rhd_destroy": (None, [_P]),
+    "rhd_set_budget": (c_int, [_P, c_uint64]),
+    "rhd_reserved_bytes": (c_uint64, [_P]),
+    "rhd_last_error": (c_char_p, [_P]),
+    "rhd_status_name": (c_char_p, [c_int]),
+    "rhd_status_is_routing": (c_int, [c_int]),
+    "rhd_supports": (c_int, [POINTER(SensorDesc), POINTER(DetailDesc)]),
+    "rhd_mosaic_upload": (c_int, [_P, _P, c_uint64, c_uint32, c_uint32, POINTER(_P)]),
+    "rhd_mosaic_release": (None, [_P]),
+    "rhd_normalize": (c_int, [_P, _P, POINTER(SensorDesc), _P, POINTER(_P), POINTER(Metrics)]),
+    "rhd_highlight": (c_int, [_P, _P, POINTER(SensorDesc), _P, POINTER(Metrics)]),
+    "rhd_demosaic": (c_int, [_P, _P, POINTER(SensorDesc), POINTER(DetailDesc), _P, POINTER(_P),
+                             POINTER(Metrics)]),
+    "rhd_capture": (c_int, [_P, _P, POINTER(DetailDesc), _P, POINTER(Metrics)]),
+    "rhd_develop_resident": (c_int, [_P, _P, POINTER(SensorDesc), POINTER(DetailDesc), _P,
+                                     POINTER(_P), POINTER(Metrics)]),
+    "rhd_develop": (c_int, [_P, _P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc), _P,
+                            POINTER(_P), POINTER(Metrics)]),
+    "rhd_plane_read": (c_int, [_P, _P, _P, c_uint64]),
+    "rhd_plane_write": (c_int, [_P, _P, _P, c_uint64]),
+    "rhd_plane_upload": (c_int, [_P, _P, c_uint32, c_uint32, c_int32, POINTER(_P)]),
+    "rhd_plane_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
+    "rhd_plane_release": (None, [_P]),
+    "rhd_source_read_rgb": (c_int, [_P, _P, _P, c_uint64]),
+    "rhd_source_planes": (c_int, [_P, POINTER(_P), POINTER(_P), POINTER(_P)]),
+    "rhd_source_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32)]),
+    "rhd_source_release": (None, [_P]),
+    "rhd_synchronize": (c_int, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load librhd.so from the package directory, binding every declared symbol."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"librhd.so is not built at {LIB_PATH}; run `python -c 'import __graft_entry__ as g;"
+                " g.build()'` (the HIP library is required; there is no CPU fallback)")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = lib
+    return _lib

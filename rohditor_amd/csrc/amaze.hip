@@ -1,0 +1,822 @@
+// AMaZE demosaic on gfx950.
+//
+// Reference: crates/demosaic/src/amaze/{mod,math,green,nyquist,diagonal,
+// chroma}.rs with the tile geometry of crates/demosaic/src/amaze_geometry.rs.
+// The result is bit-identical to the reference: every stage keeps its loop
+// bounds, plane addressing and operation order, and the tile grid (224-pixel
+// cores, 16-pixel mirrored halo, 256-pixel tile planes) is kept because the
+// Nyquist bounding box and the three in-place chains are per tile.
+//
+// Layout: every tile owns one scratch block in HBM (11 full planes of
+// 256x256 f32, 14 half planes of 256x128 f32 indexed by pixel pair, 2 byte
+// planes). Green is stored only at red/blue sites (the GRB half plane); at
+// green sites it equals the CFA sample. Stages run as grid kernels over
+// (tile, row, pixel or pair); the three sequential chains run as
+//   - bound_v: one thread per tile column walking down the rows (vcd),
+//   - bound_h: one thread per tile row walking right over LDS-staged chunks (hcd),
+//   - chains:  one workgroup per tile, one thread per pair, a barrier per row
+//              (hvwt in populate_green and pmwt in combine_diagonals).
+// The finish kernel writes planar RGB for the tile core, with the measured
+// sample restored and the inverse channel balance applied (mod.rs:196-273).
+#include <climits>
+
+#include "rhd_internal.hpp"
+
+namespace rhd {
+namespace {
+
+constexpr int TS = 256, TSH = 128, BORDER = 16, VALID = TS - 2 * BORDER;
+constexpr int FULL = TS * TS, HALF = TS * TSH;
+
+enum FullPlane { CFA, DW0, DW1, DHV, VCD, HCD, VALT, HALT, DGV, DGH, CDS, NUM_FULL };
+enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, DELP, DELM, SQ1M, SQ1P, PMWT, RBM, RBP, RBINT,
+                 NUM_HALF };
+constexpr size_t TILE_FLOATS = (size_t)NUM_FULL * FULL + (size_t)NUM_HALF * HALF + 2 * HALF / 4;
+
+constexpr float EPS = 1.0e-5f, EPSSQ = 1.0e-10f, ARTHRESH = 0.75f;
+// math.rs:28-53; GAUSSGRAD carries the exact factor NYQTHRESH = 0.5.
+constexpr float GAUSSODD[4] = {0.14659727707323927f, 0.103592713382435f, 0.0732036125103057f,
+                               0.0365543548389495f};
+constexpr float GAUSSGRAD[6] = {0.5f * 0.07384411893421103f, 0.5f * 0.06207511968171489f,
+                                0.5f * 0.0521818194747806f,  0.5f * 0.03687419286733595f,
+                                0.5f * 0.03099732204057846f, 0.5f * 0.018413194161458882f};
+constexpr float GAUSSEVEN[2] = {0.13719494435797422f, 0.05640252782101291f};
+constexpr float GQUINC[4] = {0.169917f, 0.108947f, 0.069855f, 0.0287182f};
+
+struct Tile {
+    float* base;
+    int index;
+    __device__ __forceinline__ float* full(int p) const { return base + (size_t)p * FULL; }
+    __device__ __forceinline__ float* half(int p) const {
+        return base + (size_t)NUM_FULL * FULL + (size_t)p * HALF;
+    }
+    __device__ __forceinline__ uint8_t* nyq() const {
+        return reinterpret_cast<uint8_t*>(base + (size_t)NUM_FULL * FULL + (size_t)NUM_HALF * HALF);
+    }
+    __device__ __forceinline__ uint8_t* nyq2() const { return nyq() + HALF; }
+};
+
+__device__ __forceinline__ Tile tile_of(const AmazeArgs& a, int z) {
+    int index = a.tile_base + z;
+    return Tile{a.scratch + (size_t)z * TILE_FLOATS, index};
+}
+
+__device__ __forceinline__ float sqr(float x) { return x * x; }
+__device__ __forceinline__ float median3(float a, float b, float c) {
+    return sel_max(sel_min(a, b), sel_min(sel_max(b, a), c));
+}
+__device__ __forceinline__ float intp(float a, float b, float c) { return a * (b - c) + c; }
+__device__ __forceinline__ int fc(int pattern, int row, int col) { return color_at(pattern, col, row); }
+__device__ __forceinline__ int gae(int pattern, int row) { return fc(pattern, row, 2) & 1; }
+
+// ---------------------------------------------------------------- load
+// mod.rs:171-194 (populate) plus the zeroed state of scratch.rs:113-145.
+__global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
+    const int sx = mirror_whole(tx * VALID - BORDER + cc, a.width);
+    const int sy = mirror_whole(ty * VALID - BORDER + rr, a.height);
+    const int c = fc(a.pattern, rr, cc);
+    const float v = a.mosaic[(size_t)sy * a.width + sx] * a.forward[c];
+    const int i = rr * TS + cc;
+    t.full(CFA)[i] = v;
+    if ((c & 1) == 0) {
+        const int p = i >> 1;
+        t.half(GRB)[p] = v;
+        t.half(D2H)[p] = 0.0f;
+        t.half(D2V)[p] = 0.0f;
+        t.nyq()[p] = 0;
+        t.nyq2()[p] = 0;
+    }
+    // vcd/hcd and their alternatives are read up to two rows and columns
+    // outside the region the stages write; those entries are zero.
+    if (rr < 4 || rr >= TS - 4 || cc < 4 || cc >= TS - 4) {
+        t.full(VCD)[i] = 0.0f;
+        t.full(HCD)[i] = 0.0f;
+        t.full(VALT)[i] = 0.0f;
+        t.full(HALT)[i] = 0.0f;
+    }
+    if (rr == 0 && cc == 0) {
+        int* area = a.areas + 4 * (size_t)blockIdx.z;
+        area[0] = INT_MAX;
+        area[1] = -1;
+        area[2] = INT_MAX;
+        area[3] = -1;
+    }
+}
+
+// ---------------------------------------------------------------- gradients
+// green.rs:13-31 and diagonal.rs:15-42.
+__global__ __launch_bounds__(256) void amz_gradients(AmazeArgs a) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const float* cfa = t.full(CFA);
+    if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
+        const int i = rr * TS + cc;
+        float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
+        float delv = fabsf(cfa[i + TS] - cfa[i - TS]);
+        t.full(DW0)[i] = EPS + fabsf(cfa[i + 2 * TS] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2 * TS]) + delv;
+        t.full(DW1)[i] = EPS + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
+        t.full(DHV)[i] = sqr(delh) + sqr(delv);
+    }
+    if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6 && (cc & 1) == 0) {
+        const float* up = cfa + (rr - 1) * TS;
+        const float* mid = cfa + rr * TS;
+        const float* down = cfa + (rr + 1) * TS;
+        const int p = (rr * TS + cc) >> 1;
+        float delp, delm, sqp, sqm;
+        if (gae(a.pattern, rr) == 0) {
+            delp = fabsf(up[cc + 1] - down[cc - 1]);
+            delm = fabsf(down[cc + 1] - up[cc - 1]);
+            float g = mid[cc + 1];
+            sqp = sqr(g - down[cc]) + sqr(g - up[cc + 2]);
+            sqm = sqr(g - up[cc]) + sqr(g - down[cc + 2]);
+        } else {
+            float g = mid[cc];
+            sqp = sqr(g - down[cc - 1]) + sqr(g - up[cc + 1]);
+            sqm = sqr(g - up[cc - 1]) + sqr(g - down[cc + 1]);
+            delp = fabsf(up[cc + 2] - down[cc]);
+            delm = fabsf(down[cc + 2] - up[cc]);
+        }
+        t.half(DELP)[p] = delp;
+        t.half(DELM)[p] = delm;
+        t.half(SQ1P)[p] = sqp;
+        t.half(SQ1M)[p] = sqm;
+    }
+}
+
+// ---------------------------------------------------------------- colour differences
+__device__ __forceinline__ float adaptive_or(float ratio, float centre, float fallback) {
+    return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : fallback;
+}
+
+// green.rs:46-137
+__global__ __launch_bounds__(256) void amz_colour_differences(AmazeArgs a, float clip_pt8) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (rr < 4 || rr >= TS - 4 || cc < 4 || cc >= TS - 4) return;
+    const float* cfa = t.full(CFA);
+    const float* w0 = t.full(DW0);
+    const float* w1 = t.full(DW1);
+    const int i = rr * TS + cc;
+    const bool fcswitch = (fc(a.pattern, rr, cc) & 1) == 1;  // green site
+    float cur = cfa[i];
+    float up = cfa[i - TS], up2 = cfa[i - 2 * TS], down = cfa[i + TS], down2 = cfa[i + 2 * TS];
+    float left = cfa[i - 1], left2 = cfa[i - 2], right = cfa[i + 1], right2 = cfa[i + 2];
+    float w0u2 = w0[i - 2 * TS], w0u1 = w0[i - TS], w0c = w0[i], w0d1 = w0[i + TS],
+          w0d2 = w0[i + 2 * TS];
+    float w1l2 = w1[i - 2], w1l1 = w1[i - 1], w1c = w1[i], w1r1 = w1[i + 1], w1r2 = w1[i + 2];
+    float cru = up * (w0u2 + w0c) / (w0u2 * (EPS + cur) + w0c * (EPS + up2));
+    float crd = down * (w0d2 + w0c) / (w0d2 * (EPS + cur) + w0c * (EPS + down2));
+    float crl = left * (w1l2 + w1c) / (w1l2 * (EPS + cur) + w1c * (EPS + left2));
+    float crr = right * (w1r2 + w1c) / (w1r2 * (EPS + cur) + w1c * (EPS + right2));
+    float guha = up + 0.5f * (cur - up2);
+    float gdha = down + 0.5f * (cur - down2);
+    float glha = left + 0.5f * (cur - left2);
+    float grha = right + 0.5f * (cur - right2);
+    float guar = adaptive_or(cru, cur, guha);
+    float gdar = adaptive_or(crd, cur, gdha);
+    float glar = adaptive_or(crl, cur, glha);
+    float grar = adaptive_or(crr, cur, grha);
+    float hwt = w1l1 / (w1l1 + w1r1);
+    float vwt = w0u1 / (w0d1 + w0u1);
+    float gintvha = vwt * gdha + (1.0f - vwt) * guha;
+    float ginthha = hwt * grha + (1.0f - hwt) * glha;
+    float vcd, hcd, vcdalt, hcdalt;
+    if (fcswitch) {
+        vcd = cur - (vwt * gdar + (1.0f - vwt) * guar);
+        hcd = cur - (hwt * grar + (1.0f - hwt) * glar);
+        vcdalt = cur - gintvha;
+        hcdalt = cur - ginthha;
+    } else {
+        vcd = (vwt * gdar + (1.0f - vwt) * guar) - cur;
+        hcd = (hwt * grar + (1.0f - hwt) * glar) - cur;
+        vcdalt = gintvha - cur;
+        hcdalt = ginthha - cur;
+    }
+    bool clipped = (cur > clip_pt8) | (gintvha > clip_pt8) | (ginthha > clip_pt8);
+    if (clipped) {
+        guar = guha;
+        gdar = gdha;
+        glar = glha;
+        grar = grha;
+        vcd = vcdalt;
+        hcd = hcdalt;
+    }
+    t.full(DGV)[i] = sel_min(sqr(guha - gdha), sqr(guar - gdar));
+    t.full(DGH)[i] = sel_min(sqr(glha - grha), sqr(glar - grar));
+    t.full(VCD)[i] = vcd;
+    t.full(HCD)[i] = hcd;
+    t.full(VALT)[i] = vcdalt;
+    t.full(HALT)[i] = hcdalt;
+}
+
+// ---------------------------------------------------------------- bound differences
+// green.rs:142-174
+__device__ __forceinline__ float bound_at_green(float d, float cur, float a, float b, float clip) {
+    float g = -d + cur;
+    float alt = -median3(g, a, b) + cur;
+    float bounded = d;
+    if (d > 0.0f) {
+        if (3.0f * d > g + cur) {
+            bounded = alt;
+        } else {
+            float weight = 1.0f - 3.0f * d / (EPS + g + cur);
+            bounded = weight * d + (1.0f - weight) * alt;
+        }
+    }
+    return g > clip ? alt : bounded;
+}
+__device__ __forceinline__ float bound_at_red_blue(float d, float cur, float a, float b,
+                                                   float clip) {
+    float g = d + cur;
+    float alt = median3(g, a, b) - cur;
+    float bounded = d;
+    if (d < 0.0f) {
+        if (3.0f * d < -(g + cur)) {
+            bounded = alt;
+        } else {
+            float weight = 1.0f + 3.0f * d / (EPS + g + cur);
+            bounded = weight * d + (1.0f - weight) * alt;
+        }
+    }
+    return g > clip ? alt : bounded;
+}
+__device__ __forceinline__ float spread(float a, float b, float c) {
+    return 3.0f * (sqr(a) + sqr(b) + sqr(c)) - sqr(a + b + c);
+}
+
+// green.rs:184-226, vertical half: vcd is bounded in place going down each
+// column; the value two rows up is already bounded when read.
+__global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int cc = blockIdx.x * 64 + threadIdx.x;
+    if (cc < 4 || cc >= TS - 4) return;
+    float* vcd = t.full(VCD);
+    const float* alt = t.full(VALT);
+    const float* cfa = t.full(CFA);
+    float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded vcd two and one rows up (rows 2, 3 are zero)
+    float a_m2 = 0.0f, a_m1 = 0.0f;  // vcdalt two and one rows up
+    float u0 = vcd[4 * TS + cc], u1 = vcd[5 * TS + cc];  // unbounded vcd at rr, rr + 1
+    float a0 = alt[4 * TS + cc], a1 = alt[5 * TS + cc];
+    bool green = (fc(a.pattern, 4, cc) & 1) == 1;
+    for (int rr = 4; rr < TS - 4; ++rr) {
+        const int i = rr * TS + cc;
+        float u2 = rr + 2 < TS - 4 ? vcd[i + 2 * TS] : 0.0f;
+        float a2 = rr + 2 < TS - 4 ? alt[i + 2 * TS] : 0.0f;
+        float vcdvar = spread(b_m2, u0, u2);
+        float vcdaltvar = spread(a_m2, a0, a2);
+        float v = vcdaltvar < vcdvar ? a0 : u0;
+        float cur = cfa[i];
+        float bounded = green ? bound_at_green(v, cur, cfa[i - TS], cfa[i + TS], clip_pt)
+                              : bound_at_red_blue(v, cur, cfa[i - TS], cfa[i + TS], clip_pt);
+        vcd[i] = bounded;
+        b_m2 = b_m1;
+        b_m1 = bounded;
+        a_m2 = a_m1;
+        a_m1 = a0;
+        u0 = u1;
+        u1 = u2;
+        a0 = a1;
+        a1 = a2;
+        green = !green;
+    }
+}
+
+// green.rs:184-226, horizontal half: hcd is bounded in place along each row,
+// and cddiffsq is formed at red/blue sites from the final vcd and hcd. One
+// thread per tile row; 32-column chunks are staged through LDS so global
+// reads and writes stay coalesced.
+constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_W = BH_CHUNK + 4, BH_PITCH = BH_W + 1;
+__global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int r0 = 4 + blockIdx.x * BH_ROWS;
+    const int lane = threadIdx.x;
+    const int rr = r0 + lane;
+    __shared__ float s_hcd[BH_ROWS * BH_PITCH];
+    __shared__ float s_alt[BH_ROWS * BH_PITCH];
+    __shared__ float s_cfa[BH_ROWS * BH_PITCH];
+    __shared__ float s_out[BH_ROWS * BH_PITCH];
+    __shared__ float s_cds[BH_ROWS * BH_PITCH];
+    float* hcd = t.full(HCD);
+    const float* alt = t.full(HALT);
+    const float* cfa = t.full(CFA);
+    const float* vcd = t.full(VCD);
+    float* cds = t.full(CDS);
+    const int rows = min(BH_ROWS, TS - 4 - r0);
+    float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded hcd two and one columns left (cols 2, 3 are zero)
+    bool green = rr < TS - 4 ? (fc(a.pattern, rr, 4) & 1) == 1 : false;
+    for (int c0 = 4; c0 < TS - 4; c0 += BH_CHUNK) {
+        // Stage columns c0 - 2 .. c0 + 33 (hcd, hcdalt, cfa) for `rows` rows.
+        for (int k = lane; k < rows * BH_W; k += 64) {
+            int r = k / BH_W, c = k % BH_W;
+            int col = c0 - 2 + c;
+            size_t g = (size_t)(r0 + r) * TS + col;
+            bool inside = col >= 4 && col < TS - 4;
+            s_hcd[r * BH_PITCH + c] = inside ? hcd[g] : 0.0f;
+            s_alt[r * BH_PITCH + c] = inside ? alt[g] : 0.0f;
+            s_cfa[r * BH_PITCH + c] = cfa[g];
+        }
+        __syncthreads();
+        if (lane < rows) {
+            const int base = lane * BH_PITCH + 2;  // LDS column of c0
+            for (int k = 0; k < BH_CHUNK; ++k) {
+                const int cc = c0 + k;
+                if (cc >= TS - 4) break;
+                const int j = base + k;
+                float hcdvar = spread(b_m2, s_hcd[j], s_hcd[j + 2]);
+                float hcdaltvar = spread(s_alt[j - 2], s_alt[j], s_alt[j + 2]);
+                float h = hcdaltvar < hcdvar ? s_alt[j] : s_hcd[j];
+                float cur = s_cfa[j];
+                float bounded = green ? bound_at_green(h, cur, s_cfa[j - 1], s_cfa[j + 1], clip_pt)
+                                      : bound_at_red_blue(h, cur, s_cfa[j - 1], s_cfa[j + 1], clip_pt);
+                s_out[lane * BH_PITCH + k] = bounded;
+                b_m2 = b_m1;
+                b_m1 = bounded;
+                green = !green;
+            }
+        }
+        __syncthreads();
+        // Write the chunk back coalesced; cddiffsq at red/blue sites.
+        for (int k = lane; k < rows * BH_CHUNK; k += 64) {
+            int r = k / BH_CHUNK, c = k % BH_CHUNK;
+            int col = c0 + c;
+            if (col >= TS - 4) continue;
+            int row = r0 + r;
+            size_t g = (size_t)row * TS + col;
+            float h = s_out[r * BH_PITCH + c];
+            hcd[g] = h;
+            if ((fc(a.pattern, row, col) & 1) == 0) cds[g] = sqr(vcd[g] - h);
+        }
+        __syncthreads();
+    }
+    (void)s_cds;
+}
+
+// ---------------------------------------------------------------- weights, Nyquist test, diagonals
+// green.rs:229-292 (direction_weights), nyquist.rs:25-89 (test and bounding
+// box), diagonal.rs:98-144 (diagonal_estimates). One thread per red/blue
+// site: cc = 2 * pair + green_at_even(row).
+__device__ __forceinline__ float even_kernel(const float* p, int rr, int cc) {
+    const float* up2 = p + (rr - 2) * TSH;
+    const float* up1 = p + (rr - 1) * TSH;
+    const float* mid = p + rr * TSH;
+    const float* down1 = p + (rr + 1) * TSH;
+    const float* down2 = p + (rr + 2) * TSH;
+    return GAUSSEVEN[0] * (up1[cc >> 1] + mid[(cc - 1) >> 1] + mid[(cc + 1) >> 1] + down1[cc >> 1]) +
+           GAUSSEVEN[1] * (up2[(cc - 1) >> 1] + up2[(cc + 1) >> 1] + up1[(cc - 2) >> 1] +
+                           up1[(cc + 2) >> 1] + down1[(cc - 2) >> 1] + down1[(cc + 2) >> 1] +
+                           down2[(cc - 1) >> 1] + down2[(cc + 1) >> 1]);
+}
+__device__ __forceinline__ float diagonal_estimate(float centre, float near_, float far_) {
+    float ratio = 2.0f * near_ / (EPS + centre + far_);
+    return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : near_ + 0.5f * (centre - far_);
+}
+// diagonal.rs:79-94 and :184-199 (bound_diagonal, bound_green).
+__device__ __forceinline__ float bound_towards(float estimate, float centre, float a, float b,
+                                               float clip) {
+    float value = estimate;
+    if (value < centre) {
+        if (2.0f * value < centre) {
+            value = median3(value, a, b);
+        } else {
+            float weight = 2.0f * (centre - value) / (EPS + value + centre);
+            value = weight * value + (1.0f - weight) * median3(value, a, b);
+        }
+    }
+    return value > clip ? median3(value, a, b) : value;
+}
+
+__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int cc = 2 * p + gae(a.pattern, rr);
+    const int i = rr * TS + cc;
+    if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
+        const float* vcd = t.full(VCD);
+        const float* hcd = t.full(HCD);
+        const float* dgv = t.full(DGV);
+        const float* dgh = t.full(DGH);
+        const float* w0 = t.full(DW0);
+        const float* w1 = t.full(DW1);
+        float v[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
+        float uave = v[3] + v[2] + v[1] + v[0];
+        float dave = v[3] + v[4] + v[5] + v[6];
+        float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
+        float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
+        float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
+        float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
+        float varl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) +
+                     sqr(hcd[i - 3] - lave);
+        float varr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) +
+                     sqr(hcd[i + 3] - rave);
+        float hwt = w1[i - 1] / (w1[i - 1] + w1[i + 1]);
+        float vwt = w0[i - TS] / (w0[i + TS] + w0[i - TS]);
+        float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
+        float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
+        float fluct_u = dgv[i] + dgv[i - TS] + dgv[i - 2 * TS];
+        float fluct_d = dgv[i] + dgv[i + TS] + dgv[i + 2 * TS];
+        float fluct_l = dgh[i] + dgh[i - 1] + dgh[i - 2];
+        float fluct_r = dgh[i] + dgh[i + 1] + dgh[i + 2];
+        float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
+        float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
+        float varwt = hcdvar / (vcdvar + hcdvar);
+        float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
+        t.half(HVWT)[i >> 1] =
+            ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
+                ? varwt
+                : diffwt;
+
+        // Nyquist test statistic (nyquist.rs:28-57) and bounding box.
+        const float* c = t.full(CDS) + i;
+        const float* d = t.full(DHV) + i;
+        float test =
+            (GAUSSODD[0] * c[0] +
+             GAUSSODD[1] * (c[-TS - 1] + c[-TS + 1] + c[TS - 1] + c[TS + 1]) +
+             GAUSSODD[2] * (c[-2 * TS] + c[-2] + c[2] + c[2 * TS]) +
+             GAUSSODD[3] * (c[-2 * TS - 2] + c[-2 * TS + 2] + c[2 * TS - 2] + c[2 * TS + 2])) -
+            (GAUSSGRAD[0] * d[0] + GAUSSGRAD[1] * (d[-TS] + d[1] + d[-1] + d[TS]) +
+             GAUSSGRAD[2] * (d[-TS - 1] + d[-TS + 1] + d[TS - 1] + d[TS + 1]) +
+             GAUSSGRAD[3] * (d[-2 * TS] + d[-2] + d[2] + d[2 * TS]) +
+             GAUSSGRAD[4] * (d[-2 * TS - 1] + d[-2 * TS + 1] + d[-TS - 2] + d[-TS + 2] +
+                             d[TS - 2] + d[TS + 2] + d[2 * TS - 1] + d[2 * TS + 1]) +
+             GAUSSGRAD[5] * (d[-2 * TS - 2] + d[-2 * TS + 2] + d[2 * TS - 2] + d[2 * TS + 2]));
+        if (test > 0.0f) {
+            t.nyq()[i >> 1] = 1;
+            int* area = a.areas + 4 * (size_t)blockIdx.z;
+            atomicMin(area + 0, rr);
+            atomicMax(area + 1, rr);
+            atomicMin(area + 2, cc);
+            atomicMax(area + 3, cc);
+        }
+    }
+    if (rr >= 8 && rr < TS - 8 && cc >= 8 && cc < TS - 8) {
+        const float* cfa = t.full(CFA);
+        const int i1 = cc >> 1;
+        float centre = cfa[i];
+        float se1 = cfa[i + TS + 1], se2 = cfa[i + 2 * TS + 2];
+        float nw1 = cfa[i - TS - 1], nw2 = cfa[i - 2 * TS - 2];
+        float ne1 = cfa[i - TS + 1], ne2 = cfa[i - 2 * TS + 2];
+        float sw1 = cfa[i + TS - 1], sw2 = cfa[i + 2 * TS - 2];
+        float rbse = diagonal_estimate(centre, se1, se2);
+        float rbnw = diagonal_estimate(centre, nw1, nw2);
+        float rbne = diagonal_estimate(centre, ne1, ne2);
+        float rbsw = diagonal_estimate(centre, sw1, sw2);
+        const float* delm = t.half(DELM);
+        const float* delp = t.half(DELP);
+        const int h0 = rr * TSH;
+        float wtse = EPS + delm[h0 + i1] + delm[h0 + TSH + ((cc + 1) >> 1)] +
+                     delm[h0 + 2 * TSH + ((cc + 2) >> 1)];
+        float wtnw = EPS + delm[h0 + i1] + delm[h0 - TSH + ((cc - 1) >> 1)] +
+                     delm[h0 - 2 * TSH + ((cc - 2) >> 1)];
+        float wtne = EPS + delp[h0 + i1] + delp[h0 - TSH + ((cc + 1) >> 1)] +
+                     delp[h0 - 2 * TSH + ((cc + 2) >> 1)];
+        float wtsw = EPS + delp[h0 + i1] + delp[h0 + TSH + ((cc - 1) >> 1)] +
+                     delp[h0 + 2 * TSH + ((cc - 2) >> 1)];
+        float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
+        float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
+        float rbvarm = EPSSQ + even_kernel(t.half(SQ1M), rr, cc);
+        t.half(PMWT)[h0 + i1] = rbvarm / ((EPSSQ + even_kernel(t.half(SQ1P), rr, cc)) + rbvarm);
+        t.half(RBP)[h0 + i1] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
+        t.half(RBM)[h0 + i1] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
+    }
+}
+
+// Nyquist area of a tile (nyquist.rs:59-89); false when there is none.
+struct Area {
+    int start_row, end_row, start_col, end_col;
+};
+__device__ __forceinline__ bool tile_area(const AmazeArgs& a, int z, Area& out) {
+    const int* b = a.areas + 4 * (size_t)z;
+    int start_row = b[0], end_row = b[1], start_col = b[2], end_col = b[3];
+    if (end_row < 0) return false;  // nothing flagged: the reference's (0, 0) rows
+    if (start_row == end_row || start_col == end_col) return false;
+    end_row += 1;
+    end_col += 1;
+    start_col -= start_col & 1;
+    out.start_row = max(start_row, 8);
+    out.end_row = min(end_row, TS - 8);
+    out.start_col = max(start_col, 8);
+    out.end_col = min(end_col, TS - 8);
+    return true;
+}
+__device__ __forceinline__ bool in_area(const Area& ar, int pattern, int rr, int cc) {
+    return rr >= ar.start_row && rr < ar.end_row && cc >= ar.start_col + gae(pattern, rr) &&
+           cc < ar.end_col;
+}
+
+// nyquist.rs:100-118: neighbourhood vote.
+__global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
+    Area ar;
+    if (!tile_area(a, blockIdx.z, ar)) return;
+    const Tile t = tile_of(a, blockIdx.z);
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int cc = 2 * p + gae(a.pattern, rr);
+    if (!in_area(ar, a.pattern, rr, cc)) return;
+    const int i = rr * TS + cc;
+    const uint8_t* n = t.nyq();
+    unsigned votes = n[(i - 2 * TS) >> 1] + n[(i - TS - 1) >> 1] + n[(i - TS + 1) >> 1] +
+                     n[(i - 2) >> 1] + n[(i + 2) >> 1] + n[(i + TS - 1) >> 1] +
+                     n[(i + TS + 1) >> 1] + n[(i + 2 * TS) >> 1];
+    t.nyq2()[i >> 1] = votes >= 5 ? 1 : (votes == 4 ? n[i >> 1] : 0);
+}
+
+// nyquist.rs:120-152: area interpolation of hvwt.
+__global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
+    Area ar;
+    if (!tile_area(a, blockIdx.z, ar)) return;
+    const Tile t = tile_of(a, blockIdx.z);
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int cc = 2 * p + gae(a.pattern, rr);
+    if (!in_area(ar, a.pattern, rr, cc)) return;
+    const int i = rr * TS + cc;
+    const uint8_t* n2 = t.nyq2();
+    if (n2[i >> 1] == 0) return;
+    const float* cfa = t.full(CFA);
+    float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
+    for (int di = -6; di < 7; di += 2) {
+        int i1 = i + di * TS - 6;
+        for (int dj = -6; dj < 7; dj += 2) {
+            if (n2[i1 >> 1] != 0) {
+                float c = cfa[i1];
+                sumcfa += c;
+                sumh += cfa[i1 - 1] + cfa[i1 + 1];
+                sumv += cfa[i1 - TS] + cfa[i1 + TS];
+                sumsqh += sqr(c - cfa[i1 - 1]) + sqr(c - cfa[i1 + 1]);
+                sumsqv += sqr(c - cfa[i1 - TS]) + sqr(c - cfa[i1 + TS]);
+                areawt += 1.0f;
+            }
+            i1 += 2;
+        }
+    }
+    sumh = sumcfa - 0.5f * sumh;
+    sumv = sumcfa - 0.5f * sumv;
+    areawt *= 0.5f;
+    float hcdvar = EPSSQ + fabsf(areawt * sumsqh - sumh * sumh);
+    float vcdvar = EPSSQ + fabsf(areawt * sumsqv - sumv * sumv);
+    t.half(HVWT)[i >> 1] = hcdvar / (vcdvar + hcdvar);
+}
+
+// ---------------------------------------------------------------- row chains
+// nyquist.rs:159-201 (populate_green: hvwt) and diagonal.rs:149-168
+// (combine_diagonals: pmwt), both updated in place row after row. One
+// workgroup per tile; thread p owns pair p; the previous row's final weights
+// live in LDS.
+__global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
+    const Tile t = tile_of(a, blockIdx.x);
+    const int p = threadIdx.x;
+    __shared__ float hv_prev[TSH];
+    __shared__ float pm_prev[TSH];
+    const float* cfa = t.full(CFA);
+    const float* vcd = t.full(VCD);
+    const float* hcd = t.full(HCD);
+    const uint8_t* nyq2 = t.nyq2();
+    float* hvwt = t.half(HVWT);
+    float* pmwt = t.half(PMWT);
+    float* dgrb0 = t.half(DGRB0);
+    float* grb = t.half(GRB);
+    float* d2h = t.half(D2H);
+    float* d2v = t.half(D2V);
+    const float* rbm = t.half(RBM);
+    const float* rbp = t.half(RBP);
+    float* rbint = t.half(RBINT);
+    hv_prev[p] = hvwt[7 * TSH + p];
+    pm_prev[p] = pmwt[9 * TSH + p];
+    __syncthreads();
+    for (int rr = 8; rr < TS - 8; ++rr) {
+        const int cc = 2 * p + gae(a.pattern, rr);
+        const int h = rr * TSH + p;
+        float hv = hvwt[h];
+        if (cc >= 8 && cc < TS - 8) {
+            float alt = (hv_prev[(cc - 1) >> 1] + hv_prev[(cc + 1) >> 1] +
+                         hvwt[h + TSH - p + ((cc - 1) >> 1)] + hvwt[h + TSH - p + ((cc + 1) >> 1)]) /
+                        4.0f;
+            if (fabsf(0.5f - hv) < fabsf(0.5f - alt)) {
+                hv = alt;
+                hvwt[h] = hv;
+            }
+            const int i = rr * TS + cc;
+            float dg = intp(hv, vcd[i], hcd[i]);
+            float green = cfa[i] + dg;
+            dgrb0[h] = dg;
+            grb[h] = green;
+            bool flagged = nyq2[h] != 0;
+            d2h[h] = flagged ? sqr(green - 0.5f * (cfa[i - 1] + cfa[i + 1])) : 0.0f;
+            d2v[h] = flagged ? sqr(green - 0.5f * (cfa[i - TS] + cfa[i + TS])) : 0.0f;
+        }
+        float pm = 0.0f;
+        const bool pm_row = rr >= 10 && rr < TS - 10;
+        if (pm_row) {
+            pm = pmwt[h];
+            if (cc >= 10 && cc < TS - 10) {
+                float alt = (pm_prev[(cc - 1) >> 1] + pm_prev[(cc + 1) >> 1] +
+                             pmwt[h + TSH - p + ((cc - 1) >> 1)] +
+                             pmwt[h + TSH - p + ((cc + 1) >> 1)]) /
+                            4.0f;
+                if (fabsf(0.5f - pm) < fabsf(0.5f - alt)) {
+                    pm = alt;
+                    pmwt[h] = pm;
+                }
+                const int i = rr * TS + cc;
+                rbint[h] = 0.5f * (cfa[i] + rbm[h] * (1.0f - pm) + rbp[h] * pm);
+            }
+        }
+        __syncthreads();
+        hv_prev[p] = hv;
+        if (pm_row) pm_prev[p] = pm;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- refinements
+// nyquist.rs:204-236 (refine_green), diagonal.rs:193-247
+// (refine_green_with_rb) and the G-B split of chroma.rs:26-32, in that order,
+// each on the thread's own red/blue site.
+__device__ __forceinline__ float green_from_rb(float sample, float centre, float other) {
+    float ratio = sample * 2.0f / (EPS + centre + other);
+    return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : sample + 0.5f * (centre - other);
+}
+__device__ __forceinline__ void red_offset(int pattern, int& ey, int& ex) {
+    int c00 = color_at(pattern, 0, 0), c10 = color_at(pattern, 1, 0);
+    if (c00 == 1 && c10 == 0) { ey = 0; ex = 1; }
+    else if (c00 == 1) { ey = 1; ex = 0; }
+    else if (c00 == 0) { ey = 0; ex = 0; }
+    else { ey = 1; ex = 1; }
+}
+
+__global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int cc = 2 * p + gae(a.pattern, rr);
+    const int i = rr * TS + cc;
+    const int h = i >> 1;
+    const float* cfa = t.full(CFA);
+    float* dgrb0 = t.half(DGRB0);
+    float* grb = t.half(GRB);
+    Area ar;
+    if (tile_area(a, blockIdx.z, ar) && in_area(ar, a.pattern, rr, cc) && t.nyq2()[h] != 0) {
+        auto average = [&](const float* pl) {
+            return EPSSQ + (GQUINC[0] * pl[h] +
+                            GQUINC[1] * (pl[(i - TS - 1) >> 1] + pl[(i - TS + 1) >> 1] +
+                                         pl[(i + TS - 1) >> 1] + pl[(i + TS + 1) >> 1]) +
+                            GQUINC[2] * (pl[(i - 2 * TS) >> 1] + pl[(i - 2) >> 1] +
+                                         pl[(i + 2) >> 1] + pl[(i + 2 * TS) >> 1]) +
+                            GQUINC[3] * (pl[(i - 2 * TS - 2) >> 1] + pl[(i - 2 * TS + 2) >> 1] +
+                                         pl[(i + 2 * TS - 2) >> 1] + pl[(i + 2 * TS + 2) >> 1]));
+        };
+        float gvarh = average(t.half(D2H));
+        float gvarv = average(t.half(D2V));
+        float dg = (t.full(HCD)[i] * gvarv + t.full(VCD)[i] * gvarh) / (gvarv + gvarh);
+        dgrb0[h] = dg;
+        grb[h] = cfa[i] + dg;
+    }
+    if (rr >= 12 && rr < TS - 12 && cc >= 12 && cc < TS - 12) {
+        const float* pm = t.half(PMWT);
+        const float* hv = t.half(HVWT);
+        if (!(fabsf(0.5f - pm[h]) < fabsf(0.5f - hv[h]))) {
+            const float* rb = t.half(RBINT);
+            const float* w0 = t.full(DW0);
+            const float* w1 = t.full(DW1);
+            float centre = rb[h];
+            float gu = green_from_rb(cfa[i - TS], centre, rb[h - 2 * TSH]);
+            float gd = green_from_rb(cfa[i + TS], centre, rb[h + 2 * TSH]);
+            float gl = green_from_rb(cfa[i - 1], centre, rb[h - 1]);
+            float gr = green_from_rb(cfa[i + 1], centre, rb[h + 1]);
+            float gintv = (w0[i - TS] * gd + w0[i + TS] * gu) / (w0[i + TS] + w0[i - TS]);
+            float ginth = (w1[i - 1] * gr + w1[i + 1] * gl) / (w1[i - 1] + w1[i + 1]);
+            gintv = bound_towards(gintv, centre, cfa[i - TS], cfa[i + TS], clip_pt);
+            ginth = bound_towards(ginth, centre, cfa[i - 1], cfa[i + 1], clip_pt);
+            float green = ginth * (1.0f - hv[h]) + gintv * hv[h];
+            grb[h] = green;
+            dgrb0[h] = green - cfa[i];
+        }
+    }
+    // Split G - B out of the combined colour difference on blue rows.
+    int ey, ex;
+    red_offset(a.pattern, ey, ex);
+    if (rr >= 13 - ey && rr < TS - 12 && ((rr - (13 - ey)) & 1) == 0 && p >= ((13 - ex) >> 1) &&
+        p < ((TS - 12) >> 1)) {
+        t.half(DGRB1)[h] = dgrb0[h];
+        dgrb0[h] = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------- chroma
+// chroma.rs:34-96
+__global__ __launch_bounds__(256) void amz_chroma(AmazeArgs a) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (rr < 14 || rr >= TS - 14) return;
+    const int start = 14 + gae(a.pattern, rr);
+    const int cc = 2 * p + gae(a.pattern, rr);
+    if (cc < start || cc >= TS - 14) return;
+    const int plane = 1 - fc(a.pattern, rr, start) / 2;
+    float* pl = t.half(plane ? DGRB1 : DGRB0);
+    auto d = [&](int dr, int dc) { return pl[(rr + dr) * TSH + ((cc + dc) >> 1)]; };
+    float wtnw = 1.0f / (EPS + fabsf(d(-1, -1) - d(1, 1)) + fabsf(d(-1, -1) - d(-3, -3)) +
+                         fabsf(d(1, 1) - d(-3, -3)));
+    float wtne = 1.0f / (EPS + fabsf(d(-1, 1) - d(1, -1)) + fabsf(d(-1, 1) - d(-3, 3)) +
+                         fabsf(d(1, -1) - d(-3, 3)));
+    float wtsw = 1.0f / (EPS + fabsf(d(1, -1) - d(-1, 1)) + fabsf(d(1, -1) - d(3, 3)) +
+                         fabsf(d(-1, 1) - d(3, -3)));
+    float wtse = 1.0f / (EPS + fabsf(d(1, 1) - d(-1, -1)) + fabsf(d(1, 1) - d(3, -3)) +
+                         fabsf(d(-1, -1) - d(3, 3)));
+    float value = (wtnw * (1.325f * d(-1, -1) - 0.175f * d(-3, -3) - 0.075f * d(-1, -3) -
+                           0.075f * d(-3, -1)) +
+                   wtne * (1.325f * d(-1, 1) - 0.175f * d(-3, 3) - 0.075f * d(-1, 3) -
+                           0.075f * d(1, 1)) +
+                   wtsw * (1.325f * d(1, -1) - 0.175f * d(3, -3) - 0.075f * d(1, -3) -
+                           0.075f * d(-1, -1)) +
+                   wtse * (1.325f * d(1, 1) - 0.175f * d(3, 3) - 0.075f * d(1, 3) -
+                           0.075f * d(3, 1))) /
+                  (wtnw + wtne + wtsw + wtse);
+    pl[rr * TSH + (cc >> 1)] = value;
+}
+
+// ---------------------------------------------------------------- finish + write
+// chroma.rs:103-147 (finish_planes) fused with mod.rs:196-273 (write_tile).
+__global__ __launch_bounds__(256) void amz_finish(AmazeArgs a) {
+    const Tile t = tile_of(a, blockIdx.z);
+    const int cc = BORDER + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rr = BORDER + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (cc >= TS - BORDER || rr >= TS - BORDER) return;
+    const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
+    const int gx = tx * VALID + cc - BORDER, gy = ty * VALID + rr - BORDER;
+    if (gx >= a.width || gy >= a.height) return;
+    const float* cfa = t.full(CFA);
+    const float* hv = t.half(HVWT);
+    const float* g_r = t.half(DGRB0);
+    const float* g_b = t.half(DGRB1);
+    const int i = rr * TS + cc;
+    const int ch = fc(a.pattern, rr, cc);
+    const float sample = cfa[i];
+    float green, red, blue;
+    if (ch == 1) {
+        green = sample;
+        const int r0 = rr * TSH;
+        float up = hv[r0 - TSH + (cc >> 1)], down = hv[r0 + TSH + (cc >> 1)];
+        float left = hv[r0 + ((cc - 1) >> 1)], right = hv[r0 + ((cc + 1) >> 1)];
+        float temp = 1.0f / (up + 2.0f - right - left + down);
+        float br = (up * g_r[r0 - TSH + (cc >> 1)] + (1.0f - right) * g_r[r0 + ((cc + 1) >> 1)] +
+                    (1.0f - left) * g_r[r0 + ((cc - 1) >> 1)] + down * g_r[r0 + TSH + (cc >> 1)]) *
+                   temp;
+        float bb = (up * g_b[r0 - TSH + (cc >> 1)] + (1.0f - right) * g_b[r0 + ((cc + 1) >> 1)] +
+                    (1.0f - left) * g_b[r0 + ((cc - 1) >> 1)] + down * g_b[r0 + TSH + (cc >> 1)]) *
+                   temp;
+        red = sel_max(green - br, 0.0f);
+        blue = sel_max(green - bb, 0.0f);
+    } else {
+        green = t.half(GRB)[i >> 1];
+        red = sel_max(green - g_r[i >> 1], 0.0f);
+        blue = sel_max(green - g_b[i >> 1], 0.0f);
+    }
+    float rgb[3] = {ch == 0 ? sample : red, ch == 1 ? sample : sel_max(green, 0.0f),
+                    ch == 2 ? sample : blue};
+    rgb[0] *= a.inverse[0];
+    rgb[1] *= a.inverse[1];
+    rgb[2] *= a.inverse[2];
+    const size_t n = (size_t)a.width * a.height, o = (size_t)gy * a.width + gx;
+    a.out[o] = rgb[0];
+    a.out[n + o] = rgb[1];
+    a.out[2 * n + o] = rgb[2];
+    if (!(is_finite(rgb[0]) && is_finite(rgb[1]) && is_finite(rgb[2])))
+        atomicOr(&a.stats->nonfinite, 2u);
+}
+
+}  // namespace
+
+size_t amaze_scratch_bytes_per_tile() { return TILE_FLOATS * sizeof(float) + 4 * sizeof(int); }
+
+// mod.rs:136-161: the stage sequence for `tiles` tiles starting at
+// a.tile_base; a.scratch and a.areas hold one slot per tile of this launch.
+hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s) {
+    const float clip_pt8 = 0.8f * a.clip_point;
+    const dim3 px_grid(TS / 64, TS / 4, tiles), pair_grid(TSH / 64, TS / 4, tiles), blk(256);
+    hipLaunchKernelGGL(amz_load, px_grid, blk, 0, s, a);
+    hipLaunchKernelGGL(amz_gradients, px_grid, blk, 0, s, a);
+    hipLaunchKernelGGL(amz_colour_differences, px_grid, blk, 0, s, a, clip_pt8);
+    hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a, a.clip_point);
+    hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles), dim3(64), 0,
+                       s, a, a.clip_point);
+    hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point);
+    hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a);
+    hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a);
+    hipLaunchKernelGGL(amz_chains, dim3(tiles), dim3(TSH), 0, s, a);
+    hipLaunchKernelGGL(amz_refine, pair_grid, blk, 0, s, a, a.clip_point);
+    hipLaunchKernelGGL(amz_chroma, pair_grid, blk, 0, s, a);
+    hipLaunchKernelGGL(amz_finish, dim3(VALID / 64 + 1, VALID / 4, tiles), blk, 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace rhd

@@ -1,0 +1,969 @@
+// librhd C ABI: contexts, resident handles, validation and stage sequencing.
+//
+// Mirrors crates/gpu/src/sensor/{develop,normalize,highlight,demosaic}.rs:
+// the same stage order, the same routing errors (UnsupportedEdits and
+// OverBudget keep the device; everything else is a device or input error),
+// a logical memory budget held by every live handle and scratch block
+// (crates/gpu/src/memory.rs:42-81), and a cancellation flag polled between
+// stages (crates/core/src/cancel.rs:13-42).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rhd_internal.hpp"
+
+using namespace rhd;
+
+namespace {
+
+constexpr uint64_t DEFAULT_BUDGET = 64ull << 30;  // 64 GiB of the 288 GiB HBM3E
+constexpr int MAX_TILES_PER_LAUNCH = 2048;
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct rhd_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::atomic<uint64_t> budget{DEFAULT_BUDGET};
+    std::atomic<uint64_t> reserved{0};
+    std::string last_error;
+    DeviceStats* d_stats = nullptr;
+    DeviceStats* h_stats = nullptr;
+    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed;
+    hipEvent_t ev[5] = {};
+};
+
+struct rhd_mosaic {
+    rhd_context* ctx;
+    uint16_t* d;
+    uint64_t row_stride;
+    uint32_t width, height;
+    uint64_t bytes;
+};
+
+struct rhd_plane {
+    rhd_context* ctx;
+    float* d;
+    uint32_t width, height;
+    int32_t pattern;
+    uint64_t bytes;
+};
+
+struct rhd_source {
+    rhd_context* ctx;
+    float* d;
+    uint32_t width, height;
+    uint64_t bytes;
+};
+
+namespace {
+
+rhd_status fail(rhd_context* ctx, rhd_status status, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return status;
+}
+
+rhd_status hip_fail(rhd_context* ctx, hipError_t e, const char* what) {
+    return fail(ctx, RHD_SYNCHRONIZATION, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RHD_HIP(ctx, call)                                          \
+    do {                                                            \
+        hipError_t _e = (call);                                     \
+        if (_e != hipSuccess) return hip_fail((ctx), _e, #call);    \
+    } while (0)
+
+bool cancelled(const volatile uint32_t* flag) { return flag && *flag != 0; }
+
+rhd_status reserve(rhd_context* ctx, uint64_t bytes) {
+    uint64_t cur = ctx->reserved.load();
+    while (true) {
+        if (cur + bytes > ctx->budget.load())
+            return fail(ctx, RHD_OVER_BUDGET,
+                        "GPU work does not fit: " + std::to_string(cur + bytes) +
+                            " bytes exceed the budget of " + std::to_string(ctx->budget.load()));
+        if (ctx->reserved.compare_exchange_weak(cur, cur + bytes)) return RHD_OK;
+    }
+}
+void unreserve(rhd_context* ctx, uint64_t bytes) { ctx->reserved.fetch_sub(bytes); }
+
+void free_buffer(rhd_context* ctx, DeviceBuffer& b) {
+    if (b.ptr) {
+        (void)hipFree(b.ptr);
+        unreserve(ctx, b.bytes);
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+void drop_scratch(rhd_context* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    free_buffer(ctx, ctx->amaze);
+    free_buffer(ctx, ctx->amaze_areas);
+    free_buffer(ctx, ctx->rcd);
+    free_buffer(ctx, ctx->capture);
+    free_buffer(ctx, ctx->opposed);
+}
+
+// Reserve and allocate, dropping cached scratch once if the budget is short.
+rhd_status device_alloc(rhd_context* ctx, uint64_t bytes, void** out) {
+    rhd_status st = reserve(ctx, bytes);
+    if (st == RHD_OVER_BUDGET) {
+        drop_scratch(ctx);
+        st = reserve(ctx, bytes);
+    }
+    if (st != RHD_OK) return st;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        unreserve(ctx, bytes);
+        (void)hipGetLastError();
+        return fail(ctx, RHD_OVER_BUDGET, std::string("device allocation failed: ") +
+                                              hipGetErrorString(e));
+    }
+    return RHD_OK;
+}
+
+rhd_status ensure_scratch(rhd_context* ctx, DeviceBuffer& b, uint64_t bytes) {
+    if (b.bytes >= bytes) return RHD_OK;
+    if (b.ptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_buffer(ctx, b);
+    }
+    void* p = nullptr;
+    rhd_status st = device_alloc(ctx, bytes, &p);
+    if (st != RHD_OK) return st;
+    b.ptr = p;
+    b.bytes = bytes;
+    return RHD_OK;
+}
+
+bool finite_positive3(const float v[3]) {
+    for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(v[c]) || v[c] <= 0.0f) return false;
+    return true;
+}
+
+// crates/core/src/pipeline/sensor.rs:508-528 (development_geometry),
+// :544-596 (validate_levels), :180-202 (validate_frame) and the
+// normalization output bounds of :227-244.
+rhd_status validate_desc(rhd_context* ctx, const rhd_sensor_desc* d) {
+    if (!d) return fail(ctx, RHD_INVALID_INPUT, "missing sensor description");
+    if (d->sensor_width == 0 || d->sensor_height == 0 || d->row_stride < d->sensor_width)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "decoded row stride is shorter than the sensor width");
+    if (d->sensor_pattern < 0 || d->sensor_pattern > 3)
+        return fail(ctx, RHD_INVALID_INPUT, "unsupported CFA pattern");
+    if (d->crop_width == 0 || d->crop_height == 0 ||
+        (uint64_t)d->crop_x + d->crop_width > d->sensor_width ||
+        (uint64_t)d->crop_y + d->crop_height > d->sensor_height)
+        return fail(ctx, RHD_INVALID_INPUT, "crop region is outside the sensor");
+    if (d->crop_width < 2 || d->crop_height < 2)
+        return fail(ctx, RHD_INVALID_DIMENSIONS,
+                    "normalization output must be at least 2x2 and no larger than the resolved crop");
+    if ((uint64_t)d->crop_width * d->crop_height > (1ull << 31))
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "crop exceeds 2^31 samples");
+    uint32_t rw = d->black_repeat_width, rh = d->black_repeat_height;
+    if (!d->black_levels || rw == 0 || rh == 0)
+        return fail(ctx, RHD_INVALID_INPUT, "a non-empty one-component black repeat pattern is required");
+    uint32_t expected = rw * rh;
+    for (uint32_t k = 0; k < expected; ++k)
+        if (!std::isfinite(d->black_levels[k]))
+            return fail(ctx, RHD_INVALID_INPUT, "black levels must be finite");
+    if (!d->white_levels || !(d->white_count == 1 || d->white_count == 3 || d->white_count == expected))
+        return fail(ctx, RHD_INVALID_INPUT, "expected 1, 3, or repeat-pattern white levels");
+    for (uint32_t k = 0; k < d->white_count; ++k)
+        if (!std::isfinite(d->white_levels[k]))
+            return fail(ctx, RHD_INVALID_INPUT, "all white levels must be finite");
+    for (uint32_t y = 0; y < rh; ++y)
+        for (uint32_t x = 0; x < rw; ++x) {
+            uint32_t index = y * rw + x;
+            float black = d->black_levels[index];
+            float white = d->white_count == 1   ? d->white_levels[0]
+                          : d->white_count == 3 ? d->white_levels[color_at(d->sensor_pattern, x, y)]
+                                                : d->white_levels[index];
+            if (white <= black)
+                return fail(ctx, RHD_INVALID_INPUT, "white level must exceed black level");
+        }
+    switch (d->highlight) {
+        case RHD_HIGHLIGHT_OFF:
+            break;
+        case RHD_HIGHLIGHT_CLIP:
+            if (!finite_positive3(d->clip_levels))
+                return fail(ctx, RHD_INVALID_INPUT, "highlight levels must be finite and positive");
+            break;
+        case RHD_HIGHLIGHT_OPPOSED:
+            if (!finite_positive3(d->detection_levels))
+                return fail(ctx, RHD_INVALID_INPUT, "highlight levels must be finite and positive");
+            if (d->has_neutral_ceiling && !finite_positive3(d->neutral_ceiling))
+                return fail(ctx, RHD_INVALID_INPUT, "neutral ceiling must be finite and positive");
+            break;
+        default:
+            return fail(ctx, RHD_INVALID_INPUT, "unknown highlight method");
+    }
+    if (d->demosaic < RHD_DEMOSAIC_BILINEAR || d->demosaic > RHD_DEMOSAIC_AMAZE)
+        return fail(ctx, RHD_INVALID_INPUT, "unknown demosaic algorithm");
+    if (!finite_positive3(d->demosaic_balance))
+        return fail(ctx, RHD_INVALID_INPUT, "effective R, G, and B gains must be finite and positive");
+    return RHD_OK;
+}
+
+bool capture_active(const rhd_detail_desc* dd) {
+    return dd && dd->capture_enabled && dd->amount > 0.0f;
+}
+
+// crates/edit/src/sharpening.rs:50-81 and crates/core/src/sharpening.rs:26-42.
+rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd) {
+    if (!dd) return RHD_OK;
+    auto in = [](float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; };
+    if (!in(dd->amount, 0.0f, 1.0f) || !in(dd->sigma, 0.3f, 1.2f) ||
+        !in(dd->noise_protection, 0.0f, 1.0f))
+        return fail(ctx, RHD_INVALID_INPUT, "capture sharpening settings are out of range");
+    if (capture_active(dd)) {
+        if (!finite_positive3(dd->ceilings))
+            return fail(ctx, RHD_INVALID_INPUT, "capture ceilings must be finite and positive");
+        if (dd->iterations < 0 || dd->iterations > 64)
+            return fail(ctx, RHD_INVALID_INPUT, "capture iterations must be in 0..=64");
+    }
+    return RHD_OK;
+}
+
+// crates/core/src/pipeline/sensor.rs:598-610, tabulated over the combined
+// period of the black pattern and the 2x2 CFA.
+rhd_status level_table(rhd_context* ctx, const rhd_sensor_desc* d, LevelTable& t) {
+    uint32_t rw = d->black_repeat_width, rh = d->black_repeat_height;
+    uint32_t px = rw % 2 ? 2 * rw : rw, py = rh % 2 ? 2 * rh : rh;
+    if (px > 16 || py > 16)
+        return fail(ctx, RHD_UNSUPPORTED_EDITS, "black level repeat pattern larger than 8x8");
+    t.period_x = (int)px;
+    t.period_y = (int)py;
+    for (uint32_t y = 0; y < py; ++y)
+        for (uint32_t x = 0; x < px; ++x) {
+            uint32_t index = (y % rh) * rw + (x % rw);
+            float black = d->black_levels[index];
+            float white = d->white_count == 1   ? d->white_levels[0]
+                          : d->white_count == 3 ? d->white_levels[color_at(d->sensor_pattern, x, y)]
+                                                : d->white_levels[index];
+            t.black[y * px + x] = black;
+            t.range[y * px + x] = white - black;
+        }
+    return RHD_OK;
+}
+
+int crop_pattern(const rhd_sensor_desc* d) {
+    // BayerPattern::shifted (crates/image/src/lib.rs:119-135).
+    for (int candidate = 0; candidate < 4; ++candidate) {
+        bool same = true;
+        for (int y = 0; y < 2 && same; ++y)
+            for (int x = 0; x < 2 && same; ++x)
+                same = color_at(candidate, x, y) ==
+                       color_at(d->sensor_pattern, x + (int)d->crop_x, y + (int)d->crop_y);
+        if (same) return candidate;
+    }
+    return d->sensor_pattern;
+}
+
+rhd_status new_plane(rhd_context* ctx, uint32_t w, uint32_t h, int32_t pattern, rhd_plane** out) {
+    uint64_t bytes = (uint64_t)w * h * sizeof(float);
+    void* p = nullptr;
+    rhd_status st = device_alloc(ctx, bytes, &p);
+    if (st != RHD_OK) return st;
+    *out = new rhd_plane{ctx, (float*)p, w, h, pattern, bytes};
+    return RHD_OK;
+}
+
+rhd_status new_source(rhd_context* ctx, uint32_t w, uint32_t h, rhd_source** out) {
+    uint64_t bytes = (uint64_t)w * h * 3 * sizeof(float);
+    void* p = nullptr;
+    rhd_status st = device_alloc(ctx, bytes, &p);
+    if (st != RHD_OK) return st;
+    *out = new rhd_source{ctx, (float*)p, w, h, bytes};
+    return RHD_OK;
+}
+
+rhd_status reset_stats(rhd_context* ctx) {
+    RHD_HIP(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(DeviceStats), ctx->stream));
+    return RHD_OK;
+}
+
+// Reads the device counters back and turns the non-finite flags into the
+// reference's errors.
+rhd_status finish_stats(rhd_context* ctx, rhd_metrics* m, int highlight) {
+    RHD_HIP(ctx, hipMemcpyAsync(ctx->h_stats, ctx->d_stats, sizeof(DeviceStats),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const DeviceStats& s = *ctx->h_stats;
+    if (m) {
+        m->readback_bytes += sizeof(DeviceStats);
+        m->highlight_method = highlight;
+        if (highlight == RHD_HIGHLIGHT_CLIP) {
+            m->clip.affected_sites = s.clip[0];
+            m->clip.changed_sites = s.clip[1];
+            m->clip.nominal_over_white_sites = s.clip[2];
+            for (int c = 0; c < 3; ++c) m->clip.affected_by_channel[c] = s.clip[3 + c];
+        } else if (highlight == RHD_HIGHLIGHT_OPPOSED) {
+            m->opposed.suspected_clipped_sites = s.detect[0];
+            for (int c = 0; c < 3; ++c) m->opposed.suspected_by_channel[c] = s.detect[3 + c];
+            m->opposed.reconstructed_sites = s.opposed[0];
+            m->opposed.changed_sites = s.opposed[1];
+            m->opposed.fallback_sites = s.opposed[2];
+            m->opposed.fully_unsupported_sites = s.opposed[3];
+            m->opposed.neutralised_sites = s.opposed[4];
+            m->opposed.regions = s.regions;
+        }
+    }
+    if (s.nonfinite & 1u)
+        return fail(ctx, RHD_INVALID_INPUT, "highlight processing received a non-finite sample");
+    if (s.nonfinite & 2u)
+        return fail(ctx, RHD_INVALID_INPUT, "demosaicing produced non-finite image data");
+    if (s.nonfinite & 4u)
+        return fail(ctx, RHD_INVALID_INPUT, "capture sharpening received non-finite image data");
+    return RHD_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+// Highlight on a plane already on the stream (normalize may have fused Clip
+// and the detection counts). Opposed always runs here.
+rhd_status run_opposed(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d) {
+    size_t need = opposed_scratch_bytes((int)plane->width, (int)plane->height);
+    rhd_status st = ensure_scratch(ctx, ctx->opposed, need);
+    if (st != RHD_OK) return st;
+    OpposedArgs a{};
+    a.mosaic = plane->d;
+    a.width = (int)plane->width;
+    a.height = (int)plane->height;
+    a.pattern = plane->pattern;
+    for (int c = 0; c < 3; ++c) {
+        a.levels[c] = d->detection_levels[c];
+        a.ceiling[c] = d->neutral_ceiling[c];
+    }
+    a.has_ceiling = d->has_neutral_ceiling;
+    a.scratch = ctx->opposed.ptr;
+    a.scratch_bytes = ctx->opposed.bytes;
+    a.stats = ctx->d_stats;
+    hipError_t e = launch_opposed(a, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "opposed");
+    return RHD_OK;
+}
+
+struct Balance {
+    float forward[3], inverse[3], clip_point;
+};
+
+// crates/demosaic/src/balance.rs:33-66 and lib.rs:321-356.
+Balance resolve_balance(const rhd_sensor_desc* d) {
+    Balance b{{1, 1, 1}, {1, 1, 1}, 1.0f};
+    if (d->demosaic != RHD_DEMOSAIC_AMAZE && d->demosaic != RHD_DEMOSAIC_RCD) return b;
+    const float* g = d->demosaic_balance;
+    float largest = std::fmax(std::fmax(g[0], g[1]), g[2]);
+    float f[3] = {g[0] / largest, g[1] / largest, g[2] / largest};
+    if (f[0] == 1.0f && f[1] == 1.0f && f[2] == 1.0f) return b;
+    for (int c = 0; c < 3; ++c) {
+        b.forward[c] = f[c];
+        b.inverse[c] = 1.0f / f[c];
+    }
+    b.clip_point = std::fmin(std::fmin(f[0], f[1]), f[2]);
+    return b;
+}
+
+rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sensor_desc* d,
+                        rhd_source* src) {
+    const int w = (int)plane->width, h = (int)plane->height, pattern = plane->pattern;
+    const Balance b = resolve_balance(d);
+    hipError_t e = hipSuccess;
+    switch (d->demosaic) {
+        case RHD_DEMOSAIC_BILINEAR:
+            e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d, ctx->d_stats,
+                                ctx->stream);
+            break;
+        case RHD_DEMOSAIC_MHC:
+            e = launch_mhc(plane->d, w, h, pattern, src->d, ctx->d_stats, ctx->stream);
+            break;
+        case RHD_DEMOSAIC_AMAZE: {
+            if (w <= 32 || h <= 32) {
+                e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
+                                    ctx->d_stats, ctx->stream);
+                break;
+            }
+            const int tx = (w + 223) / 224, ty = (h + 223) / 224, tiles = tx * ty;
+            const int per_launch = std::min(tiles, MAX_TILES_PER_LAUNCH);
+            rhd_status st = ensure_scratch(ctx, ctx->amaze,
+                                           (uint64_t)per_launch * amaze_scratch_bytes_per_tile());
+            if (st != RHD_OK) return st;
+            st = ensure_scratch(ctx, ctx->amaze_areas, (uint64_t)per_launch * 4 * sizeof(int));
+            if (st != RHD_OK) return st;
+            AmazeArgs a{};
+            a.mosaic = plane->d;
+            a.width = w;
+            a.height = h;
+            a.pattern = pattern;
+            a.tiles_x = tx;
+            a.tiles_y = ty;
+            for (int c = 0; c < 3; ++c) {
+                a.forward[c] = b.forward[c];
+                a.inverse[c] = b.inverse[c];
+            }
+            a.clip_point = b.clip_point;
+            a.scratch = (float*)ctx->amaze.ptr;
+            a.areas = (int*)ctx->amaze_areas.ptr;
+            a.out = src->d;
+            a.stats = ctx->d_stats;
+            for (int base = 0; base < tiles && e == hipSuccess; base += per_launch) {
+                a.tile_base = base;
+                e = launch_amaze(a, std::min(per_launch, tiles - base), ctx->stream);
+            }
+            break;
+        }
+        case RHD_DEMOSAIC_RCD: {
+            if (w <= 24 || h <= 24) {
+                e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
+                                    ctx->d_stats, ctx->stream);
+                break;
+            }
+            const int tx = (w + 173) / 174, ty = (h + 173) / 174, tiles = tx * ty;
+            const int per_launch = std::min(tiles, MAX_TILES_PER_LAUNCH);
+            rhd_status st =
+                ensure_scratch(ctx, ctx->rcd, (uint64_t)per_launch * rcd_scratch_bytes_per_tile());
+            if (st != RHD_OK) return st;
+            for (int base = 0; base < tiles && e == hipSuccess; base += per_launch)
+                e = launch_rcd(plane->d, w, h, pattern, b.forward, b.inverse, (float*)ctx->rcd.ptr,
+                               base, std::min(per_launch, tiles - base), src->d, ctx->d_stats,
+                               ctx->stream);
+            break;
+        }
+        default:
+            return fail(ctx, RHD_INVALID_INPUT, "unknown demosaic algorithm");
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "demosaic");
+    return RHD_OK;
+}
+
+// Gaussian weights of crates/detail/src/gaussian.rs:9-17, with the platform
+// expf and a sum that starts at -0.0 like Rust's f32 Sum.
+int gaussian_weights(float sigma, float* w) {
+    int radius = (int)std::ceil(3.0f * sigma);
+    if (radius > 4) radius = 4;
+    float sum = -0.0f;
+    for (int x = -radius; x <= radius; ++x) {
+        float t = (float)x / sigma;
+        w[x + radius] = std::exp(-0.5f * (t * t));
+    }
+    for (int k = 0; k <= 2 * radius; ++k) sum += w[k];
+    for (int k = 0; k <= 2 * radius; ++k) w[k] /= sum;
+    return radius;
+}
+
+rhd_status run_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc* dd) {
+    const int w = (int)src->width, h = (int)src->height;
+    rhd_status st = ensure_scratch(ctx, ctx->capture, capture_scratch_bytes(w, h));
+    if (st != RHD_OK) return st;
+    CaptureArgs a{};
+    a.rgb = src->d;
+    a.width = w;
+    a.height = h;
+    a.amount = dd->amount;
+    a.noise_protection = dd->noise_protection;
+    for (int c = 0; c < 3; ++c) a.ceilings[c] = dd->ceilings[c];
+    a.radius = gaussian_weights(dd->sigma, a.weights);
+    a.iterations = dd->iterations;
+    float* base = (float*)ctx->capture.ptr;
+    size_t n = (size_t)w * h;
+    a.guide = base;
+    a.mask = base + n;
+    a.est_a = base + 2 * n;
+    a.est_b = base + 3 * n;
+    a.tmp = base + 4 * n;
+    a.stats = ctx->d_stats;
+    hipError_t e = launch_capture(a, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "capture");
+    return RHD_OK;
+}
+
+rhd_status check_handle_ctx(rhd_context* ctx, const void* handle_ctx) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    if (handle_ctx != ctx) return fail(ctx, RHD_INVALID_INPUT, "handle belongs to another context");
+    return RHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
+    if (!out) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        (void)hipGetLastError();
+        return RHD_UNSUPPORTED;
+    }
+    if (hipSetDevice(device) != hipSuccess) return RHD_UNSUPPORTED;
+    rhd_context* ctx = new rhd_context();
+    ctx->device = device;
+    if (budget_bytes) ctx->budget = budget_bytes;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->d_stats, sizeof(DeviceStats)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_stats, sizeof(DeviceStats), hipHostMallocDefault) != hipSuccess) {
+        rhd_destroy(ctx);
+        return RHD_UNSUPPORTED;
+    }
+    for (auto& ev : ctx->ev)
+        if (hipEventCreate(&ev) != hipSuccess) {
+            rhd_destroy(ctx);
+            return RHD_UNSUPPORTED;
+        }
+    *out = ctx;
+    return RHD_OK;
+}
+
+void rhd_destroy(rhd_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_scratch(ctx);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+    if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+rhd_status rhd_set_budget(rhd_context* ctx, uint64_t budget_bytes) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    ctx->budget = budget_bytes ? budget_bytes : DEFAULT_BUDGET;
+    if (ctx->reserved.load() > ctx->budget.load()) drop_scratch(ctx);
+    return RHD_OK;
+}
+
+uint64_t rhd_reserved_bytes(const rhd_context* ctx) { return ctx ? ctx->reserved.load() : 0; }
+
+const char* rhd_last_error(const rhd_context* ctx) {
+    return ctx ? ctx->last_error.c_str() : "no context";
+}
+
+const char* rhd_status_name(rhd_status s) {
+    switch (s) {
+        case RHD_OK: return "Ok";
+        case RHD_CANCELLED: return "Cancelled";
+        case RHD_UNSUPPORTED: return "Unsupported";
+        case RHD_OVER_BUDGET: return "OverBudget";
+        case RHD_INVALID_DIMENSIONS: return "InvalidDimensions";
+        case RHD_BASE_MISMATCH: return "BaseMismatch";
+        case RHD_UNSUPPORTED_EDITS: return "UnsupportedEdits";
+        case RHD_SYNCHRONIZATION: return "Synchronization";
+        case RHD_INVALID_INPUT: return "InvalidInput";
+        case RHD_READBACK: return "Readback";
+    }
+    return "Unknown";
+}
+
+int rhd_status_is_routing(rhd_status s) {
+    return s == RHD_UNSUPPORTED_EDITS || s == RHD_OVER_BUDGET;
+}
+
+int rhd_supports(const rhd_sensor_desc* d, const rhd_detail_desc* dd) {
+    if (!d) return 0;
+    bool highlight = d->highlight == RHD_HIGHLIGHT_OFF || d->highlight == RHD_HIGHLIGHT_CLIP ||
+                     d->highlight == RHD_HIGHLIGHT_OPPOSED;
+    bool demosaic = d->demosaic >= RHD_DEMOSAIC_BILINEAR && d->demosaic <= RHD_DEMOSAIC_AMAZE;
+    bool detail = !dd || (!dd->false_colour && !dd->noise_reduction);
+    bool capture = !capture_active(dd) || std::ceil(3.0f * dd->sigma) <= 4.0f;
+    return highlight && demosaic && detail && capture ? 1 : 0;
+}
+
+rhd_status rhd_mosaic_upload(rhd_context* ctx, const uint16_t* host, uint64_t row_stride,
+                             uint32_t width, uint32_t height, rhd_mosaic** out) {
+    if (!ctx || !out) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    if (!host || width == 0 || height == 0 || row_stride < width)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "decoded row stride is shorter than the sensor width");
+    (void)hipSetDevice(ctx->device);
+    uint64_t bytes = row_stride * height * sizeof(uint16_t);
+    void* p = nullptr;
+    rhd_status st = device_alloc(ctx, bytes, &p);
+    if (st != RHD_OK) return st;
+    hipError_t e = hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        unreserve(ctx, bytes);
+        return hip_fail(ctx, e, "mosaic upload");
+    }
+    *out = new rhd_mosaic{ctx, (uint16_t*)p, row_stride, width, height, bytes};
+    return RHD_OK;
+}
+
+void rhd_mosaic_release(rhd_mosaic* m) {
+    if (!m) return;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(m->d);
+    unreserve(m->ctx, m->bytes);
+    delete m;
+}
+
+rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_sensor_desc* d,
+                         const volatile uint32_t* cancel, rhd_plane** out, rhd_metrics* m) {
+    if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
+    if (st != RHD_OK) return st;
+    if ((st = validate_desc(ctx, d)) != RHD_OK) return st;
+    if (mosaic->width != d->sensor_width || mosaic->height != d->sensor_height ||
+        mosaic->row_stride != d->row_stride)
+        return fail(ctx, RHD_INVALID_INPUT, "decoded frame does not match its normalization contract");
+    if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
+    (void)hipSetDevice(ctx->device);
+    NormalizeArgs a{};
+    if ((st = level_table(ctx, d, a.table)) != RHD_OK) return st;
+    rhd_plane* plane = nullptr;
+    if ((st = new_plane(ctx, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
+        return st;
+    a.mosaic = mosaic->d;
+    a.row_stride = mosaic->row_stride;
+    a.crop_x = (int)d->crop_x;
+    a.crop_y = (int)d->crop_y;
+    a.width = (int)d->crop_width;
+    a.height = (int)d->crop_height;
+    a.crop_pattern = plane->pattern;
+    a.mode = 0;
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream);
+    if (e != hipSuccess) {
+        rhd_plane_release(plane);
+        return hip_fail(ctx, e, "normalize");
+    }
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    e = hipEventSynchronize(ctx->ev[1]);
+    if (e != hipSuccess) {
+        rhd_plane_release(plane);
+        return hip_fail(ctx, e, "normalize");
+    }
+    if (m) {
+        m->normalize_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+        m->launches += 1;
+        m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, plane->bytes);
+    }
+    *out = plane;
+    return RHD_OK;
+}
+
+rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d,
+                         const volatile uint32_t* cancel, rhd_metrics* m) {
+    if (!ctx || !plane) return RHD_INVALID_INPUT;
+    rhd_status st = check_handle_ctx(ctx, plane->ctx);
+    if (st != RHD_OK) return st;
+    if ((st = validate_desc(ctx, d)) != RHD_OK) return st;
+    if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
+    (void)hipSetDevice(ctx->device);
+    if ((st = reset_stats(ctx)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    hipError_t e = hipSuccess;
+    const int w = (int)plane->width, h = (int)plane->height;
+    if (d->highlight == RHD_HIGHLIGHT_CLIP) {
+        e = launch_clip(plane->d, w, h, plane->pattern, d->clip_levels, ctx->d_stats, ctx->stream);
+    } else {
+        // Off and Opposed validate every sample (opposed.rs:182-215).
+        float levels[3] = {INFINITY, INFINITY, INFINITY};
+        e = launch_detect(plane->d, w, h, plane->pattern,
+                          d->highlight == RHD_HIGHLIGHT_OPPOSED ? d->detection_levels : levels,
+                          ctx->d_stats, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "highlight");
+    if (d->highlight == RHD_HIGHLIGHT_OPPOSED) {
+        // Samples must be finite before reconstruction reads them.
+        if ((st = finish_stats(ctx, nullptr, d->highlight)) != RHD_OK) return st;
+        if ((st = run_opposed(ctx, plane, d)) != RHD_OK) return st;
+    }
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if ((st = finish_stats(ctx, m, d->highlight)) != RHD_OK) return st;
+    if (m) {
+        m->highlight_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+        m->launches += 1;
+        m->readback_bytes += sizeof(DeviceStats);
+    }
+    return RHD_OK;
+}
+
+rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d,
+                        const rhd_detail_desc* dd, const volatile uint32_t* cancel,
+                        rhd_source** out, rhd_metrics* m) {
+    if (!ctx || !out || !plane) {
+        rhd_plane_release(plane);
+        return RHD_INVALID_INPUT;
+    }
+    *out = nullptr;
+    rhd_status st = check_handle_ctx(ctx, plane->ctx);
+    if (st == RHD_OK) st = validate_desc(ctx, d);
+    if (st == RHD_OK) st = validate_detail(ctx, dd);
+    if (st == RHD_OK && !rhd_supports(d, dd))
+        st = fail(ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st != RHD_OK) {
+        rhd_plane_release(plane);
+        return st;
+    }
+    (void)hipSetDevice(ctx->device);
+    rhd_source* src = nullptr;
+    st = new_source(ctx, plane->width, plane->height, &src);
+    if (st == RHD_OK) st = reset_stats(ctx);
+    if (st == RHD_OK && hipEventRecord(ctx->ev[2], ctx->stream) != hipSuccess)
+        st = fail(ctx, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) st = run_demosaic(ctx, plane, d, src);
+    if (st == RHD_OK && hipEventRecord(ctx->ev[3], ctx->stream) != hipSuccess)
+        st = fail(ctx, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && capture_active(dd)) {
+        if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+        if (st == RHD_OK) st = run_capture(ctx, src, dd);
+    }
+    if (st == RHD_OK && hipEventRecord(ctx->ev[4], ctx->stream) != hipSuccess)
+        st = fail(ctx, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) st = finish_stats(ctx, nullptr, RHD_HIGHLIGHT_OFF);
+    rhd_plane_release(plane);
+    if (st != RHD_OK) {
+        rhd_source_release(src);
+        return st;
+    }
+    if (m) {
+        m->demosaic_ms = elapsed(ctx->ev[2], ctx->ev[3]);
+        m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+        m->launches += 2;
+        m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, ctx->reserved.load());
+    }
+    *out = src;
+    return RHD_OK;
+}
+
+rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc* dd,
+                       const volatile uint32_t* cancel, rhd_metrics* m) {
+    if (!ctx || !src) return RHD_INVALID_INPUT;
+    rhd_status st = check_handle_ctx(ctx, src->ctx);
+    if (st != RHD_OK) return st;
+    if ((st = validate_detail(ctx, dd)) != RHD_OK) return st;
+    if (!capture_active(dd)) return RHD_OK;
+    if (std::ceil(3.0f * dd->sigma) > 4.0f)
+        return fail(ctx, RHD_UNSUPPORTED_EDITS, "capture radius above 4 taps");
+    if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
+    (void)hipSetDevice(ctx->device);
+    if ((st = reset_stats(ctx)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    if ((st = run_capture(ctx, src, dd)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if ((st = finish_stats(ctx, nullptr, RHD_HIGHLIGHT_OFF)) != RHD_OK) return st;
+    if (m) m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+    return RHD_OK;
+}
+
+rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
+                                const rhd_sensor_desc* d, const rhd_detail_desc* dd,
+                                const volatile uint32_t* cancel, rhd_source** out,
+                                rhd_metrics* m) {
+    if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK) st = validate_desc(ctx, d);
+    if (st == RHD_OK) st = validate_detail(ctx, dd);
+    if (st == RHD_OK && !rhd_supports(d, dd))
+        st = fail(ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
+    if (st != RHD_OK) return st;
+    if (mosaic->width != d->sensor_width || mosaic->height != d->sensor_height ||
+        mosaic->row_stride != d->row_stride)
+        return fail(ctx, RHD_INVALID_INPUT, "decoded frame does not match its normalization contract");
+    (void)hipSetDevice(ctx->device);
+    NormalizeArgs a{};
+    if ((st = level_table(ctx, d, a.table)) != RHD_OK) return st;
+    rhd_plane* plane = nullptr;
+    if ((st = new_plane(ctx, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
+        return st;
+    rhd_source* src = nullptr;
+    if ((st = new_source(ctx, plane->width, plane->height, &src)) != RHD_OK) {
+        rhd_plane_release(plane);
+        return st;
+    }
+    a.mosaic = mosaic->d;
+    a.row_stride = mosaic->row_stride;
+    a.crop_x = (int)d->crop_x;
+    a.crop_y = (int)d->crop_y;
+    a.width = (int)d->crop_width;
+    a.height = (int)d->crop_height;
+    a.crop_pattern = plane->pattern;
+    a.mode = d->highlight == RHD_HIGHLIGHT_CLIP ? 1 : (d->highlight == RHD_HIGHLIGHT_OPPOSED ? 2 : 0);
+    const float* lv = d->highlight == RHD_HIGHLIGHT_CLIP ? d->clip_levels : d->detection_levels;
+    for (int c = 0; c < 3; ++c) a.levels[c] = lv[c];
+    auto step = [&](bool ok, rhd_status err_status, const char* what) {
+        if (!ok && st == RHD_OK) st = fail(ctx, err_status, what);
+    };
+    st = reset_stats(ctx);
+    step(hipEventRecord(ctx->ev[0], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) {
+        hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "normalize");
+    }
+    step(hipEventRecord(ctx->ev[1], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK && d->highlight == RHD_HIGHLIGHT_OPPOSED) st = run_opposed(ctx, plane, d);
+    step(hipEventRecord(ctx->ev[2], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK) st = run_demosaic(ctx, plane, d, src);
+    step(hipEventRecord(ctx->ev[3], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && capture_active(dd)) {
+        if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+        if (st == RHD_OK) st = run_capture(ctx, src, dd);
+    }
+    step(hipEventRecord(ctx->ev[4], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) st = finish_stats(ctx, m, d->highlight);
+    if (st != RHD_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        rhd_plane_release(plane);
+        rhd_source_release(src);
+        return st;
+    }
+    if (m) {
+        m->normalize_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+        m->highlight_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+        m->demosaic_ms = elapsed(ctx->ev[2], ctx->ev[3]);
+        m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+        m->total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
+        m->launches += 4;
+        m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, ctx->reserved.load());
+    }
+    rhd_plane_release(plane);
+    *out = src;
+    return RHD_OK;
+}
+
+rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t row_stride,
+                       const rhd_sensor_desc* d, const rhd_detail_desc* dd,
+                       const volatile uint32_t* cancel, rhd_source** out, rhd_metrics* m) {
+    if (!ctx || !out || !d) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    rhd_mosaic* mosaic = nullptr;
+    rhd_status st =
+        rhd_mosaic_upload(ctx, host_mosaic, row_stride, d->sensor_width, d->sensor_height, &mosaic);
+    if (st != RHD_OK) return st;
+    if (m) m->uploaded_bytes += mosaic->bytes;
+    st = rhd_develop_resident(ctx, mosaic, d, dd, cancel, out, m);
+    rhd_mosaic_release(mosaic);
+    return st;
+}
+
+rhd_status rhd_plane_read(rhd_context* ctx, const rhd_plane* plane, float* dst, uint64_t count) {
+    if (!ctx || !plane || !dst) return RHD_INVALID_INPUT;
+    if (count < (uint64_t)plane->width * plane->height)
+        return fail(ctx, RHD_READBACK, "destination too small");
+    RHD_HIP(ctx, hipMemcpyAsync(dst, plane->d, (size_t)plane->width * plane->height * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RHD_OK;
+}
+
+rhd_status rhd_plane_write(rhd_context* ctx, rhd_plane* plane, const float* src, uint64_t count) {
+    if (!ctx || !plane || !src) return RHD_INVALID_INPUT;
+    if (count < (uint64_t)plane->width * plane->height)
+        return fail(ctx, RHD_INVALID_INPUT, "source too small");
+    RHD_HIP(ctx, hipMemcpyAsync(plane->d, src, (size_t)plane->width * plane->height * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RHD_OK;
+}
+
+rhd_status rhd_plane_upload(rhd_context* ctx, const float* host, uint32_t width, uint32_t height,
+                            int32_t pattern, rhd_plane** out) {
+    if (!ctx || !out || !host) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    if (width < 2 || height < 2)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "Bayer demosaicing requires at least 2x2 samples");
+    if (pattern < 0 || pattern > 3) return fail(ctx, RHD_INVALID_INPUT, "unsupported CFA pattern");
+    (void)hipSetDevice(ctx->device);
+    rhd_plane* plane = nullptr;
+    rhd_status st = new_plane(ctx, width, height, pattern, &plane);
+    if (st != RHD_OK) return st;
+    st = rhd_plane_write(ctx, plane, host, (uint64_t)width * height);
+    if (st != RHD_OK) {
+        rhd_plane_release(plane);
+        return st;
+    }
+    *out = plane;
+    return RHD_OK;
+}
+
+void rhd_plane_dims(const rhd_plane* p, uint32_t* w, uint32_t* h, int32_t* pattern) {
+    if (!p) return;
+    if (w) *w = p->width;
+    if (h) *h = p->height;
+    if (pattern) *pattern = p->pattern;
+}
+
+void rhd_plane_release(rhd_plane* p) {
+    if (!p) return;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipFree(p->d);
+    unreserve(p->ctx, p->bytes);
+    delete p;
+}
+
+rhd_status rhd_source_read_rgb(rhd_context* ctx, const rhd_source* s, float* dst, uint64_t count) {
+    if (!ctx || !s || !dst) return RHD_INVALID_INPUT;
+    const size_t n = (size_t)s->width * s->height;
+    if (count < 3 * (uint64_t)n) return fail(ctx, RHD_READBACK, "destination too small");
+    std::vector<float> planar(3 * n);
+    RHD_HIP(ctx, hipMemcpyAsync(planar.data(), s->d, 3 * n * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; ++i) {
+        dst[3 * i] = planar[i];
+        dst[3 * i + 1] = planar[n + i];
+        dst[3 * i + 2] = planar[2 * n + i];
+    }
+    return RHD_OK;
+}
+
+rhd_status rhd_source_planes(const rhd_source* s, float** r, float** g, float** b) {
+    if (!s) return RHD_INVALID_INPUT;
+    const size_t n = (size_t)s->width * s->height;
+    if (r) *r = s->d;
+    if (g) *g = s->d + n;
+    if (b) *b = s->d + 2 * n;
+    return RHD_OK;
+}
+
+void rhd_source_dims(const rhd_source* s, uint32_t* w, uint32_t* h) {
+    if (!s) return;
+    if (w) *w = s->width;
+    if (h) *h = s->height;
+}
+
+void rhd_source_release(rhd_source* s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(s->d);
+    unreserve(s->ctx, s->bytes);
+    delete s;
+}
+
+rhd_status rhd_synchronize(rhd_context* ctx) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RHD_OK;
+}
+
+}  // extern "C"

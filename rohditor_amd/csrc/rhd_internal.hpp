@@ -1,0 +1,97 @@
+// Host-side internals of librhd: handles, the context, and the launchers
+// each kernel file exports to the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <string>
+
+#include "../../include/rhd.h"
+#include "rhd_device.hpp"
+
+namespace rhd {
+
+// Normalization level table: black level and (white - black) per position in
+// the combined repeat period of the black pattern and the sensor CFA.
+struct LevelTable {
+    int period_x, period_y;
+    float black[256];
+    float range[256];
+};
+
+struct NormalizeArgs {
+    const uint16_t* mosaic;
+    uint64_t row_stride;
+    int crop_x, crop_y, width, height;
+    int crop_pattern;
+    int mode;             // 0 = off, 1 = clip, 2 = opposed detection counts
+    float levels[3];      // clip limits or detection levels
+    LevelTable table;
+};
+
+// Geometry and parameters shared by the AMaZE kernels.
+struct AmazeArgs {
+    const float* mosaic;  // normalized, highlighted mosaic (width * height)
+    int width, height, pattern;
+    int tiles_x, tiles_y, tile_base;  // tile_base: first tile index of this launch
+    float forward[3];     // channel balance applied while loading
+    float inverse[3];     // gains applied to the output
+    float clip_point;
+    float* scratch;       // per-tile planes
+    int* areas;           // per tile: start_row, end_row, start_col, end_col
+    float* out;           // planar R, G, B (3 * width * height)
+    DeviceStats* stats;
+};
+
+// Launchers (each returns the first HIP error, or hipSuccess).
+hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
+                            hipStream_t s);
+hipError_t launch_clip(float* plane, int width, int height, int pattern, const float limits[3],
+                       DeviceStats* stats, hipStream_t s);
+hipError_t launch_detect(const float* plane, int width, int height, int pattern,
+                         const float levels[3], DeviceStats* stats, hipStream_t s);
+hipError_t launch_bilinear(const float* mosaic, int width, int height, int pattern,
+                           const float forward[3], const float gains[3], float* out,
+                           DeviceStats* stats, hipStream_t s);
+hipError_t launch_mhc(const float* mosaic, int width, int height, int pattern, float* out,
+                      DeviceStats* stats, hipStream_t s);
+size_t amaze_scratch_bytes_per_tile();
+hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s);
+size_t rcd_scratch_bytes_per_tile();
+hipError_t launch_rcd(const float* mosaic, int width, int height, int pattern,
+                      const float forward[3], const float inverse[3], float* scratch,
+                      int tile_base, int tiles, float* out, DeviceStats* stats, hipStream_t s);
+
+struct CaptureArgs {
+    float* rgb;  // planar, 3 * n
+    int width, height;
+    float amount, noise_protection;
+    float ceilings[3];
+    int radius;
+    float weights[9];
+    int iterations;
+    float* guide;
+    float* mask;
+    float* est_a;
+    float* est_b;
+    float* tmp;
+    DeviceStats* stats;
+};
+size_t capture_scratch_bytes(int width, int height);
+hipError_t launch_capture(const CaptureArgs& a, hipStream_t s);
+
+struct OpposedArgs {
+    float* mosaic;
+    int width, height, pattern;
+    float levels[3];
+    int has_ceiling;
+    float ceiling[3];
+    void* scratch;
+    size_t scratch_bytes;
+    DeviceStats* stats;
+};
+size_t opposed_scratch_bytes(int width, int height);
+hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s);
+
+}  // namespace rhd
