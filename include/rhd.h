@@ -206,6 +206,18 @@ void rhd_source_release(rhd_source* source);
 /* Blocks until the context's stream is idle. */
 rhd_status rhd_synchronize(rhd_context* ctx);
 
+/* Per-kernel GPU timing, the counterpart of the reference's executor metrics
+ * (SensorMetrics etc., crates/gpu/src/sensor/normalize.rs:16-28), which carry
+ * estimated bytes but no device timestamps: when enabled, HIP events are
+ * recorded on the context's stream around every kernel launch, and each entry
+ * reports a kernel group's total milliseconds, launch count and algorithmic
+ * bytes per launch. For benchmarking; off by default. */
+rhd_status rhd_profile_enable(rhd_context* ctx, int enable);
+rhd_status rhd_profile_reset(rhd_context* ctx);
+uint32_t rhd_profile_count(rhd_context* ctx);
+rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,
+                             uint64_t* launches, uint64_t* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

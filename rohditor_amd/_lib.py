@@ -93,6 +93,11 @@ SIGNATURES = {
     "rhd_source_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32)]),
     "rhd_source_release": (None, [_P]),
     "rhd_synchronize": (c_int, [_P]),
+    "rhd_profile_enable": (c_int, [_P, c_int]),
+    "rhd_profile_reset": (c_int, [_P]),
+    "rhd_profile_count": (c_uint32, [_P]),
+    "rhd_profile_entry": (c_int, [_P, c_uint32, POINTER(c_char_p), POINTER(ctypes.c_double),
+                                  POINTER(c_uint64), POINTER(c_uint64)]),
 }
 
 _lib = None

@@ -301,7 +301,6 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     __shared__ float s_alt[BH_ROWS * BH_PITCH];
     __shared__ float s_cfa[BH_ROWS * BH_PITCH];
     __shared__ float s_out[BH_ROWS * BH_PITCH];
-    __shared__ float s_cds[BH_ROWS * BH_PITCH];
     float* hcd = t.full(HCD);
     const float* alt = t.full(HALT);
     const float* cfa = t.full(CFA);
@@ -319,7 +318,7 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
             bool inside = col >= 4 && col < TS - 4;
             s_hcd[r * BH_PITCH + c] = inside ? hcd[g] : 0.0f;
             s_alt[r * BH_PITCH + c] = inside ? alt[g] : 0.0f;
-            s_cfa[r * BH_PITCH + c] = cfa[g];
+            s_cfa[r * BH_PITCH + c] = col < TS ? cfa[g] : 0.0f;
         }
         __syncthreads();
         if (lane < rows) {
@@ -354,7 +353,6 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
         }
         __syncthreads();
     }
-    (void)s_cds;
 }
 
 // ---------------------------------------------------------------- weights, Nyquist test, diagonals
@@ -800,22 +798,34 @@ size_t amaze_scratch_bytes_per_tile() { return TILE_FLOATS * sizeof(float) + 4 *
 
 // mod.rs:136-161: the stage sequence for `tiles` tiles starting at
 // a.tile_base; a.scratch and a.areas hold one slot per tile of this launch.
-hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s) {
+hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* prof) {
     const float clip_pt8 = 0.8f * a.clip_point;
     const dim3 px_grid(TS / 64, TS / 4, tiles), pair_grid(TSH / 64, TS / 4, tiles), blk(256);
-    hipLaunchKernelGGL(amz_load, px_grid, blk, 0, s, a);
-    hipLaunchKernelGGL(amz_gradients, px_grid, blk, 0, s, a);
-    hipLaunchKernelGGL(amz_colour_differences, px_grid, blk, 0, s, a, clip_pt8);
-    hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a, a.clip_point);
-    hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles), dim3(64), 0,
-                       s, a, a.clip_point);
-    hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point);
-    hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a);
-    hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a);
-    hipLaunchKernelGGL(amz_chains, dim3(tiles), dim3(TSH), 0, s, a);
-    hipLaunchKernelGGL(amz_refine, pair_grid, blk, 0, s, a, a.clip_point);
-    hipLaunchKernelGGL(amz_chroma, pair_grid, blk, 0, s, a);
-    hipLaunchKernelGGL(amz_finish, dim3(VALID / 64 + 1, VALID / 4, tiles), blk, 0, s, a);
+    // Algorithmic bytes: the planes each stage must read and write once, per
+    // tile (f = full-plane floats per tile, h = half-plane floats per tile).
+    const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
+    RHD_TIMED(prof, s, "amz_load", f + f + h, hipLaunchKernelGGL(amz_load, px_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_gradients", f + 3 * f + 4 * h,
+              hipLaunchKernelGGL(amz_gradients, px_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_colour_differences", 3 * f + 6 * f,
+              hipLaunchKernelGGL(amz_colour_differences, px_grid, blk, 0, s, a, clip_pt8));
+    RHD_TIMED(prof, s, "amz_bound_v", 3 * f + f,
+              hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a,
+                                 a.clip_point));
+    RHD_TIMED(prof, s, "amz_bound_h", 4 * f + f + f / 2,
+              hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
+                                 dim3(64), 0, s, a, a.clip_point));
+    RHD_TIMED(prof, s, "amz_weights", 8 * f / 2 + 5 * h + 4 * h,
+              hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point));
+    RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_chains", 3 * f / 2 + 6 * h + 6 * h,
+              hipLaunchKernelGGL(amz_chains, dim3(tiles), dim3(TSH), 0, s, a));
+    RHD_TIMED(prof, s, "amz_refine", f / 2 + 2 * f / 2 + 5 * h + 3 * h,
+              hipLaunchKernelGGL(amz_refine, pair_grid, blk, 0, s, a, a.clip_point));
+    RHD_TIMED(prof, s, "amz_chroma", 2 * h + h, hipLaunchKernelGGL(amz_chroma, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_finish", 12ull * a.width * a.height * tiles / (a.tiles_x * a.tiles_y) + f / 2 + 4 * h,
+              hipLaunchKernelGGL(amz_finish, dim3(VALID / 64 + 1, VALID / 4, tiles), blk, 0, s, a));
     return hipGetLastError();
 }
 

@@ -43,7 +43,73 @@ struct rhd_context {
     DeviceStats* h_stats = nullptr;
     DeviceBuffer amaze, amaze_areas, rcd, capture, opposed;
     hipEvent_t ev[5] = {};
+    bool profiling = false;
+    Profiler profiler;
 };
+
+#define PROF(ctx) ((ctx)->profiling ? &(ctx)->profiler : nullptr)
+
+namespace rhd {
+
+hipEvent_t Profiler::take_event() {
+    if (!free_.empty()) {
+        hipEvent_t e = free_.back();
+        free_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void Profiler::begin(hipStream_t s, const char* name, uint64_t bytes) {
+    size_t k = 0;
+    while (k < totals.size() && totals[k].name != name) ++k;
+    if (k == totals.size()) totals.push_back(Total{name, 0.0, 0, 0});
+    open_ = take_event();
+    open_total_ = k;
+    open_bytes_ = bytes;
+    if (open_) (void)hipEventRecord(open_, s);
+}
+
+void Profiler::end(hipStream_t s) {
+    hipEvent_t stop = take_event();
+    if (!open_ || !stop) return;
+    (void)hipEventRecord(stop, s);
+    pending_.push_back(Pending{open_total_, open_, stop, open_bytes_});
+    open_ = nullptr;
+    if (pending_.size() > 4096) collect();
+}
+
+void Profiler::collect() {
+    for (const Pending& p : pending_) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.stop) == hipSuccess &&
+            hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            totals[p.total].ms += ms;
+            totals[p.total].launches += 1;
+            totals[p.total].bytes = p.bytes;
+        }
+        free_.push_back(p.start);
+        free_.push_back(p.stop);
+    }
+    pending_.clear();
+}
+
+void Profiler::reset() {
+    collect();
+    totals.clear();
+}
+
+Profiler::~Profiler() {
+    for (const Pending& p : pending_) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+}
+
+}  // namespace rhd
 
 struct rhd_mosaic {
     rhd_context* ctx;
@@ -359,7 +425,7 @@ rhd_status run_opposed(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc
     a.scratch = ctx->opposed.ptr;
     a.scratch_bytes = ctx->opposed.bytes;
     a.stats = ctx->d_stats;
-    hipError_t e = launch_opposed(a, ctx->stream);
+    hipError_t e = launch_opposed(a, ctx->stream, PROF(ctx));
     if (e != hipSuccess) return hip_fail(ctx, e, "opposed");
     return RHD_OK;
 }
@@ -392,15 +458,15 @@ rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sens
     switch (d->demosaic) {
         case RHD_DEMOSAIC_BILINEAR:
             e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d, ctx->d_stats,
-                                ctx->stream);
+                                ctx->stream, PROF(ctx));
             break;
         case RHD_DEMOSAIC_MHC:
-            e = launch_mhc(plane->d, w, h, pattern, src->d, ctx->d_stats, ctx->stream);
+            e = launch_mhc(plane->d, w, h, pattern, src->d, ctx->d_stats, ctx->stream, PROF(ctx));
             break;
         case RHD_DEMOSAIC_AMAZE: {
             if (w <= 32 || h <= 32) {
                 e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
-                                    ctx->d_stats, ctx->stream);
+                                    ctx->d_stats, ctx->stream, PROF(ctx));
                 break;
             }
             const int tx = (w + 223) / 224, ty = (h + 223) / 224, tiles = tx * ty;
@@ -428,14 +494,14 @@ rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sens
             a.stats = ctx->d_stats;
             for (int base = 0; base < tiles && e == hipSuccess; base += per_launch) {
                 a.tile_base = base;
-                e = launch_amaze(a, std::min(per_launch, tiles - base), ctx->stream);
+                e = launch_amaze(a, std::min(per_launch, tiles - base), ctx->stream, PROF(ctx));
             }
             break;
         }
         case RHD_DEMOSAIC_RCD: {
             if (w <= 24 || h <= 24) {
                 e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
-                                    ctx->d_stats, ctx->stream);
+                                    ctx->d_stats, ctx->stream, PROF(ctx));
                 break;
             }
             const int tx = (w + 173) / 174, ty = (h + 173) / 174, tiles = tx * ty;
@@ -446,7 +512,7 @@ rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sens
             for (int base = 0; base < tiles && e == hipSuccess; base += per_launch)
                 e = launch_rcd(plane->d, w, h, pattern, b.forward, b.inverse, (float*)ctx->rcd.ptr,
                                base, std::min(per_launch, tiles - base), src->d, ctx->d_stats,
-                               ctx->stream);
+                               ctx->stream, PROF(ctx));
             break;
         }
         default:
@@ -492,7 +558,7 @@ rhd_status run_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
     a.est_b = base + 3 * n;
     a.tmp = base + 4 * n;
     a.stats = ctx->d_stats;
-    hipError_t e = launch_capture(a, ctx->stream);
+    hipError_t e = launch_capture(a, ctx->stream, PROF(ctx));
     if (e != hipSuccess) return hip_fail(ctx, e, "capture");
     return RHD_OK;
 }
@@ -646,7 +712,7 @@ rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_s
     a.crop_pattern = plane->pattern;
     a.mode = 0;
     RHD_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream);
+    hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream, PROF(ctx));
     if (e != hipSuccess) {
         rhd_plane_release(plane);
         return hip_fail(ctx, e, "normalize");
@@ -815,7 +881,7 @@ rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
     st = reset_stats(ctx);
     step(hipEventRecord(ctx->ev[0], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK) {
-        hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream);
+        hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream, PROF(ctx));
         if (e != hipSuccess) st = hip_fail(ctx, e, "normalize");
     }
     step(hipEventRecord(ctx->ev[1], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
@@ -963,6 +1029,37 @@ void rhd_source_release(rhd_source* s) {
 rhd_status rhd_synchronize(rhd_context* ctx) {
     if (!ctx) return RHD_INVALID_INPUT;
     RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RHD_OK;
+}
+
+rhd_status rhd_profile_enable(rhd_context* ctx, int enable) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    ctx->profiling = enable != 0;
+    return RHD_OK;
+}
+
+rhd_status rhd_profile_reset(rhd_context* ctx) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    ctx->profiler.reset();
+    return RHD_OK;
+}
+
+uint32_t rhd_profile_count(rhd_context* ctx) {
+    if (!ctx) return 0;
+    ctx->profiler.collect();
+    return (uint32_t)ctx->profiler.totals.size();
+}
+
+rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,
+                             uint64_t* launches, uint64_t* bytes_per_launch) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    ctx->profiler.collect();
+    if (index >= ctx->profiler.totals.size()) return fail(ctx, RHD_INVALID_INPUT, "no such entry");
+    const Profiler::Total& t = ctx->profiler.totals[index];
+    if (name) *name = t.name.c_str();
+    if (total_ms) *total_ms = t.ms;
+    if (launches) *launches = t.launches;
+    if (bytes_per_launch) *bytes_per_launch = t.bytes;
     return RHD_OK;
 }
 

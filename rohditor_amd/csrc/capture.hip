@@ -207,32 +207,40 @@ size_t capture_scratch_bytes(int width, int height) {
     return (size_t)width * height * sizeof(float) * 5;
 }
 
-hipError_t launch_capture(const CaptureArgs& a, hipStream_t s) {
+hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     if (a.radius < 0 || a.radius > MAXR) return hipErrorInvalidValue;
     const int n = a.width * a.height;
+    const uint64_t px = (uint64_t)n;
     Weights k{};
     for (int t = 0; t <= 2 * a.radius; ++t) k.w[t] = a.weights[t];
     const int blocks = (n + THREADS - 1) / THREADS;
-    hipLaunchKernelGGL(cap_init, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n, a.ceilings[0],
-                       a.ceilings[1], a.ceilings[2], a.guide, a.mask, a.stats);
+    RHD_TIMED(prof, s, "cap_init", 20 * px,
+              hipLaunchKernelGGL(cap_init, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n,
+                                 a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide, a.mask,
+                                 a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
     // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
-    hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.mask, a.mask, a.tmp, a.width,
-                       a.height, a.radius, k, 0, a.noise_protection);
-    hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.guide, a.tmp, a.mask, a.width,
-                       a.height, a.radius, k, 1, a.noise_protection);
+    RHD_TIMED(prof, s, "cap_blur", 12 * px,
+              hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.mask, a.mask, a.tmp,
+                                 a.width, a.height, a.radius, k, 0, a.noise_protection));
+    RHD_TIMED(prof, s, "cap_blur", 12 * px,
+              hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.guide, a.tmp, a.mask,
+                                 a.width, a.height, a.radius, k, 1, a.noise_protection));
     hipMemcpyAsync(a.est_a, a.guide, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s);
     float* cur = a.est_a;
     float* next = a.est_b;
     for (int it = 0; it < a.iterations; ++it) {
-        hipLaunchKernelGGL(cap_iterate, tiles, dim3(THREADS), 0, s, cur, a.guide, next, a.width,
-                           a.height, a.radius, k);
+        // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
+        RHD_TIMED(prof, s, "cap_iterate", 12 * px,
+                  hipLaunchKernelGGL(cap_iterate, tiles, dim3(THREADS), 0, s, cur, a.guide, next,
+                                     a.width, a.height, a.radius, k));
         float* t = cur;
         cur = next;
         next = t;
     }
-    hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n, a.amount, a.guide,
-                       a.mask, cur);
+    RHD_TIMED(prof, s, "cap_apply", 36 * px,
+              hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n, a.amount,
+                                 a.guide, a.mask, cur));
     return hipGetLastError();
 }
 

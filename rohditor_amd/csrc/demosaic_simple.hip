@@ -140,19 +140,21 @@ __global__ __launch_bounds__(256) void mhc_kernel(Src s, float* __restrict__ out
 
 hipError_t launch_bilinear(const float* mosaic, int width, int height, int pattern,
                            const float forward[3], const float gains[3], float* out,
-                           DeviceStats* stats, hipStream_t s) {
+                           DeviceStats* stats, hipStream_t s, Profiler* prof) {
     Src src{mosaic, width, height, pattern, {forward[0], forward[1], forward[2]}};
     dim3 grid((width + 63) / 64, (height + 3) / 4);
-    hipLaunchKernelGGL(bilinear_kernel, grid, dim3(256), 0, s, src, gains[0], gains[1], gains[2],
-                       out, stats);
+    RHD_TIMED(prof, s, "bilinear", 16ull * width * height,
+              hipLaunchKernelGGL(bilinear_kernel, grid, dim3(256), 0, s, src, gains[0], gains[1],
+                                 gains[2], out, stats));
     return hipGetLastError();
 }
 
 hipError_t launch_mhc(const float* mosaic, int width, int height, int pattern, float* out,
-                      DeviceStats* stats, hipStream_t s) {
+                      DeviceStats* stats, hipStream_t s, Profiler* prof) {
     Src src{mosaic, width, height, pattern, {1.0f, 1.0f, 1.0f}};
     dim3 grid((width + 63) / 64, (height + 3) / 4);
-    hipLaunchKernelGGL(mhc_kernel, grid, dim3(256), 0, s, src, out, stats);
+    RHD_TIMED(prof, s, "mhc", 16ull * width * height,
+              hipLaunchKernelGGL(mhc_kernel, grid, dim3(256), 0, s, src, out, stats));
     return hipGetLastError();
 }
 

@@ -113,9 +113,10 @@ __global__ __launch_bounds__(BLOCK) void classify_kernel(float* __restrict__ pla
 }  // namespace
 
 hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
-                            hipStream_t s) {
+                            hipStream_t s, Profiler* prof) {
     dim3 grid((a.width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), a.height);
-    hipLaunchKernelGGL(normalize_kernel, grid, dim3(BLOCK), 0, s, a, out, stats);
+    RHD_TIMED(prof, s, "normalize", 6ull * a.width * a.height,
+              hipLaunchKernelGGL(normalize_kernel, grid, dim3(BLOCK), 0, s, a, out, stats));
     return hipGetLastError();
 }
 

@@ -242,7 +242,8 @@ size_t rcd_scratch_bytes_per_tile() { return TILE_FLOATS * sizeof(float); }
 
 hipError_t launch_rcd(const float* mosaic, int width, int height, int pattern,
                       const float forward[3], const float inverse[3], float* scratch,
-                      int tile_base, int tiles, float* out, DeviceStats* stats, hipStream_t s) {
+                      int tile_base, int tiles, float* out, DeviceStats* stats, hipStream_t s,
+                      Profiler* prof) {
     Args a{};
     a.mosaic = mosaic;
     a.width = width;
@@ -258,13 +259,14 @@ hipError_t launch_rcd(const float* mosaic, int width, int height, int pattern,
     a.out = out;
     a.stats = stats;
     const dim3 grid((TS + 63) / 64, (TS + 3) / 4, tiles), blk(256);
-    hipLaunchKernelGGL(rcd_load, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_stats, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_green, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_directions, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_opposite, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_at_green, grid, blk, 0, s, a);
-    hipLaunchKernelGGL(rcd_write, grid, blk, 0, s, a);
+    const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
+    RHD_TIMED(prof, s, "rcd_load", f + 4 * f + 3 * h, hipLaunchKernelGGL(rcd_load, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_stats", f + f + 3 * h, hipLaunchKernelGGL(rcd_stats, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_green", f + f + h + f / 2, hipLaunchKernelGGL(rcd_green, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_directions", 3 * h, hipLaunchKernelGGL(rcd_directions, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_opposite", h + 2 * f, hipLaunchKernelGGL(rcd_opposite, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_at_green", 4 * f, hipLaunchKernelGGL(rcd_at_green, grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "rcd_write", 4 * f, hipLaunchKernelGGL(rcd_write, grid, blk, 0, s, a));
     return hipGetLastError();
 }
 

@@ -6,11 +6,53 @@
 
 #include <atomic>
 #include <string>
+#include <vector>
 
 #include "../../include/rhd.h"
 #include "rhd_device.hpp"
 
 namespace rhd {
+
+// Optional per-launch timing: HIP events recorded on the launch stream around
+// one kernel, with the kernel's algorithmic bytes per launch. Launchers take a
+// nullable pointer; the C ABI enables it for benchmarking only.
+class Profiler {
+   public:
+    void begin(hipStream_t s, const char* name, uint64_t bytes);
+    void end(hipStream_t s);
+    // Resolves recorded launches into the per-name totals (synchronizes).
+    void collect();
+    void reset();
+    ~Profiler();
+
+    struct Total {
+        std::string name;
+        double ms = 0.0;
+        uint64_t launches = 0;
+        uint64_t bytes = 0;  // algorithmic bytes of the latest launch
+    };
+    std::vector<Total> totals;
+
+   private:
+    struct Pending {
+        size_t total;
+        hipEvent_t start, stop;
+        uint64_t bytes;
+    };
+    hipEvent_t take_event();
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> free_;
+    hipEvent_t open_ = nullptr;
+    size_t open_total_ = 0;
+    uint64_t open_bytes_ = 0;
+};
+
+#define RHD_TIMED(prof, stream, name, bytes, ...)          \
+    do {                                                   \
+        if (prof) (prof)->begin((stream), (name), (bytes)); \
+        __VA_ARGS__;                                       \
+        if (prof) (prof)->end((stream));                   \
+    } while (0)
 
 // Normalization level table: black level and (white - black) per position in
 // the combined repeat period of the black pattern and the sensor CFA.
@@ -46,22 +88,23 @@ struct AmazeArgs {
 
 // Launchers (each returns the first HIP error, or hipSuccess).
 hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
-                            hipStream_t s);
+                            hipStream_t s, Profiler* prof);
 hipError_t launch_clip(float* plane, int width, int height, int pattern, const float limits[3],
                        DeviceStats* stats, hipStream_t s);
 hipError_t launch_detect(const float* plane, int width, int height, int pattern,
                          const float levels[3], DeviceStats* stats, hipStream_t s);
 hipError_t launch_bilinear(const float* mosaic, int width, int height, int pattern,
                            const float forward[3], const float gains[3], float* out,
-                           DeviceStats* stats, hipStream_t s);
+                           DeviceStats* stats, hipStream_t s, Profiler* prof);
 hipError_t launch_mhc(const float* mosaic, int width, int height, int pattern, float* out,
-                      DeviceStats* stats, hipStream_t s);
+                      DeviceStats* stats, hipStream_t s, Profiler* prof);
 size_t amaze_scratch_bytes_per_tile();
-hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s);
+hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* prof);
 size_t rcd_scratch_bytes_per_tile();
 hipError_t launch_rcd(const float* mosaic, int width, int height, int pattern,
                       const float forward[3], const float inverse[3], float* scratch,
-                      int tile_base, int tiles, float* out, DeviceStats* stats, hipStream_t s);
+                      int tile_base, int tiles, float* out, DeviceStats* stats, hipStream_t s,
+                      Profiler* prof);
 
 struct CaptureArgs {
     float* rgb;  // planar, 3 * n
@@ -79,7 +122,7 @@ struct CaptureArgs {
     DeviceStats* stats;
 };
 size_t capture_scratch_bytes(int width, int height);
-hipError_t launch_capture(const CaptureArgs& a, hipStream_t s);
+hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof);
 
 struct OpposedArgs {
     float* mosaic;
@@ -92,6 +135,6 @@ struct OpposedArgs {
     DeviceStats* stats;
 };
 size_t opposed_scratch_bytes(int width, int height);
-hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s);
+hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);
 
 }  // namespace rhd
