@@ -251,47 +251,82 @@ __device__ __forceinline__ float spread(float a, float b, float c) {
 }
 
 // green.rs:184-226, vertical half: vcd is bounded in place going down each
-// column; the value two rows up is already bounded when read.
+// column; the value two rows up is already bounded when read. Rows are
+// processed four at a time with the next four rows' inputs loaded up front,
+// so the loop waits for memory once per four rows.
 __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
     const Tile t = tile_of(a, blockIdx.z);
     const int cc = blockIdx.x * 64 + threadIdx.x;
     if (cc < 4 || cc >= TS - 4) return;
-    float* vcd = t.full(VCD);
-    const float* alt = t.full(VALT);
-    const float* cfa = t.full(CFA);
+    float* __restrict__ vcd = t.full(VCD);
+    const float* __restrict__ alt = t.full(VALT);
+    const float* __restrict__ cfa = t.full(CFA);
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded vcd two and one rows up (rows 2, 3 are zero)
     float a_m2 = 0.0f, a_m1 = 0.0f;  // vcdalt two and one rows up
     float u0 = vcd[4 * TS + cc], u1 = vcd[5 * TS + cc];  // unbounded vcd at rr, rr + 1
     float a0 = alt[4 * TS + cc], a1 = alt[5 * TS + cc];
+    float c_m1 = cfa[3 * TS + cc], c0 = cfa[4 * TS + cc];
     bool green = (fc(a.pattern, 4, cc) & 1) == 1;
-    for (int rr = 4; rr < TS - 4; ++rr) {
-        const int i = rr * TS + cc;
-        float u2 = rr + 2 < TS - 4 ? vcd[i + 2 * TS] : 0.0f;
-        float a2 = rr + 2 < TS - 4 ? alt[i + 2 * TS] : 0.0f;
-        float vcdvar = spread(b_m2, u0, u2);
-        float vcdaltvar = spread(a_m2, a0, a2);
-        float v = vcdaltvar < vcdvar ? a0 : u0;
-        float cur = cfa[i];
-        float bounded = green ? bound_at_green(v, cur, cfa[i - TS], cfa[i + TS], clip_pt)
-                              : bound_at_red_blue(v, cur, cfa[i - TS], cfa[i + TS], clip_pt);
-        vcd[i] = bounded;
-        b_m2 = b_m1;
-        b_m1 = bounded;
-        a_m2 = a_m1;
-        a_m1 = a0;
-        u0 = u1;
-        u1 = u2;
-        a0 = a1;
-        a1 = a2;
-        green = !green;
+    for (int rr = 4; rr < TS - 4; rr += 4) {
+        float U[4], A[4], C[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int r2 = rr + 2 + k;
+            U[k] = r2 < TS - 4 ? vcd[r2 * TS + cc] : 0.0f;
+            A[k] = r2 < TS - 4 ? alt[r2 * TS + cc] : 0.0f;
+            C[k] = cfa[(rr + 1 + k) * TS + cc];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float u2 = U[k], a2 = A[k], c1 = C[k];
+            float vcdvar = spread(b_m2, u0, u2);
+            float vcdaltvar = spread(a_m2, a0, a2);
+            float v = vcdaltvar < vcdvar ? a0 : u0;
+            float bounded = green ? bound_at_green(v, c0, c_m1, c1, clip_pt)
+                                  : bound_at_red_blue(v, c0, c_m1, c1, clip_pt);
+            vcd[(rr + k) * TS + cc] = bounded;
+            b_m2 = b_m1;
+            b_m1 = bounded;
+            a_m2 = a_m1;
+            a_m1 = a0;
+            u0 = u1;
+            u1 = u2;
+            a0 = a1;
+            a1 = a2;
+            c_m1 = c0;
+            c0 = c1;
+            green = !green;
+        }
     }
 }
 
 // green.rs:184-226, horizontal half: hcd is bounded in place along each row,
 // and cddiffsq is formed at red/blue sites from the final vcd and hcd. One
 // thread per tile row; 32-column chunks are staged through LDS so global
-// reads and writes stay coalesced.
+// reads and writes stay coalesced, with each plane's chunk loaded in one
+// batch of independent loads.
 constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_W = BH_CHUNK + 4, BH_PITCH = BH_W + 1;
+constexpr int BH_PER_LANE = BH_ROWS * BH_W / 64;  // 36
+
+__device__ __forceinline__ void bh_stage(float* lds, const float* __restrict__ plane, int r0,
+                                         int c0, int rows, bool zero_outside) {
+    float v[BH_PER_LANE];
+#pragma unroll
+    for (int j = 0; j < BH_PER_LANE; ++j) {
+        const int e = threadIdx.x + 64 * j;
+        const int r = e / BH_W, c = e % BH_W;
+        const int col = c0 - 2 + c;
+        const bool inside = zero_outside ? (col >= 4 && col < TS - 4) : (col < TS);
+        v[j] = (r < rows && inside) ? plane[(r0 + r) * TS + col] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < BH_PER_LANE; ++j) {
+        const int e = threadIdx.x + 64 * j;
+        const int r = e / BH_W, c = e % BH_W;
+        lds[r * BH_PITCH + c] = v[j];
+    }
+}
+
 __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     const Tile t = tile_of(a, blockIdx.z);
     const int r0 = 4 + blockIdx.x * BH_ROWS;
@@ -310,16 +345,10 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded hcd two and one columns left (cols 2, 3 are zero)
     bool green = rr < TS - 4 ? (fc(a.pattern, rr, 4) & 1) == 1 : false;
     for (int c0 = 4; c0 < TS - 4; c0 += BH_CHUNK) {
-        // Stage columns c0 - 2 .. c0 + 33 (hcd, hcdalt, cfa) for `rows` rows.
-        for (int k = lane; k < rows * BH_W; k += 64) {
-            int r = k / BH_W, c = k % BH_W;
-            int col = c0 - 2 + c;
-            size_t g = (size_t)(r0 + r) * TS + col;
-            bool inside = col >= 4 && col < TS - 4;
-            s_hcd[r * BH_PITCH + c] = inside ? hcd[g] : 0.0f;
-            s_alt[r * BH_PITCH + c] = inside ? alt[g] : 0.0f;
-            s_cfa[r * BH_PITCH + c] = col < TS ? cfa[g] : 0.0f;
-        }
+        // Columns c0 - 2 .. c0 + 33 of hcd, hcdalt and the CFA for `rows` rows.
+        bh_stage(s_hcd, hcd, r0, c0, rows, true);
+        bh_stage(s_alt, alt, r0, c0, rows, true);
+        bh_stage(s_cfa, cfa, r0, c0, rows, false);
         __syncthreads();
         if (lane < rows) {
             const int base = lane * BH_PITCH + 2;  // LDS column of c0
@@ -341,15 +370,25 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
         }
         __syncthreads();
         // Write the chunk back coalesced; cddiffsq at red/blue sites.
-        for (int k = lane; k < rows * BH_CHUNK; k += 64) {
-            int r = k / BH_CHUNK, c = k % BH_CHUNK;
-            int col = c0 + c;
-            if (col >= TS - 4) continue;
-            int row = r0 + r;
-            size_t g = (size_t)row * TS + col;
-            float h = s_out[r * BH_PITCH + c];
-            hcd[g] = h;
-            if ((fc(a.pattern, row, col) & 1) == 0) cds[g] = sqr(vcd[g] - h);
+        constexpr int OUT_PER_LANE = BH_ROWS * BH_CHUNK / 64;
+        float vv[OUT_PER_LANE];
+#pragma unroll
+        for (int j = 0; j < OUT_PER_LANE; ++j) {
+            const int e = lane + 64 * j;
+            const int r = e / BH_CHUNK, c = e % BH_CHUNK;
+            const int col = c0 + c, row = r0 + r;
+            const bool ok = r < rows && col < TS - 4;
+            vv[j] = ok && (fc(a.pattern, row, col) & 1) == 0 ? vcd[row * TS + col] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < OUT_PER_LANE; ++j) {
+            const int e = lane + 64 * j;
+            const int r = e / BH_CHUNK, c = e % BH_CHUNK;
+            const int col = c0 + c, row = r0 + r;
+            if (r >= rows || col >= TS - 4) continue;
+            const float h = s_out[r * BH_PITCH + c];
+            hcd[row * TS + col] = h;
+            if ((fc(a.pattern, row, col) & 1) == 0) cds[row * TS + col] = sqr(vv[j] - h);
         }
         __syncthreads();
     }
@@ -589,49 +628,79 @@ __global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
     float* rbint = t.half(RBINT);
     hv_prev[p] = hvwt[7 * TSH + p];
     pm_prev[p] = pmwt[9 * TSH + p];
+    // Inputs of one row. Every value is final before the kernel starts except
+    // the current row's weights, which only this row's pass writes, so the
+    // next row's inputs are loaded while the current row is processed.
+    struct RowIn {
+        float hv, hv_dl, hv_dr, vcd, hcd, cfa, cfa_l, cfa_r, cfa_u, cfa_d;
+        float pm, pm_dl, pm_dr, rbm, rbp;
+        uint8_t nyq;
+    };
+    auto load = [&](int rr) {
+        RowIn in{};
+        const int cc = 2 * p + gae(a.pattern, rr);
+        const int h = rr * TSH + p, down = (rr + 1) * TSH;
+        const int i = rr * TS + min(max(cc, 1), TS - 2);
+        in.hv = hvwt[h];
+        in.hv_dl = hvwt[down + ((cc - 1) >> 1)];
+        in.hv_dr = hvwt[down + min((cc + 1) >> 1, TSH - 1)];
+        in.vcd = vcd[i];
+        in.hcd = hcd[i];
+        in.cfa = cfa[i];
+        in.cfa_l = cfa[i - 1];
+        in.cfa_r = cfa[i + 1];
+        in.cfa_u = cfa[i - TS];
+        in.cfa_d = cfa[i + TS];
+        in.nyq = nyq2[h];
+        in.pm = pmwt[h];
+        in.pm_dl = pmwt[down + ((cc - 1) >> 1)];
+        in.pm_dr = pmwt[down + min((cc + 1) >> 1, TSH - 1)];
+        in.rbm = rbm[h];
+        in.rbp = rbp[h];
+        return in;
+    };
+    RowIn cur = load(8);
     __syncthreads();
     for (int rr = 8; rr < TS - 8; ++rr) {
+        const RowIn next = load(rr + 1 < TS - 8 ? rr + 1 : rr);
         const int cc = 2 * p + gae(a.pattern, rr);
         const int h = rr * TSH + p;
-        float hv = hvwt[h];
+        float hv = cur.hv;
         if (cc >= 8 && cc < TS - 8) {
-            float alt = (hv_prev[(cc - 1) >> 1] + hv_prev[(cc + 1) >> 1] +
-                         hvwt[h + TSH - p + ((cc - 1) >> 1)] + hvwt[h + TSH - p + ((cc + 1) >> 1)]) /
+            float alt = (hv_prev[(cc - 1) >> 1] + hv_prev[(cc + 1) >> 1] + cur.hv_dl + cur.hv_dr) /
                         4.0f;
             if (fabsf(0.5f - hv) < fabsf(0.5f - alt)) {
                 hv = alt;
                 hvwt[h] = hv;
             }
-            const int i = rr * TS + cc;
-            float dg = intp(hv, vcd[i], hcd[i]);
-            float green = cfa[i] + dg;
+            float dg = intp(hv, cur.vcd, cur.hcd);
+            float green = cur.cfa + dg;
             dgrb0[h] = dg;
             grb[h] = green;
-            bool flagged = nyq2[h] != 0;
-            d2h[h] = flagged ? sqr(green - 0.5f * (cfa[i - 1] + cfa[i + 1])) : 0.0f;
-            d2v[h] = flagged ? sqr(green - 0.5f * (cfa[i - TS] + cfa[i + TS])) : 0.0f;
+            bool flagged = cur.nyq != 0;
+            d2h[h] = flagged ? sqr(green - 0.5f * (cur.cfa_l + cur.cfa_r)) : 0.0f;
+            d2v[h] = flagged ? sqr(green - 0.5f * (cur.cfa_u + cur.cfa_d)) : 0.0f;
         }
         float pm = 0.0f;
         const bool pm_row = rr >= 10 && rr < TS - 10;
         if (pm_row) {
-            pm = pmwt[h];
+            pm = cur.pm;
             if (cc >= 10 && cc < TS - 10) {
-                float alt = (pm_prev[(cc - 1) >> 1] + pm_prev[(cc + 1) >> 1] +
-                             pmwt[h + TSH - p + ((cc - 1) >> 1)] +
-                             pmwt[h + TSH - p + ((cc + 1) >> 1)]) /
+                float alt = (pm_prev[(cc - 1) >> 1] + pm_prev[(cc + 1) >> 1] + cur.pm_dl +
+                             cur.pm_dr) /
                             4.0f;
                 if (fabsf(0.5f - pm) < fabsf(0.5f - alt)) {
                     pm = alt;
                     pmwt[h] = pm;
                 }
-                const int i = rr * TS + cc;
-                rbint[h] = 0.5f * (cfa[i] + rbm[h] * (1.0f - pm) + rbp[h] * pm);
+                rbint[h] = 0.5f * (cur.cfa + cur.rbm * (1.0f - pm) + cur.rbp * pm);
             }
         }
         __syncthreads();
         hv_prev[p] = hv;
         if (pm_row) pm_prev[p] = pm;
         __syncthreads();
+        cur = next;
     }
 }
 

@@ -45,7 +45,21 @@ struct rhd_context {
     hipEvent_t ev[5] = {};
     bool profiling = false;
     Profiler profiler;
+    std::atomic<int> refs{1};  // rhd_create's reference plus one per live handle
 };
+
+namespace {
+void ctx_unref(rhd_context* ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+    if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+}  // namespace
 
 #define PROF(ctx) ((ctx)->profiling ? &(ctx)->profiler : nullptr)
 
@@ -344,6 +358,7 @@ rhd_status new_plane(rhd_context* ctx, uint32_t w, uint32_t h, int32_t pattern, 
     void* p = nullptr;
     rhd_status st = device_alloc(ctx, bytes, &p);
     if (st != RHD_OK) return st;
+    ctx->refs.fetch_add(1);
     *out = new rhd_plane{ctx, (float*)p, w, h, pattern, bytes};
     return RHD_OK;
 }
@@ -353,6 +368,7 @@ rhd_status new_source(rhd_context* ctx, uint32_t w, uint32_t h, rhd_source** out
     void* p = nullptr;
     rhd_status st = device_alloc(ctx, bytes, &p);
     if (st != RHD_OK) return st;
+    ctx->refs.fetch_add(1);
     *out = new rhd_source{ctx, (float*)p, w, h, bytes};
     return RHD_OK;
 }
@@ -600,17 +616,14 @@ rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
     return RHD_OK;
 }
 
+// The context outlives rhd_destroy while handles made from it are alive: each
+// handle holds a reference, and the last release frees the stream and stats.
 void rhd_destroy(rhd_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_scratch(ctx);
-    for (auto& ev : ctx->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (ctx->d_stats) (void)hipFree(ctx->d_stats);
-    if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_unref(ctx);
 }
 
 rhd_status rhd_set_budget(rhd_context* ctx, uint64_t budget_bytes) {
@@ -674,6 +687,7 @@ rhd_status rhd_mosaic_upload(rhd_context* ctx, const uint16_t* host, uint64_t ro
         unreserve(ctx, bytes);
         return hip_fail(ctx, e, "mosaic upload");
     }
+    ctx->refs.fetch_add(1);
     *out = new rhd_mosaic{ctx, (uint16_t*)p, row_stride, width, height, bytes};
     return RHD_OK;
 }
@@ -683,6 +697,7 @@ void rhd_mosaic_release(rhd_mosaic* m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     (void)hipFree(m->d);
     unreserve(m->ctx, m->bytes);
+    ctx_unref(m->ctx);
     delete m;
 }
 
@@ -984,6 +999,7 @@ void rhd_plane_release(rhd_plane* p) {
     (void)hipStreamSynchronize(p->ctx->stream);
     (void)hipFree(p->d);
     unreserve(p->ctx, p->bytes);
+    ctx_unref(p->ctx);
     delete p;
 }
 
@@ -1023,6 +1039,7 @@ void rhd_source_release(rhd_source* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     (void)hipFree(s->d);
     unreserve(s->ctx, s->bytes);
+    ctx_unref(s->ctx);
     delete s;
 }
 

@@ -55,133 +55,171 @@ __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rg
 }
 
 // Horizontal taps at image column x of an LDS row whose first element is
-// image column `col0`; `border` selects the reference's mirrored `.sum()`.
-__device__ __forceinline__ float hblur(const float* row, int col0, int x, int width, int r,
-                                       const Weights& k, bool fast) {
-    bool border = width <= 2 * r || x < r || x >= width - r;
+// image column `col0`; the reference's mirrored border `.sum()` starts at -0.0.
+template <int R, bool FAST>
+__device__ __forceinline__ float hblur(const float* row, int col0, int x, int width,
+                                       const Weights& k) {
+    const bool border = width <= 2 * R || x < R || x >= width - R;
     float v = border ? -0.0f : 0.0f;
-    if (fast) {
-        for (int t = 0; t <= 2 * r; ++t) v += row[x + t - r - col0] * k.w[t];
-    } else {
-        for (int t = 0; t <= 2 * r; ++t) v += row[mirror_half(x + t - r, width) - col0] * k.w[t];
+#pragma unroll
+    for (int t = 0; t <= 2 * R; ++t) {
+        const int src = FAST ? x + t - R : mirror_half(x + t - R, width);
+        v += row[src - col0] * k.w[t];
     }
     return v;
 }
 
 // Vertical taps at image row y over an LDS plane whose first row is image row
-// `row0` and pitch `pitch`, at LDS column `c`.
+// `row0` (pitch `pitch`), at LDS column `c`.
+template <int R, bool FAST>
 __device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, int y, int c,
-                                       int height, int r, const Weights& k, bool fast) {
+                                       int height, const Weights& k) {
     float v = 0.0f;
-    if (fast) {
-        for (int t = 0; t <= 2 * r; ++t) v += plane[(y + t - r - row0) * pitch + c] * k.w[t];
-    } else {
-        for (int t = 0; t <= 2 * r; ++t)
-            v += plane[(mirror_half(y + t - r, height) - row0) * pitch + c] * k.w[t];
+#pragma unroll
+    for (int t = 0; t <= 2 * R; ++t) {
+        const int src = FAST ? y + t - R : mirror_half(y + t - R, height);
+        v += plane[(src - row0) * pitch + c] * k.w[t];
     }
     return v;
+}
+
+// Threads walk a (rows x cols) LDS window as 4 rows of 64 lanes.
+#define FOR_WINDOW(rows, cols, r, c)                                   \
+    for (int r = (int)(threadIdx.x >> 6); r < (rows); r += THREADS / 64) \
+        for (int c = (int)(threadIdx.x & 63); c < (cols); c += 64)
+
+// Copies a (rows x cols) window of a global plane into LDS. The trip counts
+// are compile-time bounds (MAXROWS x MAXCOLS), so every load of the window is
+// issued before the first LDS store and the memory latency is paid once.
+template <int MAXROWS, int MAXCOLS>
+__device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ src,
+                                      size_t src_pitch, int rows, int cols) {
+    constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
+    const int ty = threadIdx.x >> 6, tx = threadIdx.x & 63;
+    float v[RI * CI];
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < CI; ++j) {
+            const int r = ty + 4 * i, c = tx + 64 * j;
+            v[i * CI + j] = (r < rows && c < cols) ? src[(size_t)r * src_pitch + c] : 0.0f;
+        }
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < CI; ++j) {
+            const int r = ty + 4 * i, c = tx + 64 * j;
+            if (r < rows && c < cols) lds[r * pitch + c] = v[i * CI + j];
+        }
 }
 
 // One blur of `in` with an epilogue:
 //   mode 0: out = min(aux, b^4)                          (sharpening.rs:227-237)
 //   mode 1: out = aux * detail gate(guide = in, b)       (sharpening.rs:238-262)
-__global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
-                                                    const float* __restrict__ aux,
-                                                    float* __restrict__ out, int width,
-                                                    int height, int r, Weights k, int mode,
-                                                    float noise_protection) {
-    __shared__ float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
-    __shared__ float s_h[(TH + 2 * MAXR) * TW];
+template <int R, bool FAST>
+__device__ void blur_tile(const float* __restrict__ in, const float* __restrict__ aux,
+                          float* __restrict__ out, int width, int height, const Weights& k,
+                          int mode, float noise_protection, float* s_in, float* s_h) {
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const int cx0 = max(0, x0 - r), cx1 = min(width, x0 + TW + r);
-    const int ry0 = max(0, y0 - r), ry1 = min(height, y0 + TH + r);
+    const int cx0 = max(0, x0 - R), cx1 = min(width, x0 + TW + R);
+    const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
     const int pw = cx1 - cx0, ph = ry1 - ry0;
-    const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
-    const bool fast = x0 - r >= 0 && x0 + TW + r <= width && y0 - r >= 0 && y0 + TH + r <= height;
-    for (int e = threadIdx.x; e < pw * ph; e += THREADS) {
-        int rr = e / pw, cc = e % pw;
-        s_in[rr * pw + cc] = in[(size_t)(ry0 + rr) * width + cx0 + cc];
-    }
+    const int ow = min(width, x0 + TW) - x0, oh = min(height, y0 + TH) - y0;
+    stage<TH + 2 * MAXR, TW + 2 * MAXR>(s_in, pw, in + (size_t)ry0 * width + cx0, width, ph, pw);
     __syncthreads();
-    const int ow = ox1 - x0;
-    for (int e = threadIdx.x; e < ow * ph; e += THREADS) {
-        int rr = e / ow, cc = e % ow;
-        s_h[rr * TW + cc] = hblur(s_in + rr * pw, cx0, x0 + cc, width, r, k, fast);
-    }
+    FOR_WINDOW(ph, ow, r, c) s_h[r * TW + c] = hblur<R, FAST>(s_in + r * pw, cx0, x0 + c, width, k);
     __syncthreads();
-    for (int e = threadIdx.x; e < ow * (oy1 - y0); e += THREADS) {
-        int rr = e / ow, cc = e % ow;
-        int y = y0 + rr, x = x0 + cc;
-        float b = vblur(s_h, TW, ry0, y, cc, height, r, k, fast);
-        size_t i = (size_t)y * width + x;
+    FOR_WINDOW(oh, ow, r, c) {
+        const int y = y0 + r, x = x0 + c;
+        const float b = vblur<R, FAST>(s_h, TW, ry0, y, c, height, k);
+        const size_t i = (size_t)y * width + x;
         if (mode == 0) {
             out[i] = num_min(aux[i], (b * b) * (b * b));
         } else {
-            float g = in[i];
-            float threshold = 0.0005f + noise_protection * 0.008f + b * noise_protection * 0.02f;
-            float detail = fabsf(g - b);
+            const float g = in[i];
+            const float threshold = 0.0005f + noise_protection * 0.008f + b * noise_protection * 0.02f;
+            const float detail = fabsf(g - b);
             out[i] = aux[i] * (smoothstep(threshold, threshold * 3.0f, detail) *
                                smoothstep(0.001f, 0.01f, g));
         }
     }
 }
 
-// One RL iteration (sharpening.rs:268-304) for a 64x32 tile.
+template <int R>
+__global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
+                                                    const float* __restrict__ aux,
+                                                    float* __restrict__ out, int width,
+                                                    int height, Weights k, int mode,
+                                                    float noise_protection) {
+    __shared__ float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
+    __shared__ float s_h[(TH + 2 * MAXR) * TW];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const bool fast = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
+    if (fast)
+        blur_tile<R, true>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+    else
+        blur_tile<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+}
+
+// One RL iteration (sharpening.rs:268-304) for a 64x32 tile: blur(estimate)
+// over the tile widened by r, ratio = guide / max(b, floor), blur(ratio) over
+// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide).
 constexpr int E_W = TW + 4 * MAXR, E_H = TH + 4 * MAXR;
 constexpr int H1_W = TW + 2 * MAXR;
 constexpr int R_W = TW + 2 * MAXR, R_H = TH + 2 * MAXR;
+
+template <int R, bool FAST>
+__device__ void iterate_tile(const float* __restrict__ est_in, const float* __restrict__ guide,
+                             float* __restrict__ est_out, int width, int height, const Weights& k,
+                             float* s_e, float* s_h1, float* s_r, float* s_h2, float* s_g) {
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
+    const int ex0 = max(0, x0 - 2 * R), ex1 = min(width, x0 + TW + 2 * R);
+    const int ey0 = max(0, y0 - 2 * R), ey1 = min(height, y0 + TH + 2 * R);
+    const int rx0 = max(0, x0 - R), rx1 = min(width, x0 + TW + R);
+    const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
+    const int ew = ex1 - ex0, eh = ey1 - ey0, rw = rx1 - rx0, rh = ry1 - ry0;
+    const int ow = ox1 - x0, oh = oy1 - y0;
+    stage<E_H, E_W>(s_e, E_W, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
+    stage<R_H, R_W>(s_g, R_W, guide + (size_t)ry0 * width + rx0, width, rh, rw);
+    __syncthreads();
+    FOR_WINDOW(eh, rw, r, c) s_h1[r * H1_W + c] = hblur<R, FAST>(s_e + r * E_W, ex0, rx0 + c, width, k);
+    __syncthreads();
+    FOR_WINDOW(rh, rw, r, c) {
+        const int y = ry0 + r, x = rx0 + c;
+        const float b = vblur<R, FAST>(s_h1, H1_W, ey0, y, c, height, k);
+        (void)x;
+        s_r[r * R_W + c] = s_g[r * R_W + c] / num_max(b, FLOOR);
+    }
+    __syncthreads();
+    FOR_WINDOW(rh, ow, r, c) s_h2[r * TW + c] = hblur<R, FAST>(s_r + r * R_W, rx0, x0 + c, width, k);
+    __syncthreads();
+    FOR_WINDOW(oh, ow, r, c) {
+        const int y = y0 + r, x = x0 + c;
+        const float b = vblur<R, FAST>(s_h2, TW, ry0, y, c, height, k);
+        const float g = s_g[(y - ry0) * R_W + (x - rx0)];
+        const float v = s_e[(y - ey0) * E_W + (x - ex0)] * b;
+        const float lo = g * 0.5f, hi = g * 2.0f;
+        est_out[(size_t)y * width + x] = v < lo ? lo : (v > hi ? hi : v);
+    }
+}
+
+template <int R>
 __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__ est_in,
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
-                                                       int height, int r, Weights k) {
+                                                       int height, Weights k) {
     __shared__ float s_e[E_H * E_W];
-    __shared__ float s_h1[E_H * H1_W];
+    __shared__ float s_h1[E_H * H1_W];  // horizontal blur of the estimate, then of the ratio
     __shared__ float s_r[R_H * R_W];
-    __shared__ float s_h2[R_H * TW];
+    __shared__ float s_g[R_H * R_W];
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
-    // Estimate: image window widened by 2r; ratio window by r.
-    const int ex0 = max(0, x0 - 2 * r), ex1 = min(width, x0 + TW + 2 * r);
-    const int ey0 = max(0, y0 - 2 * r), ey1 = min(height, y0 + TH + 2 * r);
-    const int rx0 = max(0, x0 - r), rx1 = min(width, x0 + TW + r);
-    const int ry0 = max(0, y0 - r), ry1 = min(height, y0 + TH + r);
-    const int ew = ex1 - ex0, eh = ey1 - ey0, rw = rx1 - rx0, rh = ry1 - ry0, ow = ox1 - x0;
     const bool fast =
-        x0 - 2 * r >= 0 && x0 + TW + 2 * r <= width && y0 - 2 * r >= 0 && y0 + TH + 2 * r <= height;
-    for (int e = threadIdx.x; e < ew * eh; e += THREADS) {
-        int rr = e / ew, cc = e % ew;
-        s_e[rr * E_W + cc] = est_in[(size_t)(ey0 + rr) * width + ex0 + cc];
-    }
-    __syncthreads();
-    // Horizontal blur of the estimate over the ratio columns, all estimate rows.
-    for (int e = threadIdx.x; e < rw * eh; e += THREADS) {
-        int rr = e / rw, cc = e % rw;
-        s_h1[rr * H1_W + cc] = hblur(s_e + rr * E_W, ex0, rx0 + cc, width, r, k, fast);
-    }
-    __syncthreads();
-    // Vertical blur -> ratio over the ratio window.
-    for (int e = threadIdx.x; e < rw * rh; e += THREADS) {
-        int rr = e / rw, cc = e % rw;
-        int y = ry0 + rr, x = rx0 + cc;
-        float b = vblur(s_h1, H1_W, ey0, y, cc, height, r, k, fast);
-        s_r[rr * R_W + cc] = guide[(size_t)y * width + x] / num_max(b, FLOOR);
-    }
-    __syncthreads();
-    // Horizontal blur of the ratio over the output columns, all ratio rows.
-    for (int e = threadIdx.x; e < ow * rh; e += THREADS) {
-        int rr = e / ow, cc = e % ow;
-        s_h2[rr * TW + cc] = hblur(s_r + rr * R_W, rx0, x0 + cc, width, r, k, fast);
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ow * (oy1 - y0); e += THREADS) {
-        int rr = e / ow, cc = e % ow;
-        int y = y0 + rr, x = x0 + cc;
-        float b = vblur(s_h2, TW, ry0, y, cc, height, r, k, fast);
-        float g = guide[(size_t)y * width + x];
-        float v = s_e[(y - ey0) * E_W + (x - ex0)] * b;
-        float lo = g * 0.5f, hi = g * 2.0f;
-        est_out[(size_t)y * width + x] = v < lo ? lo : (v > hi ? hi : v);
-    }
+        x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
+    if (fast)
+        iterate_tile<R, true>(est_in, guide, est_out, width, height, k, s_e, s_h1, s_r, s_h1, s_g);
+    else
+        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_e, s_h1, s_r, s_h1, s_g);
 }
 
 // sharpening.rs:105-126
@@ -208,7 +246,7 @@ size_t capture_scratch_bytes(int width, int height) {
 }
 
 hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
-    if (a.radius < 0 || a.radius > MAXR) return hipErrorInvalidValue;
+    if (a.radius < 1 || a.radius > MAXR) return hipErrorInvalidValue;
     const int n = a.width * a.height;
     const uint64_t px = (uint64_t)n;
     Weights k{};
@@ -219,21 +257,32 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
                                  a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide, a.mask,
                                  a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
+    // The radius is a template parameter so the tap loops unroll.
+    auto blur = [&](const float* in, const float* aux, float* out, int mode) {
+        switch (a.radius) {
+            case 1: hipLaunchKernelGGL(cap_blur<1>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
+            case 2: hipLaunchKernelGGL(cap_blur<2>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
+            case 3: hipLaunchKernelGGL(cap_blur<3>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
+            default: hipLaunchKernelGGL(cap_blur<4>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
+        }
+    };
+    auto iterate = [&](const float* in, float* out) {
+        switch (a.radius) {
+            case 1: hipLaunchKernelGGL(cap_iterate<1>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
+            case 2: hipLaunchKernelGGL(cap_iterate<2>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
+            case 3: hipLaunchKernelGGL(cap_iterate<3>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
+            default: hipLaunchKernelGGL(cap_iterate<4>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
+        }
+    };
     // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
-    RHD_TIMED(prof, s, "cap_blur", 12 * px,
-              hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.mask, a.mask, a.tmp,
-                                 a.width, a.height, a.radius, k, 0, a.noise_protection));
-    RHD_TIMED(prof, s, "cap_blur", 12 * px,
-              hipLaunchKernelGGL(cap_blur, tiles, dim3(THREADS), 0, s, a.guide, a.tmp, a.mask,
-                                 a.width, a.height, a.radius, k, 1, a.noise_protection));
+    RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.mask, a.mask, a.tmp, 0));
+    RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.guide, a.tmp, a.mask, 1));
     hipMemcpyAsync(a.est_a, a.guide, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s);
     float* cur = a.est_a;
     float* next = a.est_b;
     for (int it = 0; it < a.iterations; ++it) {
         // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
-        RHD_TIMED(prof, s, "cap_iterate", 12 * px,
-                  hipLaunchKernelGGL(cap_iterate, tiles, dim3(THREADS), 0, s, cur, a.guide, next,
-                                     a.width, a.height, a.radius, k));
+        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, next));
         float* t = cur;
         cur = next;
         next = t;

@@ -19,10 +19,11 @@
 //              and the ring flags (label != 0, no clipped channel)
 //   scan       ring cells compacted in raster order, plus each region's
 //              first/last ring index
-//   corrections one workgroup per group (0 = image-wide pool, r = region):
-//              both passes over its ring entries in raster order, each
-//              slot accumulated sequentially in f64 by its own lane, so the
-//              sums are bit-identical to the reference's
+//   contrib    every ring cell's nine correction samples, in parallel
+//   sums       one wavefront per (group, slot), group 0 = image-wide pool,
+//              group r = region r: both passes over its ring entries in
+//              raster order, accumulated one f64 add at a time, so the sums
+//              are bit-identical to the reference's
 //   sites      one thread per photosite: the per-site estimate, the bound,
 //              the neutral ceiling, and the statistics
 // cbrt and log2 are evaluated in double and rounded to f32 (correctly
@@ -48,6 +49,10 @@ struct Layout {
     unsigned* rank;                // root rank, then scratch
     unsigned* block_sums;
     unsigned* ring_cell;
+    unsigned* ring_label;          // label of each ring entry
+    unsigned* contrib_flags;       // per ring entry: valid slots, then valid ratios << 9
+    float* contrib_value;          // per ring entry: nine correction samples
+    float* contrib_ratio;          // per ring entry: three two-channel support ratios
     unsigned* region_clipped;      // per label
     unsigned* region_wanted;
     unsigned* region_first;
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(THREADS) void opp_relabel(Layout L) {
 }
 
 // Zero the per-region arrays for labels 0..count.
-__global__ __launch_bounds__(THREADS) void opp_region_init(Layout L) {
+__global__ __launch_bounds__(THREADS) void opp_region_init(Layout L, DeviceStats* stats) {
     const unsigned count = min(L.scalars[0], (unsigned)L.cap - 1);
     for (unsigned r = blockIdx.x * THREADS + threadIdx.x; r <= count; r += gridDim.x * THREADS) {
         L.region_clipped[r] = 0;
@@ -345,7 +350,10 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L) {
         L.region_first[r] = NONE;
         L.region_last[r] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) L.scalars[2] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        L.scalars[2] = 0;
+        stats->regions = L.scalars[0];
+    }
 }
 
 // regions.rs:44-55 (clipped channels per region) and region_correction.rs:
@@ -353,15 +361,16 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L) {
 __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float l1, float l2) {
     int i = blockIdx.x * THREADS + threadIdx.x;
     const float lv[3] = {l0, l1, l2};
-    unsigned any = 0;
+    const int lane = threadIdx.x & 63;
+    unsigned lab = 0, clipped = 0, bits = 0;
+    bool has = false;
     if (i < L.cells) {
-        unsigned lab = L.label[i];
+        lab = L.label[i];
         float4 cell = L.cell[i];
         unsigned f = flags_of(cell);
-        unsigned clipped = f >> 3;
+        clipped = f >> 3;
         if (lab != 0 && lab < (unsigned)L.cap && clipped != 0) {
-            atomicOr(L.region_clipped + lab, clipped);
-            unsigned bits = 0;
+            has = true;
             for (int c = 0; c < 3; ++c) {
                 if (!(clipped & (1u << c))) continue;
                 bool has_clean = false;
@@ -371,118 +380,150 @@ __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float
                 int p = pattern_position(c, support_mask(cell, c, lv));
                 if (p >= 0) bits |= 1u << (c * 3 + p);
             }
-            if (bits) atomicOr(L.region_wanted + lab, bits);
-            any = bits;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) any |= __shfl_xor(any, off, 64);
-    if ((threadIdx.x & 63) == 0 && any) atomicOr(L.scalars + 2, any);
+    // Neighbouring cells share their region, so OR the bits per label across
+    // the wavefront and let one lane per distinct label touch memory; skip the
+    // atomic when every bit is already set.
+    unsigned long long active = __ballot(has);
+    unsigned any = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const unsigned target = __shfl(lab, leader, 64);
+        const bool mine = has && lab == target;
+        unsigned vc = mine ? clipped : 0u, vw = mine ? bits : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            vc |= __shfl_xor(vc, off, 64);
+            vw |= __shfl_xor(vw, off, 64);
+        }
+        if (lane == leader) {
+            if ((L.region_clipped[target] | vc) != L.region_clipped[target])
+                atomicOr(L.region_clipped + target, vc);
+            if (vw && (L.region_wanted[target] | vw) != L.region_wanted[target])
+                atomicOr(L.region_wanted + target, vw);
+        }
+        any |= vw;
+        active &= ~__ballot(mine);
+    }
+    if (lane == 0 && any && (L.scalars[2] | any) != L.scalars[2]) atomicOr(L.scalars + 2, any);
 }
 
 // ---------------------------------------------------------------- corrections
 // region_correction.rs:53-110 (Slot) and :196-261 (two raster-order passes).
-struct Slot {
-    unsigned count, kept;
-    double accum, second;
-    float mean, tolerance;
-};
-
-constexpr int CHUNK = THREADS;
-
-__global__ __launch_bounds__(THREADS) void opp_corrections(Layout L, float l0, float l1, float l2) {
+//
+// The per-sample correction of a ring cell does not depend on which
+// accumulator receives it, so opp_contrib computes every ring cell's nine
+// (channel, support set) samples once, in parallel. opp_sums then gives each
+// (group, slot) pair one wavefront: lanes load 64 consecutive ring entries,
+// and the accumulation walks the valid ones in raster order, one f64 add at a
+// time, exactly as the reference's sequential loop does.
+__global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float l1, float l2) {
     const float lv[3] = {l0, l1, l2};
-    __shared__ float s_value[CHUNK][SLOTS];
-    __shared__ float s_ratio[CHUNK][SLOTS];
-    __shared__ unsigned s_flags[CHUNK];  // bit s: slot s has a sample; bit 9+s: it has a ratio
+    const unsigned ring = L.scalars[1];
+    const unsigned any = L.scalars[2];
+    for (unsigned k = blockIdx.x * THREADS + threadIdx.x; k < ring; k += gridDim.x * THREADS) {
+        const unsigned i = L.ring_cell[k];
+        const unsigned lab = L.label[i];
+        const float4 cell = L.cell[i];
+        const unsigned rc = lab < (unsigned)L.cap ? L.region_clipped[lab] : 0u;
+        unsigned flags = 0;
+        for (int c = 0; c < 3; ++c) {
+            if (!(rc & (1u << c))) continue;
+            if (mean_of(cell, c) < 0.5f * lv[c]) continue;
+            for (int p = 0; p < 3; ++p) {
+                const int off = c * 3 + p;
+                if (!(any & (1u << off))) continue;
+                float value;
+                if (!correction(cell, c, support_pattern(c, p), lv, value)) continue;
+                L.contrib_value[(size_t)k * SLOTS + off] = value;
+                flags |= 1u << off;
+                if (p == 2) {
+                    float ratio;
+                    if (support_log_ratio(cell, support_pattern(c, 2), ratio)) {
+                        L.contrib_ratio[(size_t)k * 3 + c] = ratio;
+                        flags |= 1u << (SLOTS + c);
+                    }
+                }
+            }
+        }
+        L.ring_label[k] = lab;
+        L.contrib_flags[k] = flags;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void opp_sums(Layout L) {
+    const int lane = threadIdx.x & 63;
     const unsigned count = min(L.scalars[0], (unsigned)L.cap - 1);
     const unsigned ring = L.scalars[1];
-    const unsigned any_wanted = L.scalars[2];
-    for (unsigned g = blockIdx.x; g <= count; g += gridDim.x) {
-        unsigned begin, end;
-        unsigned wanted;
-        if (g == 0) {
-            begin = 0;
-            end = ring;
-            wanted = any_wanted;
-        } else {
+    const unsigned any = L.scalars[2];
+    const unsigned waves = gridDim.x * (THREADS / 64);
+    for (unsigned w = blockIdx.x * (THREADS / 64) + (threadIdx.x >> 6); w < (count + 1) * SLOTS;
+         w += waves) {
+        const unsigned g = w / SLOTS;
+        const int s = (int)(w % SLOTS), c = s / 3, p = s % 3;
+        unsigned begin = 0, end = ring, use = any;
+        if (g != 0) {
             begin = L.region_first[g];
-            end = begin == NONE ? 0 : L.region_last[g] + 1;
+            end = begin == NONE ? 0u : L.region_last[g] + 1u;
             if (begin == NONE) begin = 0;
-            wanted = L.region_wanted[g];
+            use = L.region_wanted[g];
         }
-        Slot slot{0, 0, 0.0, 0.0, 0.0f, 0.0f};
-        const int s = threadIdx.x;  // lanes 0..8 own one slot each
-        for (int pass = 0; pass < 2; ++pass) {
-            for (unsigned base = begin; base < end; base += CHUNK) {
-                const unsigned k = base + threadIdx.x;
-                unsigned fl = 0;
-                if (k < end) {
-                    unsigned i = L.ring_cell[k];
-                    unsigned lab = L.label[i];
-                    if ((g == 0 && lab < (unsigned)L.cap) || lab == g) {
-                        float4 cell = L.cell[i];
-                        unsigned region_clipped = L.region_clipped[lab];
-                        unsigned use = g == 0 ? any_wanted : wanted;
-                        for (int c = 0; c < 3; ++c) {
-                            if (!(region_clipped & (1u << c))) continue;
-                            if (mean_of(cell, c) < 0.5f * lv[c]) continue;
-                            for (int p = 0; p < 3; ++p) {
-                                int off = c * 3 + p;
-                                if (!(use & (1u << off))) continue;
-                                unsigned sup = support_pattern(c, p);
-                                float value;
-                                if (!correction(cell, c, sup, lv, value)) continue;
-                                float ratio = 0.0f;
-                                bool has_ratio = support_log_ratio(cell, sup, ratio);
-                                s_value[threadIdx.x][off] = value;
-                                s_ratio[threadIdx.x][off] = ratio;
-                                fl |= (1u << off) | (has_ratio ? 1u << (9 + off) : 0u);
-                            }
+        unsigned n = 0, kept = 0;
+        double accum = 0.0, second = 0.0;
+        float mean = 0.0f, tolerance = 0.0f;
+        if (use & (1u << s)) {
+            for (int pass = 0; pass < 2; ++pass) {
+                for (unsigned base = begin; base < end; base += 64) {
+                    const unsigned k = base + lane;
+                    bool valid = false;
+                    float v = 0.0f, r = 0.0f;
+                    if (k < end && (g == 0 || L.ring_label[k] == g)) {
+                        const unsigned f = L.contrib_flags[k];
+                        if (f & (1u << s)) {
+                            valid = true;
+                            v = L.contrib_value[(size_t)k * SLOTS + s];
+                            if (p == 2 && (f & (1u << (SLOTS + c))))
+                                r = L.contrib_ratio[(size_t)k * 3 + c];
                         }
                     }
-                }
-                s_flags[threadIdx.x] = fl;
-                __syncthreads();
-                if (s < SLOTS) {
-                    const unsigned n = min(end - base, (unsigned)CHUNK);
-                    for (unsigned j = 0; j < n; ++j) {
-                        unsigned f = s_flags[j];
-                        if (!(f & (1u << s))) continue;
-                        float value = s_value[j][s];
-                        double wide = (double)value;
+                    unsigned long long mask = __ballot(valid);
+                    while (mask) {
+                        const int j = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1;
+                        const float vj = __shfl(v, j, 64);
+                        const float rj = __shfl(r, j, 64);
+                        const double wide = (double)vj;
                         if (pass == 0) {
-                            slot.count += 1;
-                            slot.accum += wide;
-                            slot.second += wide * wide;
-                        } else if (fabsf(value - slot.mean) <= slot.tolerance) {
-                            slot.kept += 1;
-                            slot.accum += wide;
-                            slot.second += (double)((f & (1u << (9 + s))) ? s_ratio[j][s] : 0.0f);
+                            n += 1;
+                            accum += wide;
+                            second += wide * wide;
+                        } else if (fabsf(vj - mean) <= tolerance) {
+                            kept += 1;
+                            accum += wide;
+                            second += (double)rj;
                         }
                     }
                 }
-                __syncthreads();
-            }
-            if (pass == 0 && s < SLOTS) {
-                if (slot.count != 0) {
-                    double c = (double)slot.count;
-                    double mean = slot.accum / c;
-                    double variance = fmax(slot.second / c - mean * mean, 0.0);
-                    slot.mean = (float)mean;
-                    slot.tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
+                if (pass == 0) {
+                    if (n != 0) {
+                        const double cnt = (double)n;
+                        const double m = accum / cnt;
+                        const double variance = fmax(second / cnt - m * m, 0.0);
+                        mean = (float)m;
+                        tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
+                    }
+                    accum = 0.0;
+                    second = 0.0;
                 }
-                slot.accum = 0.0;
-                slot.second = 0.0;
             }
         }
-        if (s < SLOTS) {
-            // Slot::value: a sample count below the minimum or nothing kept
-            // leaves the slot empty; ratios are kept for every slot and used
-            // only when the support has two channels.
+        if (lane == 0) {
+            // Slot::value: fewer than the minimum samples or nothing kept
+            // leaves the slot empty.
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (slot.count >= MIN_BORDER_CELLS && slot.kept != 0) {
-                double kept = (double)slot.kept;
-                out = make_float4((float)(slot.accum / kept), (float)(slot.second / kept), 1.0f, 0.0f);
+            if (n >= MIN_BORDER_CELLS && kept != 0) {
+                const double kd = (double)kept;
+                out = make_float4((float)(accum / kd), (float)(second / kd), 1.0f, 0.0f);
             }
             L.corrections[(size_t)g * SLOTS + s] = out;
         }
@@ -667,6 +708,10 @@ Layout layout_of(const OpposedArgs& a) {
     L.rank = (unsigned*)take(cells * 4);
     L.block_sums = (unsigned*)take(scan_blocks * 4);
     L.ring_cell = (unsigned*)take(cells * 4);
+    L.ring_label = (unsigned*)take(cells * 4);
+    L.contrib_flags = (unsigned*)take(cells * 4);
+    L.contrib_value = (float*)take(cells * SLOTS * 4);
+    L.contrib_ratio = (float*)take(cells * 3 * 4);
     L.region_clipped = (unsigned*)take(cap * 4);
     L.region_wanted = (unsigned*)take(cap * 4);
     L.region_first = (unsigned*)take(cap * 4);
@@ -710,7 +755,7 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
                                  (int)ROOTS);
               hipLaunchKernelGGL(opp_relabel, dim3(cell_blocks), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_regions", 24 * cells,
-              hipLaunchKernelGGL(opp_region_init, dim3(256), dim3(THREADS), 0, s, L);
+              hipLaunchKernelGGL(opp_region_init, dim3(256), dim3(THREADS), 0, s, L, a.stats);
               hipLaunchKernelGGL(opp_regions, dim3(cell_blocks), dim3(THREADS), 0, s, L, l0, l1,
                                  l2);
               hipLaunchKernelGGL(opp_scan_count, dim3(scan_blocks), dim3(THREADS), 0, s, L,
@@ -720,7 +765,8 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_scan_emit, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)RING));
     RHD_TIMED(prof, s, "opp_corrections", 0,
-              hipLaunchKernelGGL(opp_corrections, dim3(1024), dim3(THREADS), 0, s, L, l0, l1, l2));
+              hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2);
+              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_sites", 8 * px,
               hipLaunchKernelGGL(opp_sites, dim3((L.w + 63) / 64, (L.h + 3) / 4), dim3(THREADS), 0,
                                  s, a.mosaic, L, a.pattern, l0, l1, l2, a.has_ceiling,
