@@ -114,16 +114,21 @@ def main():
     ms_per_step = total * 1e3 / args.steps
     value = mp * args.steps / total
 
-    # Dominant kernel: the one with the most GPU time in the timed region.
+    # Dominant kernel: the one with the most GPU time in the timed region among
+    # those with a host-known algorithmic byte count (the Opposed ring-entry
+    # kernels' sizes are known only on the device).
     prof = profile_table(proc)
-    dominant = max(prof, key=lambda k: prof[k][0])
+    top = max(prof, key=lambda k: prof[k][0])
+    dominant = max((k for k in prof if prof[k][2] > 0), key=lambda k: prof[k][0])
     dom_ms, dom_launches, dom_bytes = prof[dominant]
     avg_ms = dom_ms / max(dom_launches, 1)
     achieved = dom_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                 "kernel": dominant, "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": int(dom_bytes)}
+                "launches_per_step": dom_launches / args.steps,
+                "algorithmic_bytes_per_launch": int(dom_bytes),
+                "top_kernel": top, "top_kernel_ms_per_step": round(prof[top][0] / args.steps, 4)}
     stage = {"stage_GBps": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9, 1),
              "stage_frac": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
 

@@ -14,6 +14,12 @@
  * reference (each stage consumes its input); here a stage that consumes a
  * handle says so and releases it, even on error. All functions are safe to
  * call from one thread per context; contexts are independent.
+ *
+ * `cancel` (may be NULL) points at a one-byte flag laid out like the
+ * reference's CancellationToken, an Arc<AtomicBool> (crates/core/src/
+ * cancel.rs:13-33): nonzero means cancelled. It is read with acquire
+ * ordering between stages; a cancelled call publishes no handle and releases
+ * what it reserved.
  */
 #ifndef RHD_H
 #define RHD_H
@@ -160,31 +166,31 @@ void rhd_mosaic_release(rhd_mosaic* mosaic);
 
 /* GpuSensorProcessor::normalize (crates/gpu/src/sensor/normalize.rs:170). */
 rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_sensor_desc* desc,
-                         const volatile uint32_t* cancel, rhd_plane** out, rhd_metrics* metrics);
+                         const volatile uint8_t* cancel, rhd_plane** out, rhd_metrics* metrics);
 /* GpuSensorProcessor::apply_highlight (crates/gpu/src/sensor/highlight.rs:87),
  * in place on `plane`, which stays owned by the caller. */
 rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* desc,
-                         const volatile uint32_t* cancel, rhd_metrics* metrics);
+                         const volatile uint8_t* cancel, rhd_metrics* metrics);
 /* GpuSensorProcessor::demosaic (crates/gpu/src/sensor/demosaic.rs:74),
  * including capture sharpening when `detail` is active. Consumes `plane`. */
 rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* desc,
-                        const rhd_detail_desc* detail, const volatile uint32_t* cancel,
+                        const rhd_detail_desc* detail, const volatile uint8_t* cancel,
                         rhd_source** out, rhd_metrics* metrics);
 /* GpuCaptureProcessor::process_resident_from_resident
  * (crates/gpu/src/capture/mod.rs:54), in place on `source`. */
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* source, const rhd_detail_desc* detail,
-                       const volatile uint32_t* cancel, rhd_metrics* metrics);
+                       const volatile uint8_t* cancel, rhd_metrics* metrics);
 
 /* GpuSensorProcessor::develop (crates/gpu/src/sensor/develop.rs:68-142):
  * normalize -> highlight -> demosaic (+ capture) from a resident mosaic. */
 rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
                                 const rhd_sensor_desc* desc, const rhd_detail_desc* detail,
-                                const volatile uint32_t* cancel, rhd_source** out,
+                                const volatile uint8_t* cancel, rhd_source** out,
                                 rhd_metrics* metrics);
 /* The same from a host RawFrame mosaic (upload included). */
 rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t row_stride,
                        const rhd_sensor_desc* desc, const rhd_detail_desc* detail,
-                       const volatile uint32_t* cancel, rhd_source** out, rhd_metrics* metrics);
+                       const volatile uint8_t* cancel, rhd_source** out, rhd_metrics* metrics);
 
 /* Test and CPU-handoff readbacks (the reference's test readbacks). */
 rhd_status rhd_plane_read(rhd_context* ctx, const rhd_plane* plane, float* dst, uint64_t count);

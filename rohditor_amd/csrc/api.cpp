@@ -165,7 +165,9 @@ rhd_status hip_fail(rhd_context* ctx, hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail((ctx), _e, #call);    \
     } while (0)
 
-bool cancelled(const volatile uint32_t* flag) { return flag && *flag != 0; }
+bool cancelled(const volatile uint8_t* flag) {
+    return flag && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != 0;
+}
 
 rhd_status reserve(rhd_context* ctx, uint64_t bytes) {
     uint64_t cur = ctx->reserved.load();
@@ -702,7 +704,7 @@ void rhd_mosaic_release(rhd_mosaic* m) {
 }
 
 rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_sensor_desc* d,
-                         const volatile uint32_t* cancel, rhd_plane** out, rhd_metrics* m) {
+                         const volatile uint8_t* cancel, rhd_plane** out, rhd_metrics* m) {
     if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
     *out = nullptr;
     rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
@@ -748,7 +750,7 @@ rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_s
 }
 
 rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d,
-                         const volatile uint32_t* cancel, rhd_metrics* m) {
+                         const volatile uint8_t* cancel, rhd_metrics* m) {
     if (!ctx || !plane) return RHD_INVALID_INPUT;
     rhd_status st = check_handle_ctx(ctx, plane->ctx);
     if (st != RHD_OK) return st;
@@ -785,7 +787,7 @@ rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_de
 }
 
 rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d,
-                        const rhd_detail_desc* dd, const volatile uint32_t* cancel,
+                        const rhd_detail_desc* dd, const volatile uint8_t* cancel,
                         rhd_source** out, rhd_metrics* m) {
     if (!ctx || !out || !plane) {
         rhd_plane_release(plane);
@@ -834,7 +836,7 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
 }
 
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc* dd,
-                       const volatile uint32_t* cancel, rhd_metrics* m) {
+                       const volatile uint8_t* cancel, rhd_metrics* m) {
     if (!ctx || !src) return RHD_INVALID_INPUT;
     rhd_status st = check_handle_ctx(ctx, src->ctx);
     if (st != RHD_OK) return st;
@@ -855,7 +857,7 @@ rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
 
 rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
                                 const rhd_sensor_desc* d, const rhd_detail_desc* dd,
-                                const volatile uint32_t* cancel, rhd_source** out,
+                                const volatile uint8_t* cancel, rhd_source** out,
                                 rhd_metrics* m) {
     if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
     *out = nullptr;
@@ -934,7 +936,7 @@ rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
 
 rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t row_stride,
                        const rhd_sensor_desc* d, const rhd_detail_desc* dd,
-                       const volatile uint32_t* cancel, rhd_source** out, rhd_metrics* m) {
+                       const volatile uint8_t* cancel, rhd_source** out, rhd_metrics* m) {
     if (!ctx || !out || !d) return RHD_INVALID_INPUT;
     *out = nullptr;
     rhd_mosaic* mosaic = nullptr;

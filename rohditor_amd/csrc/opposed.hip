@@ -8,11 +8,13 @@
 // log2 chromaticity gate).
 //
 // Pipeline (all on the context stream, no host round trip):
-//   cells      one thread per 2x2 cell: means and flags (float4 per cell)
-//   dilate     fg = a clipped cell within Chebyshev distance 1
-//   union      concurrent union-find; every set's root is its smallest index,
-//              i.e. the component's first cell in raster order
-//   flatten    parent -> root
+//   cells      one thread per 2x2 cell: means and flags (float4 per cell), and
+//              per excluded channel the two-channel cube-root reference and
+//              log2 ratio of the other two channels
+//   label      per 64x16-cell tile in LDS: the Chebyshev-1 dilation and
+//              union-find of the dilated cells; then the unions across tile
+//              borders in global memory, and a flatten. Every set's root is
+//              its smallest index, i.e. the component's first raster cell
 //   scan       exclusive scan of the root flags -> compact label = rank + 1
 //              (the reference's numbering by first raster appearance)
 //   regions    per-region clipped channels and wanted slots (atomic OR),
@@ -20,12 +22,13 @@
 //   scan       ring cells compacted in raster order, plus each region's
 //              first/last ring index
 //   contrib    every ring cell's nine correction samples, in parallel
-//   sums       one wavefront per (group, slot), group 0 = image-wide pool,
-//              group r = region r: both passes over its ring entries in
-//              raster order, accumulated one f64 add at a time, so the sums
-//              are bit-identical to the reference's
-//   sites      one thread per photosite: the per-site estimate, the bound,
-//              the neutral ceiling, and the statistics
+//   sums       one wavefront per group (0 = image-wide pool, r = region r),
+//              lane s owning slot s: both passes over the group's ring
+//              entries in raster order, accumulated one f64 add at a time, so
+//              the sums are bit-identical to the reference's
+//   sites      one thread per (cell, channel) with a suspected site: the
+//              estimate, then per site the bound, the neutral ceiling and the
+//              statistics
 // cbrt and log2 are evaluated in double and rounded to f32 (correctly
 // rounded in practice); the reference calls the platform libm's cbrtf and
 // log2f, which are within 1 ulp of that. Results therefore agree with the
@@ -45,6 +48,9 @@ constexpr unsigned MIN_BORDER_CELLS = 3;
 struct Layout {
     int w, h, cw, ch, cells, cap;  // cap: region-count bound (+1)
     float4* cell;                  // means (x, y, z), flags as bits in w
+    float* pair_ref;               // [3][cells]: cube-root reference of the two channels
+                                   // other than the plane's index, NaN when undefined
+    float* pair_lr;                // [3][cells]: log2 ratio of those two channels, or NaN
     unsigned* label;               // parent during labelling, then compact label
     unsigned* rank;                // root rank, then scratch
     unsigned* block_sums;
@@ -84,9 +90,33 @@ __device__ __forceinline__ unsigned support_mask(const float4& cell, int target,
 __device__ __forceinline__ float cbrt_f32(float v) { return (float)cbrt((double)v); }
 __device__ __forceinline__ float log2_f32(float v) { return (float)log2((double)v); }
 
-// support.rs:51-72
-__device__ __forceinline__ bool reference(const float4& cell, unsigned support, float& out) {
-    if (__popc(support) == 1) {
+// The two-channel parts of support.rs:51-72 (reference) and :101-110
+// (support_log_ratio) depend only on a cell's means, and a two-channel support
+// set of a target channel is always the other two channels. opp_cells
+// therefore evaluates them once per cell and excluded channel; NaN marks the
+// cases where the reference returns None.
+__device__ __forceinline__ void pair_values(const float4& cell, int excluded, float& ref,
+                                            float& lr) {
+    const int a = excluded == 0 ? 1 : 0, b = excluded == 2 ? 1 : 2;
+    const float ma = mean_of(cell, a), mb = mean_of(cell, b);
+    const float nan = __int_as_float(0x7fc00000);
+    ref = nan;
+    if (is_finite(ma) && ma > 0.0f && is_finite(mb) && mb > 0.0f) {
+        float sum = 0.0f;
+        sum += cbrt_f32(ma);
+        sum += cbrt_f32(mb);
+        const float root = sum / 2.0f;
+        ref = root * root * root;
+    }
+    const float ratio = ma / mb;
+    lr = is_finite(ratio) && ratio > 0.0f ? log2_f32(ratio) : nan;
+}
+
+// support.rs:51-72, with the two-channel value taken from pair_values.
+__device__ __forceinline__ bool reference(const float4& cell, unsigned support, float pair_ref,
+                                          float& out) {
+    const int n = __popc(support);
+    if (n == 1) {
         float v = mean_of(cell, __ffs(support) - 1);
         if (is_finite(v) && v > 0.0f) {
             out = v;
@@ -94,41 +124,27 @@ __device__ __forceinline__ bool reference(const float4& cell, unsigned support, 
         }
         return false;
     }
-    float sum = 0.0f;
-    unsigned count = 0;
-    for (int c = 0; c < 3; ++c) {
-        if (!(support & (1u << c))) continue;
-        float v = mean_of(cell, c);
-        if (!is_finite(v) || v <= 0.0f) return false;
-        sum += cbrt_f32(v);
-        ++count;
-    }
-    if (count == 0) return false;
-    float root = sum / (float)count;
-    out = root * root * root;
+    if (n != 2 || pair_ref != pair_ref) return false;
+    out = pair_ref;
     return true;
 }
 // support.rs:78-95
 __device__ __forceinline__ bool correction(const float4& cell, int target, unsigned support,
-                                           const float* lv, float& out) {
+                                           float pair_ref, const float* lv, float& out) {
     if (!bright_clean(cell, target, lv)) return false;
     for (int c = 0; c < 3; ++c)
         if ((support & (1u << c)) && !bright_clean(cell, c, lv)) return false;
     float ref;
-    if (!reference(cell, support, ref)) return false;
+    if (!reference(cell, support, pair_ref, ref)) return false;
     float v = mean_of(cell, target) - ref;
     if (!is_finite(v)) return false;
     out = v;
     return true;
 }
-// support.rs:101-110
-__device__ __forceinline__ bool support_log_ratio(const float4& cell, unsigned support, float& out) {
-    if (__popc(support) != 2) return false;
-    int first = __ffs(support) - 1;
-    int second = __ffs(support & ~(1u << first)) - 1;
-    float ratio = mean_of(cell, first) / mean_of(cell, second);
-    if (!(is_finite(ratio) && ratio > 0.0f)) return false;
-    out = log2_f32(ratio);
+// support.rs:101-110, with the ratio's log2 taken from pair_values.
+__device__ __forceinline__ bool support_log_ratio(unsigned support, float pair_lr, float& out) {
+    if (__popc(support) != 2 || pair_lr != pair_lr) return false;
+    out = pair_lr;
     return true;
 }
 // support.rs:116-125
@@ -178,41 +194,38 @@ __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m
             means[c] = sums[c] / (float)counts[c];
             flags |= 1u << c;
         }
-    L.cell[i] = make_float4(means[0], means[1], means[2], __uint_as_float(flags));
+    const float4 cell = make_float4(means[0], means[1], means[2], __uint_as_float(flags));
+    L.cell[i] = cell;
+    for (int e = 0; e < 3; ++e) {
+        float ref, lr;
+        pair_values(cell, e, ref, lr);
+        L.pair_ref[(size_t)e * L.cells + i] = ref;
+        L.pair_lr[(size_t)e * L.cells + i] = lr;
+    }
 }
 
-// regions.rs:126-183: Chebyshev radius-1 dilation of the clipped cells.
-__global__ __launch_bounds__(THREADS) void opp_dilate(Layout L) {
-    int i = blockIdx.x * THREADS + threadIdx.x;
-    if (i >= L.cells) return;
-    int cx = i % L.cw, cy = i / L.cw;
-    bool any = false;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-            int x = cx + dx, y = cy + dy;
-            if (x < 0 || y < 0 || x >= L.cw || y >= L.ch) continue;
-            any |= (flags_of(L.cell[y * L.cw + x]) >> 3) != 0;
-        }
-    L.label[i] = any ? (unsigned)i : NONE;
-}
-
+// Union-find with union by linking the larger root under the smaller one, so
+// every set's root is its smallest member: the component's first cell in
+// raster order, which is the order the reference numbers its labels in.
+// Parents are read with relaxed atomic loads at the scope the array lives at.
+template <int SCOPE>
 __device__ __forceinline__ unsigned load_parent(unsigned* p, unsigned x) {
-    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, SCOPE);
 }
+template <int SCOPE>
 __device__ __forceinline__ unsigned find_root(unsigned* p, unsigned x) {
-    unsigned y = load_parent(p, x);
+    unsigned y = load_parent<SCOPE>(p, x);
     while (y != x) {
         x = y;
-        y = load_parent(p, x);
+        y = load_parent<SCOPE>(p, x);
     }
     return x;
 }
-// Union by linking the larger root under the smaller one, so every set's root
-// is its smallest member.
+template <int SCOPE>
 __device__ void merge(unsigned* p, unsigned a, unsigned b) {
     while (true) {
-        a = find_root(p, a);
-        b = find_root(p, b);
+        a = find_root<SCOPE>(p, a);
+        b = find_root<SCOPE>(p, b);
         if (a == b) return;
         if (a > b) {
             unsigned t = a;
@@ -225,18 +238,93 @@ __device__ void merge(unsigned* p, unsigned a, unsigned b) {
     }
 }
 
-// 8-connected unions with the already-visited neighbours W, NW, N, NE
-// (regions.rs:200-214).
-__global__ __launch_bounds__(THREADS) void opp_union(Layout L) {
-    int i = blockIdx.x * THREADS + threadIdx.x;
-    if (i >= L.cells || L.label[i] == NONE) return;
-    int cx = i % L.cw, cy = i / L.cw;
+// Labelling tiles: LX x LY cells per workgroup.
+constexpr int LX = 64, LY = 16;
+constexpr int LHX = LX + 2, LHY = LY + 2;
+
+// regions.rs:126-183 (Chebyshev radius-1 dilation of the clipped cells) and
+// regions.rs:188-252 (8-connected unions with the already-visited neighbours
+// W, NW, N, NE), inside one tile in LDS. Each cell's parent leaves as the
+// global index of its tile-local root, which is the smallest index of its
+// tile-local component; cells outside the mask get NONE.
+__global__ __launch_bounds__(THREADS) void opp_label_tiles(Layout L) {
+    __shared__ unsigned char s_clip[LHY * LHX];
+    __shared__ unsigned s_par[LY * LX];
+    const int x0 = blockIdx.x * LX, y0 = blockIdx.y * LY;
+    for (int k = threadIdx.x; k < LHY * LHX; k += THREADS) {
+        const int x = x0 + k % LHX - 1, y = y0 + k / LHX - 1;
+        bool clipped = false;
+        if (x >= 0 && y >= 0 && x < L.cw && y < L.ch)
+            clipped = (flags_of(L.cell[(size_t)y * L.cw + x]) >> 3) != 0;
+        s_clip[k] = clipped;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
+        const int lx = k % LX, ly = k / LX;
+        bool fg = false;
+        if (x0 + lx < L.cw && y0 + ly < L.ch)
+            for (int dy = 0; dy < 3; ++dy)
+                for (int dx = 0; dx < 3; ++dx) fg |= s_clip[(ly + dy) * LHX + lx + dx] != 0;
+        s_par[k] = fg ? (unsigned)k : NONE;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
+        if (s_par[k] == NONE) continue;
+        const int lx = k % LX, ly = k / LX;
+        const int nx[4] = {-1, -1, 0, 1}, ny[4] = {0, -1, -1, -1};
+        for (int q = 0; q < 4; ++q) {
+            const int x = lx + nx[q], y = ly + ny[q];
+            if (x < 0 || y < 0 || x >= LX) continue;
+            const unsigned j = (unsigned)(y * LX + x);
+            if (load_parent<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, j) != NONE)
+                merge<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, (unsigned)k, j);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
+        const int lx = k % LX, ly = k / LX;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= L.cw || y >= L.ch) continue;
+        unsigned out = NONE;
+        if (s_par[k] != NONE) {
+            const unsigned r = find_root<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, (unsigned)k);
+            out = (unsigned)((y0 + (int)(r / LX)) * L.cw + x0 + (int)(r % LX));
+        }
+        L.label[(size_t)y * L.cw + x] = out;
+    }
+}
+
+// The unions that cross tile borders: a cell in a tile's first row, first
+// column or last column with a W, NW, N or NE neighbour in another tile.
+__global__ __launch_bounds__(THREADS) void opp_label_borders(Layout L, int tiles_x) {
+    const int per_tile = LX + 2 * (LY - 1);
+    const int t = blockIdx.x * THREADS + threadIdx.x;
+    const int tile = t / per_tile, e = t % per_tile;
+    if (tile >= tiles_x * ((L.ch + LY - 1) / LY)) return;
+    const int x0 = (tile % tiles_x) * LX, y0 = (tile / tiles_x) * LY;
+    int lx, ly;
+    if (e < LX) {
+        lx = e;
+        ly = 0;
+    } else if (e < LX + LY - 1) {
+        lx = 0;
+        ly = e - LX + 1;
+    } else {
+        lx = LX - 1;
+        ly = e - (LX + LY - 1) + 1;
+    }
+    const int cx = x0 + lx, cy = y0 + ly;
+    if (cx >= L.cw || cy >= L.ch) return;
+    const unsigned i = (unsigned)(cy * L.cw + cx);
+    if (load_parent<__HIP_MEMORY_SCOPE_AGENT>(L.label, i) == NONE) return;
     const int nx[4] = {-1, -1, 0, 1}, ny[4] = {0, -1, -1, -1};
-    for (int k = 0; k < 4; ++k) {
-        int x = cx + nx[k], y = cy + ny[k];
+    for (int q = 0; q < 4; ++q) {
+        const int x = cx + nx[q], y = cy + ny[q];
         if (x < 0 || y < 0 || x >= L.cw) continue;
-        unsigned j = (unsigned)(y * L.cw + x);
-        if (load_parent(L.label, j) != NONE) merge(L.label, (unsigned)i, j);
+        if (x >= x0 && x < x0 + LX && y >= y0) continue;  // same tile: already united
+        const unsigned j = (unsigned)(y * L.cw + x);
+        if (load_parent<__HIP_MEMORY_SCOPE_AGENT>(L.label, j) != NONE)
+            merge<__HIP_MEMORY_SCOPE_AGENT>(L.label, i, j);
     }
 }
 
@@ -245,7 +333,7 @@ __global__ __launch_bounds__(THREADS) void opp_flatten(Layout L) {
     if (i >= L.cells) return;
     unsigned p = L.label[i];
     if (p == NONE) return;
-    unsigned r = find_root(L.label, (unsigned)i);
+    unsigned r = find_root<__HIP_MEMORY_SCOPE_AGENT>(L.label, (unsigned)i);
     if (r != p) L.label[i] = r;
 }
 
@@ -316,6 +404,15 @@ __global__ __launch_bounds__(THREADS) void opp_scan_emit(Layout L, int mode) {
     unsigned excl;
     block_exclusive_scan(n, excl);
     unsigned pos = L.block_sums[blockIdx.x] + excl;
+    // Ring entries of one label come in runs; each run updates its region's
+    // first/last index once, and only when it would change them (the values
+    // only move one way, so an observed value that is already better holds).
+    unsigned run_lab = NONE, run_first = 0, run_last = 0;
+    auto flush = [&]() {
+        if (run_lab >= (unsigned)L.cap) return;
+        if (run_first < L.region_first[run_lab]) atomicMin(L.region_first + run_lab, run_first);
+        if (run_last > L.region_last[run_lab]) atomicMax(L.region_last + run_lab, run_last);
+    };
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         if (!(flags & (1u << k))) continue;
         int i = base + k;
@@ -323,14 +420,17 @@ __global__ __launch_bounds__(THREADS) void opp_scan_emit(Layout L, int mode) {
             L.rank[i] = pos;
         } else {
             L.ring_cell[pos] = (unsigned)i;
-            unsigned lab = L.label[i];
-            if (lab < (unsigned)L.cap) {
-                atomicMin(L.region_first + lab, pos);
-                atomicMax(L.region_last + lab, pos);
+            const unsigned lab = L.label[i];
+            if (lab != run_lab) {
+                flush();
+                run_lab = lab;
+                run_first = pos;
             }
+            run_last = pos;
         }
         ++pos;
     }
+    if (mode != ROOTS) flush();
 }
 
 // Compact labels (regions.rs:254-268): a root's rank + 1.
@@ -414,9 +514,12 @@ __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float
 // The per-sample correction of a ring cell does not depend on which
 // accumulator receives it, so opp_contrib computes every ring cell's nine
 // (channel, support set) samples once, in parallel. opp_sums then gives each
-// (group, slot) pair one wavefront: lanes load 64 consecutive ring entries,
-// and the accumulation walks the valid ones in raster order, one f64 add at a
-// time, exactly as the reference's sequential loop does.
+// group (0 = the image-wide pool, r = region r) one wavefront. The wavefront
+// stages SUM_BATCH consecutive ring entries at a time in LDS, with the next
+// batch's loads in flight, and lane s walks the staged entries in raster order
+// for slot s, one f64 add at a time, exactly as the reference's sequential
+// loop does. Entries that do not count add +0.0, which leaves an accumulator
+// that starts at +0.0 unchanged, so the adds run without branches.
 __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float l1, float l2) {
     const float lv[3] = {l0, l1, l2};
     const unsigned ring = L.scalars[1];
@@ -430,16 +533,18 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
         for (int c = 0; c < 3; ++c) {
             if (!(rc & (1u << c))) continue;
             if (mean_of(cell, c) < 0.5f * lv[c]) continue;
+            const float pref = L.pair_ref[(size_t)c * L.cells + i];
             for (int p = 0; p < 3; ++p) {
                 const int off = c * 3 + p;
                 if (!(any & (1u << off))) continue;
                 float value;
-                if (!correction(cell, c, support_pattern(c, p), lv, value)) continue;
+                if (!correction(cell, c, support_pattern(c, p), pref, lv, value)) continue;
                 L.contrib_value[(size_t)k * SLOTS + off] = value;
                 flags |= 1u << off;
                 if (p == 2) {
                     float ratio;
-                    if (support_log_ratio(cell, support_pattern(c, 2), ratio)) {
+                    if (support_log_ratio(support_pattern(c, 2),
+                                          L.pair_lr[(size_t)c * L.cells + i], ratio)) {
                         L.contrib_ratio[(size_t)k * 3 + c] = ratio;
                         flags |= 1u << (SLOTS + c);
                     }
@@ -451,16 +556,117 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
     }
 }
 
-__global__ __launch_bounds__(THREADS) void opp_sums(Layout L) {
-    const int lane = threadIdx.x & 63;
+constexpr int SUM_BATCH = 256;
+constexpr int SUM_PER_LANE = SUM_BATCH / 64;
+constexpr int SUM_UNROLL = 8;
+
+// One batch of ring entries in registers, entry q * 64 + lane of the batch in
+// slot q: its slot flags (zero when it belongs to another region), nine
+// samples and three ratios.
+struct SumBatch {
+    unsigned fl[SUM_PER_LANE];
+    float val[SUM_PER_LANE][SLOTS];
+    float rat[SUM_PER_LANE][3];
+};
+
+__device__ __forceinline__ void sum_load(const Layout& L, unsigned g, unsigned base,
+                                         unsigned end, int lane, SumBatch& b) {
+#pragma unroll
+    for (int q = 0; q < SUM_PER_LANE; ++q) {
+        const unsigned k = base + q * 64 + lane;
+        const bool in = k < end;
+        const unsigned lab = in ? L.ring_label[k] : 0u;
+        const unsigned f = in ? L.contrib_flags[k] : 0u;
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s)
+            b.val[q][s] = in ? L.contrib_value[(size_t)k * SLOTS + s] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) b.rat[q][c] = in ? L.contrib_ratio[(size_t)k * 3 + c] : 0.0f;
+        b.fl[q] = in && (g == 0 || lab == g) ? f : 0u;
+    }
+}
+
+// One pass of every slot of one group over ring entries [begin, end). Per
+// batch, the whole wavefront turns the staged entries into the exact f64
+// terms each slot adds (+0.0 for an entry that does not count, which leaves an
+// accumulator that starts at +0.0 unchanged) and counts the contributing
+// entries with ballots; lane s then adds slot s's terms in raster order, one
+// f64 add at a time, as the reference's sequential loop does.
+template <int PASS>
+__device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned begin,
+                                         unsigned end, int lane, const float* means,
+                                         const float* tols, double* s_a, double* s_b,
+                                         unsigned& count, double& accum, double& second) {
+    if (begin >= end) return;
+    SumBatch b;
+    sum_load(L, g, begin, end, lane, b);
+    for (unsigned base = begin; base < end; base += SUM_BATCH) {
+        unsigned mine = 0;
+#pragma unroll
+        for (int q = 0; q < SUM_PER_LANE; ++q) mine |= b.fl[q];
+        const bool staged = __ballot(mine != 0) != 0;
+        if (staged) {
+#pragma unroll
+            for (int q = 0; q < SUM_PER_LANE; ++q) {
+                const int e = q * 64 + lane;
+                const unsigned f = b.fl[q];
+#pragma unroll
+                for (int t = 0; t < SLOTS; ++t) {
+                    const float v = b.val[q][t];
+                    bool take = (f >> t) & 1u;
+                    double ta, tb;
+                    if (PASS == 0) {
+                        ta = take ? (double)v : 0.0;
+                        tb = ta * ta;
+                    } else {
+                        take = take && fabsf(v - means[t]) <= tols[t];
+                        const bool ratio = t % 3 == 2 && ((f >> (SLOTS + t / 3)) & 1u);
+                        ta = take ? (double)v : 0.0;
+                        tb = take && ratio ? (double)b.rat[q][t / 3] : 0.0;
+                    }
+                    s_a[t * SUM_BATCH + e] = ta;
+                    s_b[t * SUM_BATCH + e] = tb;
+                    const unsigned c = __popcll(__ballot(take));
+                    count += lane == t ? c : 0u;
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned next = base + SUM_BATCH;
+        if (next < end) sum_load(L, g, next, end, lane, b);
+        if (staged && lane < SLOTS) {
+            const double2* pa = reinterpret_cast<const double2*>(s_a + lane * SUM_BATCH);
+            const double2* pb = reinterpret_cast<const double2*>(s_b + lane * SUM_BATCH);
+            for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
+                double2 a[SUM_UNROLL], c2[SUM_UNROLL];
+#pragma unroll
+                for (int u = 0; u < SUM_UNROLL; ++u) {
+                    a[u] = pa[j + u];
+                    c2[u] = pb[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < SUM_UNROLL; ++u) {
+                    accum += a[u].x;
+                    second += c2[u].x;
+                    accum += a[u].y;
+                    second += c2[u].y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One wavefront per group (0 = the image-wide pool, r = region r), lane s
+// owning slot s (region_correction.rs:196-261).
+__global__ __launch_bounds__(64) void opp_sums(Layout L) {
+    __shared__ __align__(16) double s_a[SLOTS * SUM_BATCH];
+    __shared__ __align__(16) double s_b[SLOTS * SUM_BATCH];
+    const int lane = threadIdx.x;
     const unsigned count = min(L.scalars[0], (unsigned)L.cap - 1);
     const unsigned ring = L.scalars[1];
     const unsigned any = L.scalars[2];
-    const unsigned waves = gridDim.x * (THREADS / 64);
-    for (unsigned w = blockIdx.x * (THREADS / 64) + (threadIdx.x >> 6); w < (count + 1) * SLOTS;
-         w += waves) {
-        const unsigned g = w / SLOTS;
-        const int s = (int)(w % SLOTS), c = s / 3, p = s % 3;
+    for (unsigned g = blockIdx.x; g <= count; g += gridDim.x) {
         unsigned begin = 0, end = ring, use = any;
         if (g != 0) {
             begin = L.region_first[g];
@@ -468,64 +674,36 @@ __global__ __launch_bounds__(THREADS) void opp_sums(Layout L) {
             if (begin == NONE) begin = 0;
             use = L.region_wanted[g];
         }
+        if (use == 0) end = begin;
         unsigned n = 0, kept = 0;
         double accum = 0.0, second = 0.0;
+        float means[SLOTS], tols[SLOTS];
+        sum_pass<0>(L, g, begin, end, lane, means, tols, s_a, s_b, n, accum, second);
         float mean = 0.0f, tolerance = 0.0f;
-        if (use & (1u << s)) {
-            for (int pass = 0; pass < 2; ++pass) {
-                for (unsigned base = begin; base < end; base += 64) {
-                    const unsigned k = base + lane;
-                    bool valid = false;
-                    float v = 0.0f, r = 0.0f;
-                    if (k < end && (g == 0 || L.ring_label[k] == g)) {
-                        const unsigned f = L.contrib_flags[k];
-                        if (f & (1u << s)) {
-                            valid = true;
-                            v = L.contrib_value[(size_t)k * SLOTS + s];
-                            if (p == 2 && (f & (1u << (SLOTS + c))))
-                                r = L.contrib_ratio[(size_t)k * 3 + c];
-                        }
-                    }
-                    unsigned long long mask = __ballot(valid);
-                    while (mask) {
-                        const int j = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1;
-                        const float vj = __shfl(v, j, 64);
-                        const float rj = __shfl(r, j, 64);
-                        const double wide = (double)vj;
-                        if (pass == 0) {
-                            n += 1;
-                            accum += wide;
-                            second += wide * wide;
-                        } else if (fabsf(vj - mean) <= tolerance) {
-                            kept += 1;
-                            accum += wide;
-                            second += (double)rj;
-                        }
-                    }
-                }
-                if (pass == 0) {
-                    if (n != 0) {
-                        const double cnt = (double)n;
-                        const double m = accum / cnt;
-                        const double variance = fmax(second / cnt - m * m, 0.0);
-                        mean = (float)m;
-                        tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
-                    }
-                    accum = 0.0;
-                    second = 0.0;
-                }
-            }
+        if (n != 0) {
+            const double cnt = (double)n;
+            const double m = accum / cnt;
+            const double variance = fmax(second / cnt - m * m, 0.0);
+            mean = (float)m;
+            tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
         }
-        if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < SLOTS; ++t) {
+            means[t] = __shfl(mean, t, 64);
+            tols[t] = __shfl(tolerance, t, 64);
+        }
+        accum = 0.0;
+        second = 0.0;
+        sum_pass<1>(L, g, begin, end, lane, means, tols, s_a, s_b, kept, accum, second);
+        if (lane < SLOTS) {
             // Slot::value: fewer than the minimum samples or nothing kept
             // leaves the slot empty.
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (n >= MIN_BORDER_CELLS && kept != 0) {
+            if ((use & (1u << lane)) && n >= MIN_BORDER_CELLS && kept != 0) {
                 const double kd = (double)kept;
                 out = make_float4((float)(accum / kd), (float)(second / kd), 1.0f, 0.0f);
             }
-            L.corrections[(size_t)g * SLOTS + s] = out;
+            L.corrections[(size_t)g * SLOTS + lane] = out;
         }
     }
 }
@@ -535,64 +713,92 @@ __device__ __forceinline__ int total_key(float v) {
     int i = __float_as_int(v);
     return i ^ (int)(((unsigned)(i >> 31)) >> 1);
 }
+__device__ __forceinline__ float from_total_key(int k) {
+    return __int_as_float(k ^ (int)(((unsigned)(k >> 31)) >> 1));
+}
 
+// At most two candidates carry over from radius 2 (three or more end the
+// search), and the radius-3 ring holds 24 cells.
+constexpr int MAX_LOCAL = 26;
+
+// The estimate for one (cell, channel): every suspected site of that channel
+// in the cell gets the same one (opposed.rs:243-278 reads only cell data).
 struct Site {
     const Layout* L;
     const float* lv;
+    int target;
+    unsigned support;
+    bool two;  // two-channel support: reference and ratio come from the pair planes
+    bool ht;
+    float tr;
+    int* keys;  // this thread's LDS column, stride THREADS
 
-    __device__ bool lookup(unsigned label, int target, unsigned support, float& offset,
-                           bool& has_ratio, float& ratio) const {
+    __device__ bool lookup(unsigned label, float& offset, bool& has_ratio, float& ratio) const {
         int p = pattern_position(target, support);
         if (p < 0) return false;
         int off = target * 3 + p;
-        bool want_ratio = __popc(support) == 2;
         float4 v = label < (unsigned)L->cap ? L->corrections[(size_t)label * SLOTS + off]
                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (v.z == 0.0f) v = L->corrections[off];
         if (v.z == 0.0f) return false;
         offset = v.x;
-        has_ratio = want_ratio;
+        has_ratio = two;
         ratio = v.y;
         return true;
     }
 
-    // opposed.rs:282-327
-    __device__ bool local_correction(int target, unsigned support, bool ht, float tr, int tx,
-                                     int ty, float& out, int& out_radius) const {
-        float corr[48];
+    // One candidate of the local search (opposed.rs:295-312).
+    __device__ void candidate(int cx, int cy, int& count) const {
+        const size_t j = (size_t)cy * L->cw + cx;
+        const float4 cand = L->cell[j];
+        const unsigned f = flags_of(cand);
+        if (!(usable(f, target) && clean(f, target))) return;
+        const size_t pj = (size_t)target * L->cells + j;
+        const float pref = two ? L->pair_ref[pj] : 0.0f;
+        float value;
+        if (!correction(cand, target, support, pref, lv, value)) return;
+        float cr = 0.0f;
+        const bool hc = two && support_log_ratio(support, L->pair_lr[pj], cr);
+        if (!chromaticity_agrees(support, ht, tr, hc, cr)) return;
+        if (count < MAX_LOCAL) keys[count * THREADS] = total_key(value);
+        ++count;
+    }
+
+    // opposed.rs:282-327: rings of radius 1..3 cells around the target cell;
+    // the first radius with at least three candidates gives their median.
+    __device__ bool local_correction(int tx, int ty, float& out, int& out_radius) const {
         int count = 0;
         for (int radius = 1; radius <= 3; ++radius) {
-            int y0 = max(ty - radius, 0), y1 = min(ty + radius, L->ch - 1);
-            int x0 = max(tx - radius, 0), x1 = min(tx + radius, L->cw - 1);
-            for (int cy = y0; cy <= y1; ++cy)
-                for (int cx = x0; cx <= x1; ++cx) {
-                    if (max(abs(cx - tx), abs(cy - ty)) != radius) continue;
-                    float4 cand = L->cell[cy * L->cw + cx];
-                    unsigned f = flags_of(cand);
-                    if (!(usable(f, target) && clean(f, target))) continue;
-                    float value;
-                    if (!correction(cand, target, support, lv, value)) continue;
-                    float cr = 0.0f;
-                    bool hc = support_log_ratio(cand, support, cr);
-                    if (!chromaticity_agrees(support, ht, tr, hc, cr)) continue;
-                    if (count < 48) corr[count++] = value;
+            const int x0 = max(tx - radius, 0), x1 = min(tx + radius, L->cw - 1);
+            for (int cy = max(ty - radius, 0); cy <= min(ty + radius, L->ch - 1); ++cy) {
+                if (cy == ty - radius || cy == ty + radius) {
+                    for (int cx = x0; cx <= x1; ++cx) candidate(cx, cy, count);
+                } else {
+                    if (tx - radius >= 0) candidate(tx - radius, cy, count);
+                    if (tx + radius < L->cw) candidate(tx + radius, cy, count);
                 }
+            }
             if (count >= 3) {
                 // Median in Rust's total order: the element(s) of rank
-                // count / 2 (and count / 2 - 1 when even), ties by position.
-                int middle = count / 2;
-                float lo = 0.0f, hi = 0.0f;
-                for (int a = 0; a < count; ++a) {
-                    int ka = total_key(corr[a]);
-                    int rank = 0;
-                    for (int b = 0; b < count; ++b) {
-                        int kb = total_key(corr[b]);
-                        rank += (kb < ka) || (kb == ka && b < a);
+                // count / 2 (and count / 2 - 1 when even). Equal keys are
+                // equal bit patterns, so ties need no position order.
+                const int n = min(count, MAX_LOCAL);
+                const int middle = n / 2;
+                int lo = 0, hi = 0;
+                for (int a = 0; a < n; ++a) {
+                    const int ka = keys[a * THREADS];
+                    int below = 0, equal_before = 0;
+                    for (int b = 0; b < n; ++b) {
+                        const int kb = keys[b * THREADS];
+                        below += kb < ka;
+                        equal_before += (kb == ka) & (b < a);
                     }
-                    if (rank == middle) hi = corr[a];
-                    if (rank == middle - 1) lo = corr[a];
+                    const int rank = below + equal_before;
+                    if (rank == middle) hi = ka;
+                    if (rank == middle - 1) lo = ka;
                 }
-                out = count % 2 == 0 ? (lo + hi) * 0.5f : hi;
+                out = n % 2 == 0 ? (from_total_key(lo) + from_total_key(hi)) * 0.5f
+                                 : from_total_key(hi);
                 out_radius = radius;
                 return true;
             }
@@ -601,20 +807,18 @@ struct Site {
     }
 
     // opposed.rs:243-278
-    __device__ bool estimate(const float4& target, int channel, int tx, int ty, float& out) const {
-        unsigned support = support_mask(target, channel, lv);
+    __device__ bool estimate(const float4& cell, int tx, int ty, float& out) const {
+        const size_t pi = (size_t)target * L->cells + (size_t)ty * L->cw + tx;
         float target_reference;
-        if (!reference(target, support, target_reference)) return false;
-        float tr = 0.0f;
-        bool ht = support_log_ratio(target, support, tr);
+        if (!reference(cell, support, two ? L->pair_ref[pi] : 0.0f, target_reference)) return false;
         float local = 0.0f;
         int radius = 0;
-        bool has_local = local_correction(channel, support, ht, tr, tx, ty, local, radius);
-        unsigned label = L->label[ty * L->cw + tx];
+        const bool has_local = local_correction(tx, ty, local, radius);
+        const unsigned label = L->label[(size_t)ty * L->cw + tx];
         float region = 0.0f, br = 0.0f;
         bool bhr = false;
-        bool has_region = lookup(label, channel, support, region, bhr, br) &&
-                          chromaticity_agrees(support, ht, tr, bhr, br);
+        const bool has_region =
+            lookup(label, region, bhr, br) && chromaticity_agrees(support, ht, tr, bhr, br);
         float corr;
         if (has_local && has_region) {
             float weight = (float)(3 + 1 - radius) / 3.0f;
@@ -633,50 +837,64 @@ struct Site {
     }
 };
 
-// opposed.rs:97-162
+// opposed.rs:97-162: one thread per (cell, channel) with a suspected site of
+// that channel; the estimate is computed once and applied to each such site.
 __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
                                                      float l0, float l1, float l2, int has_ceiling,
                                                      float c0, float c1, float c2,
                                                      DeviceStats* stats) {
+    __shared__ int s_keys[MAX_LOCAL * THREADS];
     const float lv[3] = {l0, l1, l2};
     const float ceiling[3] = {c0, c1, c2};
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int channel = blockIdx.z;
+    const int tx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int ty = blockIdx.y * (THREADS / 64) + (threadIdx.x >> 6);
     unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
-    if (x < L.w && y < L.h) {
-        float* sp = m + (size_t)y * L.w + x;
-        float sample = *sp;
-        int c = color_at(pattern, x, y);
-        float level = lv[c];
-        if (sample >= level) {
-            int tx = x / 2, ty = y / 2;
-            float4 target = L.cell[ty * L.cw + tx];
-            unsigned f = flags_of(target);
+    if (tx < L.cw && ty < L.ch) {
+        const float4 target = L.cell[(size_t)ty * L.cw + tx];
+        const unsigned f = flags_of(target);
+        if (f & (1u << (channel + 3))) {
             int support_count = 0;
             for (int o = 0; o < 3; ++o)
-                if (o != c && usable(f, o) && clean(f, o)) ++support_count;
-            Site site{&L, lv};
+                if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
             float est = 0.0f;
-            bool has = support_count != 0 && site.estimate(target, c, tx, ty, est);
-            bool neutralise = false;
-            if (!has) {
-                fallback = 1;
-                unsupported = support_count == 0;
-                neutralise = true;
-            } else {
-                reconstructed = 1;
-                float bounded = fminf(est, 4.0f * level);
-                if (bounded > sample) {
-                    *sp = bounded;
-                    changed = 1;
-                } else {
-                    neutralise = true;
+            bool has = false;
+            if (support_count != 0) {
+                Site site{&L, lv, channel, support_mask(target, channel, lv), false, false, 0.0f,
+                          s_keys + threadIdx.x};
+                site.two = __popc(site.support) == 2;
+                const size_t pi = (size_t)channel * L.cells + (size_t)ty * L.cw + tx;
+                site.ht = site.two && support_log_ratio(site.support, L.pair_lr[pi], site.tr);
+                has = site.estimate(target, tx, ty, est);
+            }
+            const float level = lv[channel];
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int x = 2 * tx + dx, y = 2 * ty + dy;
+                    if (x >= L.w || y >= L.h || color_at(pattern, x, y) != channel) continue;
+                    float* sp = m + (size_t)y * L.w + x;
+                    const float sample = *sp;
+                    if (!(sample >= level)) continue;
+                    bool neutralise = false;
+                    if (!has) {
+                        fallback += 1;
+                        unsupported += support_count == 0;
+                        neutralise = true;
+                    } else {
+                        reconstructed += 1;
+                        const float bounded = fminf(est, 4.0f * level);
+                        if (bounded > sample) {
+                            *sp = bounded;
+                            changed += 1;
+                        } else {
+                            neutralise = true;
+                        }
+                    }
+                    if (neutralise && has_ceiling && sample > ceiling[channel]) {
+                        *sp = ceiling[channel];
+                        neutralised += 1;
+                    }
                 }
-            }
-            if (neutralise && has_ceiling && sample > ceiling[c]) {
-                *sp = ceiling[c];
-                neutralised = 1;
-            }
         }
     }
     unsigned v[5] = {reconstructed, changed, fallback, unsupported, neutralised};
@@ -704,6 +922,8 @@ Layout layout_of(const OpposedArgs& a) {
     const size_t cells = (size_t)L.cells, cap = (size_t)L.cap;
     const size_t scan_blocks = (cells + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
     L.cell = (float4*)take(cells * sizeof(float4));
+    L.pair_ref = (float*)take(cells * 3 * 4);
+    L.pair_lr = (float*)take(cells * 3 * 4);
     L.label = (unsigned*)take(cells * 4);
     L.rank = (unsigned*)take(cells * 4);
     L.block_sums = (unsigned*)take(scan_blocks * 4);
@@ -739,38 +959,50 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     const float l0 = a.levels[0], l1 = a.levels[1], l2 = a.levels[2];
     const int cell_blocks = (L.cells + THREADS - 1) / THREADS;
     const int scan_blocks = (L.cells + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const int tiles_x = (L.cw + LX - 1) / LX, tiles_y = (L.ch + LY - 1) / LY;
+    const int border_cells = tiles_x * tiles_y * (LX + 2 * (LY - 1));
     const uint64_t px = (uint64_t)L.w * L.h, cells = (uint64_t)L.cells;
-    RHD_TIMED(prof, s, "opp_cells", 4 * px + 16 * cells,
+    // Algorithmic bytes per launch. The ring-entry kernels (contrib, sums)
+    // depend on the ring size, which only the device knows, and report 0.
+    RHD_TIMED(prof, s, "opp_cells", 4 * px + 16 * cells + 24 * cells,
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
-    RHD_TIMED(prof, s, "opp_label", 16 * cells + 4 * cells + 8 * cells + 8 * cells,
-              hipLaunchKernelGGL(opp_dilate, dim3(cell_blocks), dim3(THREADS), 0, s, L);
-              hipLaunchKernelGGL(opp_union, dim3(cell_blocks), dim3(THREADS), 0, s, L);
-              hipLaunchKernelGGL(opp_flatten, dim3(cell_blocks), dim3(THREADS), 0, s, L);
+    RHD_TIMED(prof, s, "opp_label_tiles", 16 * cells + 4 * cells,
+              hipLaunchKernelGGL(opp_label_tiles, dim3(tiles_x, tiles_y), dim3(THREADS), 0, s, L));
+    RHD_TIMED(prof, s, "opp_label_borders", 20ull * border_cells,
+              hipLaunchKernelGGL(opp_label_borders, dim3((border_cells + THREADS - 1) / THREADS),
+                                 dim3(THREADS), 0, s, L, tiles_x));
+    RHD_TIMED(prof, s, "opp_flatten", 8 * cells,
+              hipLaunchKernelGGL(opp_flatten, dim3(cell_blocks), dim3(THREADS), 0, s, L));
+    RHD_TIMED(prof, s, "opp_scan_roots", 8 * cells,
               hipLaunchKernelGGL(opp_scan_count, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)ROOTS);
               hipLaunchKernelGGL(opp_scan_blocks, dim3(1), dim3(THREADS), 0, s, L, scan_blocks,
                                  (int)ROOTS);
               hipLaunchKernelGGL(opp_scan_emit, dim3(scan_blocks), dim3(THREADS), 0, s, L,
-                                 (int)ROOTS);
+                                 (int)ROOTS));
+    RHD_TIMED(prof, s, "opp_relabel", 12 * cells,
               hipLaunchKernelGGL(opp_relabel, dim3(cell_blocks), dim3(THREADS), 0, s, L));
-    RHD_TIMED(prof, s, "opp_regions", 24 * cells,
+    RHD_TIMED(prof, s, "opp_regions", 20 * cells,
               hipLaunchKernelGGL(opp_region_init, dim3(256), dim3(THREADS), 0, s, L, a.stats);
               hipLaunchKernelGGL(opp_regions, dim3(cell_blocks), dim3(THREADS), 0, s, L, l0, l1,
-                                 l2);
+                                 l2));
+    RHD_TIMED(prof, s, "opp_scan_ring", 40 * cells,
               hipLaunchKernelGGL(opp_scan_count, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)RING);
               hipLaunchKernelGGL(opp_scan_blocks, dim3(1), dim3(THREADS), 0, s, L, scan_blocks,
                                  (int)RING);
               hipLaunchKernelGGL(opp_scan_emit, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)RING));
-    RHD_TIMED(prof, s, "opp_corrections", 0,
-              hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2);
-              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(THREADS), 0, s, L));
-    RHD_TIMED(prof, s, "opp_sites", 8 * px,
-              hipLaunchKernelGGL(opp_sites, dim3((L.w + 63) / 64, (L.h + 3) / 4), dim3(THREADS), 0,
-                                 s, a.mosaic, L, a.pattern, l0, l1, l2, a.has_ceiling,
-                                 a.ceiling[0], a.ceiling[1], a.ceiling[2], a.stats));
+    RHD_TIMED(prof, s, "opp_contrib", 0,
+              hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
+    RHD_TIMED(prof, s, "opp_sums", 0,
+              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(64), 0, s, L));
+    RHD_TIMED(prof, s, "opp_sites", 8 * px + 16 * cells,
+              hipLaunchKernelGGL(opp_sites, dim3((L.cw + 63) / 64, (L.ch + 3) / 4, 3),
+                                 dim3(THREADS), 0, s, a.mosaic, L, a.pattern, l0, l1, l2,
+                                 a.has_ceiling, a.ceiling[0], a.ceiling[1], a.ceiling[2],
+                                 a.stats));
     return hipGetLastError();
 }
 

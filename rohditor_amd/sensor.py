@@ -273,7 +273,7 @@ class CancellationToken:
     """crates/core/src/cancel.rs:13-42: a flag the library polls between stages."""
 
     def __init__(self):
-        self._flag = ctypes.c_uint32(0)
+        self._flag = ctypes.c_uint8(0)
 
     def cancel(self) -> None:
         self._flag.value = 1
