@@ -434,6 +434,7 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
     const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
     const int i = rr * TS + cc;
+    bool flagged = false;
     if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
         const float* vcd = t.full(VCD);
         const float* hcd = t.full(HCD);
@@ -487,11 +488,23 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
              GAUSSGRAD[5] * (d[-2 * TS - 2] + d[-2 * TS + 2] + d[2 * TS - 2] + d[2 * TS + 2]));
         if (test > 0.0f) {
             t.nyq()[i >> 1] = 1;
+            flagged = true;
+        }
+    }
+    // A wavefront is one row of the tile, with cc increasing by lane, so its
+    // flagged sites span [cc of the first flagged lane, cc of the last]; one
+    // lane updates the tile's box. The atomics return nothing, so the wave
+    // never waits on them (a load here would also wait for its stores).
+    const unsigned long long m = __ballot(flagged);
+    if (m) {
+        const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
+        const int c_lo = __shfl(cc, first, 64), c_hi = __shfl(cc, last, 64);
+        if ((threadIdx.x & 63) == 0) {
             int* area = a.areas + 4 * (size_t)blockIdx.z;
             atomicMin(area + 0, rr);
             atomicMax(area + 1, rr);
-            atomicMin(area + 2, cc);
-            atomicMax(area + 3, cc);
+            atomicMin(area + 2, c_lo);
+            atomicMax(area + 3, c_hi);
         }
     }
     if (rr >= 8 && rr < TS - 8 && cc >= 8 && cc < TS - 8) {

@@ -83,10 +83,13 @@ __device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, 
     return v;
 }
 
-// Threads walk a (rows x cols) LDS window as 4 rows of 64 lanes.
-#define FOR_WINDOW(rows, cols, r, c)                                   \
-    for (int r = (int)(threadIdx.x >> 6); r < (rows); r += THREADS / 64) \
-        for (int c = (int)(threadIdx.x & 63); c < (cols); c += 64)
+// Threads walk a (rows x cols) window in row-major order, THREADS elements
+// per step, so every lane has work whatever the window width; the (row,
+// column) position advances by a fixed step with one carry.
+#define FOR_WINDOW(rows, cols, r, c)                                                    \
+    for (int r = (int)threadIdx.x / (cols), c = (int)threadIdx.x % (cols),              \
+             r##_step = THREADS / (cols), c##_step = THREADS % (cols);                  \
+         r < (rows); r += r##_step, c += c##_step, (c >= (cols) ? (c -= (cols), ++r) : 0))
 
 // Copies a (rows x cols) window of a global plane into LDS. The trip counts
 // are compile-time bounds (MAXROWS x MAXCOLS), so every load of the window is
@@ -163,15 +166,23 @@ __global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in
 
 // One RL iteration (sharpening.rs:268-304) for a 64x32 tile: blur(estimate)
 // over the tile widened by r, ratio = guide / max(b, floor), blur(ratio) over
-// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide).
-constexpr int E_W = TW + 4 * MAXR, E_H = TH + 4 * MAXR;
-constexpr int H1_W = TW + 2 * MAXR;
-constexpr int R_W = TW + 2 * MAXR, R_H = TH + 2 * MAXR;
+// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide). The LDS
+// windows are sized for the radius: the estimate with a 2r halo, its
+// horizontal blur, and the guide with an r halo, which the ratio overwrites
+// in place (the final clamp reads the guide from global memory).
+template <int R>
+struct IterWindows {
+    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;  // estimate
+    static constexpr int HW = TW + 2 * R;                   // horizontal blurs
+    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;  // guide / ratio
+};
 
 template <int R, bool FAST>
 __device__ void iterate_tile(const float* __restrict__ est_in, const float* __restrict__ guide,
                              float* __restrict__ est_out, int width, int height, const Weights& k,
-                             float* s_e, float* s_h1, float* s_r, float* s_h2, float* s_g) {
+                             float* s_e, float* s_h, float* s_g) {
+    using Win = IterWindows<R>;
+    constexpr int EW = Win::EW, HW = Win::HW, GW = Win::GW;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
     const int ex0 = max(0, x0 - 2 * R), ex1 = min(width, x0 + TW + 2 * R);
@@ -180,27 +191,80 @@ __device__ void iterate_tile(const float* __restrict__ est_in, const float* __re
     const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
     const int ew = ex1 - ex0, eh = ey1 - ey0, rw = rx1 - rx0, rh = ry1 - ry0;
     const int ow = ox1 - x0, oh = oy1 - y0;
-    stage<E_H, E_W>(s_e, E_W, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
-    stage<R_H, R_W>(s_g, R_W, guide + (size_t)ry0 * width + rx0, width, rh, rw);
+    stage<Win::EH, Win::EW>(s_e, EW, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
+    stage<Win::GH, Win::GW>(s_g, GW, guide + (size_t)ry0 * width + rx0, width, rh, rw);
     __syncthreads();
-    FOR_WINDOW(eh, rw, r, c) s_h1[r * H1_W + c] = hblur<R, FAST>(s_e + r * E_W, ex0, rx0 + c, width, k);
+    FOR_WINDOW(eh, rw, r, c) s_h[r * HW + c] = hblur<R, FAST>(s_e + r * EW, ex0, rx0 + c, width, k);
     __syncthreads();
     FOR_WINDOW(rh, rw, r, c) {
-        const int y = ry0 + r, x = rx0 + c;
-        const float b = vblur<R, FAST>(s_h1, H1_W, ey0, y, c, height, k);
-        (void)x;
-        s_r[r * R_W + c] = s_g[r * R_W + c] / num_max(b, FLOOR);
+        const float b = vblur<R, FAST>(s_h, HW, ey0, ry0 + r, c, height, k);
+        s_g[r * GW + c] = s_g[r * GW + c] / num_max(b, FLOOR);
     }
     __syncthreads();
-    FOR_WINDOW(rh, ow, r, c) s_h2[r * TW + c] = hblur<R, FAST>(s_r + r * R_W, rx0, x0 + c, width, k);
+    FOR_WINDOW(rh, ow, r, c) s_h[r * HW + c] = hblur<R, FAST>(s_g + r * GW, rx0, x0 + c, width, k);
     __syncthreads();
     FOR_WINDOW(oh, ow, r, c) {
         const int y = y0 + r, x = x0 + c;
-        const float b = vblur<R, FAST>(s_h2, TW, ry0, y, c, height, k);
-        const float g = s_g[(y - ry0) * R_W + (x - rx0)];
-        const float v = s_e[(y - ey0) * E_W + (x - ex0)] * b;
+        const float b = vblur<R, FAST>(s_h, HW, ry0, y, c, height, k);
+        const size_t i = (size_t)y * width + x;
+        const float g = guide[i];
+        const float v = s_e[(y - ey0) * EW + (x - ex0)] * b;
         const float lo = g * 0.5f, hi = g * 2.0f;
-        est_out[(size_t)y * width + x] = v < lo ? lo : (v > hi ? hi : v);
+        est_out[i] = v < lo ? lo : (v > hi ? hi : v);
+    }
+}
+
+// The same iteration for a tile whose windows lie inside the image: every
+// horizontal tap is interior (the sum starts at 0.0), no mirroring, and all
+// LDS offsets are compile-time constants. The guide values of the tile's
+// outputs are loaded with the windows, so the final clamp waits on nothing.
+template <int R>
+__device__ void iterate_interior(const float* __restrict__ est_in, const float* __restrict__ guide,
+                                 float* __restrict__ est_out, int width, const Weights& k,
+                                 float* s_e, float* s_h, float* s_g) {
+    using Win = IterWindows<R>;
+    constexpr int EW = Win::EW, EH = Win::EH, HW = Win::HW, GW = Win::GW, GH = Win::GH;
+    constexpr int OUT_PER_THREAD = TW * TH / THREADS;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    float g_out[OUT_PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < OUT_PER_THREAD; ++q)
+        g_out[q] = guide[(size_t)(y0 + ty + 4 * q) * width + x0 + tx];
+    stage<EH, EW>(s_e, EW, est_in + (size_t)(y0 - 2 * R) * width + x0 - 2 * R, width, EH, EW);
+    stage<GH, GW>(s_g, GW, guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
+    __syncthreads();
+    FOR_WINDOW(EH, GW, r, c) {
+        float v = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) v += s_e[r * EW + c + t] * k.w[t];
+        s_h[r * HW + c] = v;
+    }
+    __syncthreads();
+    FOR_WINDOW(GH, GW, r, c) {
+        float b = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) b += s_h[(r + t) * HW + c] * k.w[t];
+        s_g[r * GW + c] = s_g[r * GW + c] / num_max(b, FLOOR);
+    }
+    __syncthreads();
+    FOR_WINDOW(GH, TW, r, c) {
+        float v = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) v += s_g[r * GW + c + t] * k.w[t];
+        s_h[r * HW + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < OUT_PER_THREAD; ++q) {
+        const int r = ty + 4 * q, c = tx;
+        float b = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) b += s_h[(r + t) * HW + c] * k.w[t];
+        const float g = g_out[q];
+        const float v = s_e[(r + 2 * R) * EW + c + 2 * R] * b;
+        const float lo = g * 0.5f, hi = g * 2.0f;
+        est_out[(size_t)(y0 + r) * width + x0 + c] = v < lo ? lo : (v > hi ? hi : v);
     }
 }
 
@@ -209,17 +273,17 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
                                                        int height, Weights k) {
-    __shared__ float s_e[E_H * E_W];
-    __shared__ float s_h1[E_H * H1_W];  // horizontal blur of the estimate, then of the ratio
-    __shared__ float s_r[R_H * R_W];
-    __shared__ float s_g[R_H * R_W];
+    using Win = IterWindows<R>;
+    __shared__ float s_e[Win::EH * Win::EW];
+    __shared__ float s_h[Win::EH * Win::HW];
+    __shared__ float s_g[Win::GH * Win::GW];
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const bool fast =
+    const bool interior =
         x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
-    if (fast)
-        iterate_tile<R, true>(est_in, guide, est_out, width, height, k, s_e, s_h1, s_r, s_h1, s_g);
+    if (interior)
+        iterate_interior<R>(est_in, guide, est_out, width, k, s_e, s_h, s_g);
     else
-        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_e, s_h1, s_r, s_h1, s_g);
+        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_e, s_h, s_g);
 }
 
 // sharpening.rs:105-126

@@ -586,70 +586,97 @@ __device__ __forceinline__ void sum_load(const Layout& L, unsigned g, unsigned b
     }
 }
 
-// One pass of every slot of one group over ring entries [begin, end). Per
-// batch, the whole wavefront turns the staged entries into the exact f64
-// terms each slot adds (+0.0 for an entry that does not count, which leaves an
-// accumulator that starts at +0.0 unchanged) and counts the contributing
-// entries with ballots; lane s then adds slot s's terms in raster order, one
-// f64 add at a time, as the reference's sequential loop does.
+// Staging buffers of one batch: per slot the exact f64 terms it adds, the
+// number of contributing entries, and whether any entry of the batch belongs
+// to the group.
+struct SumStage {
+    double a[SLOTS * SUM_BATCH];
+    double b[SLOTS * SUM_BATCH];
+    unsigned count[SLOTS];
+    unsigned staged;
+};
+
+// One pass of every slot of one group over ring entries [begin, end), by two
+// wavefronts. Wave 1 loads batch i (the next batch's loads in flight behind
+// it) and turns it into the terms each slot adds: +0.0 for an entry that does
+// not count, which leaves an accumulator that starts at +0.0 unchanged, and
+// ballot counts. Meanwhile lane s of wave 0 adds slot s's terms of batch i - 1
+// in raster order, one f64 add at a time, as the reference's sequential loop
+// does. The two stage buffers alternate.
 template <int PASS>
 __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned begin,
-                                         unsigned end, int lane, const float* means,
-                                         const float* tols, double* s_a, double* s_b,
-                                         unsigned& count, double& accum, double& second) {
-    if (begin >= end) return;
+                                         unsigned end, const float* means, const float* tols,
+                                         SumStage* st, unsigned& count, double& accum,
+                                         double& second) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned batches = begin < end ? (end - begin + SUM_BATCH - 1) / SUM_BATCH : 0u;
     SumBatch b;
-    sum_load(L, g, begin, end, lane, b);
-    for (unsigned base = begin; base < end; base += SUM_BATCH) {
-        unsigned mine = 0;
+    if (wave == 1 && batches != 0) sum_load(L, g, begin, end, lane, b);
+    for (unsigned i = 0; i <= batches; ++i) {
+        if (wave == 1 && i < batches) {
+            SumStage& out = st[i & 1];
+            const unsigned base = begin + i * SUM_BATCH;
+            unsigned mine = 0;
 #pragma unroll
-        for (int q = 0; q < SUM_PER_LANE; ++q) mine |= b.fl[q];
-        const bool staged = __ballot(mine != 0) != 0;
-        if (staged) {
+            for (int q = 0; q < SUM_PER_LANE; ++q) mine |= b.fl[q];
+            const bool staged = __ballot(mine != 0) != 0;
+            if (staged) {
+                unsigned counts[SLOTS];
 #pragma unroll
-            for (int q = 0; q < SUM_PER_LANE; ++q) {
-                const int e = q * 64 + lane;
-                const unsigned f = b.fl[q];
+                for (int t = 0; t < SLOTS; ++t) counts[t] = 0;
 #pragma unroll
-                for (int t = 0; t < SLOTS; ++t) {
-                    const float v = b.val[q][t];
-                    bool take = (f >> t) & 1u;
-                    double ta, tb;
-                    if (PASS == 0) {
-                        ta = take ? (double)v : 0.0;
-                        tb = ta * ta;
-                    } else {
-                        take = take && fabsf(v - means[t]) <= tols[t];
-                        const bool ratio = t % 3 == 2 && ((f >> (SLOTS + t / 3)) & 1u);
-                        ta = take ? (double)v : 0.0;
-                        tb = take && ratio ? (double)b.rat[q][t / 3] : 0.0;
+                for (int q = 0; q < SUM_PER_LANE; ++q) {
+                    const int e = q * 64 + lane;
+                    const unsigned f = b.fl[q];
+#pragma unroll
+                    for (int t = 0; t < SLOTS; ++t) {
+                        const float v = b.val[q][t];
+                        bool take = (f >> t) & 1u;
+                        double ta, tb;
+                        if (PASS == 0) {
+                            ta = take ? (double)v : 0.0;
+                            tb = ta * ta;
+                        } else {
+                            take = take && fabsf(v - means[t]) <= tols[t];
+                            const bool ratio = t % 3 == 2 && ((f >> (SLOTS + t / 3)) & 1u);
+                            ta = take ? (double)v : 0.0;
+                            tb = take && ratio ? (double)b.rat[q][t / 3] : 0.0;
+                        }
+                        out.a[t * SUM_BATCH + e] = ta;
+                        out.b[t * SUM_BATCH + e] = tb;
+                        counts[t] += __popcll(__ballot(take));
                     }
-                    s_a[t * SUM_BATCH + e] = ta;
-                    s_b[t * SUM_BATCH + e] = tb;
-                    const unsigned c = __popcll(__ballot(take));
-                    count += lane == t ? c : 0u;
+                }
+                if (lane < SLOTS) {
+                    unsigned c = 0;
+#pragma unroll
+                    for (int t = 0; t < SLOTS; ++t) c = lane == t ? counts[t] : c;
+                    out.count[lane] = c;
                 }
             }
+            if (lane == 0) out.staged = staged;
+            if (base + SUM_BATCH < end) sum_load(L, g, base + SUM_BATCH, end, lane, b);
         }
-        __syncthreads();
-        const unsigned next = base + SUM_BATCH;
-        if (next < end) sum_load(L, g, next, end, lane, b);
-        if (staged && lane < SLOTS) {
-            const double2* pa = reinterpret_cast<const double2*>(s_a + lane * SUM_BATCH);
-            const double2* pb = reinterpret_cast<const double2*>(s_b + lane * SUM_BATCH);
-            for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
-                double2 a[SUM_UNROLL], c2[SUM_UNROLL];
+        if (wave == 0 && i >= 1 && lane < SLOTS) {
+            const SumStage& in = st[(i - 1) & 1];
+            if (in.staged) {
+                count += in.count[lane];
+                const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_BATCH);
+                const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_BATCH);
+                for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
+                    double2 a[SUM_UNROLL], c2[SUM_UNROLL];
 #pragma unroll
-                for (int u = 0; u < SUM_UNROLL; ++u) {
-                    a[u] = pa[j + u];
-                    c2[u] = pb[j + u];
-                }
+                    for (int u = 0; u < SUM_UNROLL; ++u) {
+                        a[u] = pa[j + u];
+                        c2[u] = pb[j + u];
+                    }
 #pragma unroll
-                for (int u = 0; u < SUM_UNROLL; ++u) {
-                    accum += a[u].x;
-                    second += c2[u].x;
-                    accum += a[u].y;
-                    second += c2[u].y;
+                    for (int u = 0; u < SUM_UNROLL; ++u) {
+                        accum += a[u].x;
+                        second += c2[u].x;
+                        accum += a[u].y;
+                        second += c2[u].y;
+                    }
                 }
             }
         }
@@ -657,12 +684,12 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
     }
 }
 
-// One wavefront per group (0 = the image-wide pool, r = region r), lane s
-// owning slot s (region_correction.rs:196-261).
-__global__ __launch_bounds__(64) void opp_sums(Layout L) {
-    __shared__ __align__(16) double s_a[SLOTS * SUM_BATCH];
-    __shared__ __align__(16) double s_b[SLOTS * SUM_BATCH];
-    const int lane = threadIdx.x;
+// One workgroup of two wavefronts per group (0 = the image-wide pool, r =
+// region r); lane s of wave 0 owns slot s (region_correction.rs:196-261).
+__global__ __launch_bounds__(128) void opp_sums(Layout L) {
+    __shared__ __align__(16) SumStage s_stage[2];
+    __shared__ float s_mean[SLOTS], s_tol[SLOTS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned count = min(L.scalars[0], (unsigned)L.cap - 1);
     const unsigned ring = L.scalars[1];
     const unsigned any = L.scalars[2];
@@ -678,24 +705,31 @@ __global__ __launch_bounds__(64) void opp_sums(Layout L) {
         unsigned n = 0, kept = 0;
         double accum = 0.0, second = 0.0;
         float means[SLOTS], tols[SLOTS];
-        sum_pass<0>(L, g, begin, end, lane, means, tols, s_a, s_b, n, accum, second);
-        float mean = 0.0f, tolerance = 0.0f;
-        if (n != 0) {
-            const double cnt = (double)n;
-            const double m = accum / cnt;
-            const double variance = fmax(second / cnt - m * m, 0.0);
-            mean = (float)m;
-            tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
+#pragma unroll
+        for (int t = 0; t < SLOTS; ++t) means[t] = tols[t] = 0.0f;
+        sum_pass<0>(L, g, begin, end, means, tols, s_stage, n, accum, second);
+        if (wave == 0 && lane < SLOTS) {
+            float mean = 0.0f, tolerance = 0.0f;
+            if (n != 0) {
+                const double cnt = (double)n;
+                const double m = accum / cnt;
+                const double variance = fmax(second / cnt - m * m, 0.0);
+                mean = (float)m;
+                tolerance = (float)(2.0 * sqrt(variance)) + 1e-5f;
+            }
+            s_mean[lane] = mean;
+            s_tol[lane] = tolerance;
         }
+        __syncthreads();
 #pragma unroll
         for (int t = 0; t < SLOTS; ++t) {
-            means[t] = __shfl(mean, t, 64);
-            tols[t] = __shfl(tolerance, t, 64);
+            means[t] = s_mean[t];
+            tols[t] = s_tol[t];
         }
         accum = 0.0;
         second = 0.0;
-        sum_pass<1>(L, g, begin, end, lane, means, tols, s_a, s_b, kept, accum, second);
-        if (lane < SLOTS) {
+        sum_pass<1>(L, g, begin, end, means, tols, s_stage, kept, accum, second);
+        if (wave == 0 && lane < SLOTS) {
             // Slot::value: fewer than the minimum samples or nothing kept
             // leaves the slot empty.
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -705,6 +739,7 @@ __global__ __launch_bounds__(64) void opp_sums(Layout L) {
             }
             L.corrections[(size_t)g * SLOTS + lane] = out;
         }
+        __syncthreads();
     }
 }
 
@@ -721,11 +756,22 @@ __device__ __forceinline__ float from_total_key(int k) {
 // search), and the radius-3 ring holds 24 cells.
 constexpr int MAX_LOCAL = 26;
 
+// Sites blocks: SX x SY cells of one channel; the candidate window adds the
+// three-cell search radius on every side and is staged in LDS.
+constexpr int SX = 64, SY = 8, SR = 3;
+constexpr int WX = SX + 2 * SR, WY = SY + 2 * SR;
+
 // The estimate for one (cell, channel): every suspected site of that channel
 // in the cell gets the same one (opposed.rs:243-278 reads only cell data).
+// Cell data comes from the block's LDS window, where cells outside the image
+// carry no flags, so they fail every candidate test as the reference's
+// image-clamped rings skip them.
 struct Site {
     const Layout* L;
     const float* lv;
+    const float4* cells;  // LDS window
+    const float* pref;    // LDS window: pair reference for the target channel
+    const float* plr;     // LDS window: pair log ratio for the target channel
     int target;
     unsigned support;
     bool two;  // two-channel support: reference and ratio come from the pair planes
@@ -747,35 +793,33 @@ struct Site {
         return true;
     }
 
-    // One candidate of the local search (opposed.rs:295-312).
-    __device__ void candidate(int cx, int cy, int& count) const {
-        const size_t j = (size_t)cy * L->cw + cx;
-        const float4 cand = L->cell[j];
+    // One candidate of the local search (opposed.rs:295-312), at window index j.
+    __device__ void candidate(int j, int& count) const {
+        const float4 cand = cells[j];
         const unsigned f = flags_of(cand);
         if (!(usable(f, target) && clean(f, target))) return;
-        const size_t pj = (size_t)target * L->cells + j;
-        const float pref = two ? L->pair_ref[pj] : 0.0f;
         float value;
-        if (!correction(cand, target, support, pref, lv, value)) return;
+        if (!correction(cand, target, support, pref[j], lv, value)) return;
         float cr = 0.0f;
-        const bool hc = two && support_log_ratio(support, L->pair_lr[pj], cr);
+        const bool hc = two && support_log_ratio(support, plr[j], cr);
         if (!chromaticity_agrees(support, ht, tr, hc, cr)) return;
         if (count < MAX_LOCAL) keys[count * THREADS] = total_key(value);
         ++count;
     }
 
-    // opposed.rs:282-327: rings of radius 1..3 cells around the target cell;
-    // the first radius with at least three candidates gives their median.
-    __device__ bool local_correction(int tx, int ty, float& out, int& out_radius) const {
+    // opposed.rs:282-327: rings of radius 1..3 cells around the target cell
+    // (window index `centre`); the first radius with at least three
+    // candidates gives their median.
+    __device__ bool local_correction(int centre, float& out, int& out_radius) const {
         int count = 0;
-        for (int radius = 1; radius <= 3; ++radius) {
-            const int x0 = max(tx - radius, 0), x1 = min(tx + radius, L->cw - 1);
-            for (int cy = max(ty - radius, 0); cy <= min(ty + radius, L->ch - 1); ++cy) {
-                if (cy == ty - radius || cy == ty + radius) {
-                    for (int cx = x0; cx <= x1; ++cx) candidate(cx, cy, count);
+        for (int radius = 1; radius <= SR; ++radius) {
+            for (int dy = -radius; dy <= radius; ++dy) {
+                const int row = centre + dy * WX;
+                if (dy == -radius || dy == radius) {
+                    for (int dx = -radius; dx <= radius; ++dx) candidate(row + dx, count);
                 } else {
-                    if (tx - radius >= 0) candidate(tx - radius, cy, count);
-                    if (tx + radius < L->cw) candidate(tx + radius, cy, count);
+                    candidate(row - radius, count);
+                    candidate(row + radius, count);
                 }
             }
             if (count >= 3) {
@@ -807,14 +851,12 @@ struct Site {
     }
 
     // opposed.rs:243-278
-    __device__ bool estimate(const float4& cell, int tx, int ty, float& out) const {
-        const size_t pi = (size_t)target * L->cells + (size_t)ty * L->cw + tx;
+    __device__ bool estimate(const float4& cell, int centre, unsigned label, float& out) const {
         float target_reference;
-        if (!reference(cell, support, two ? L->pair_ref[pi] : 0.0f, target_reference)) return false;
+        if (!reference(cell, support, pref[centre], target_reference)) return false;
         float local = 0.0f;
         int radius = 0;
-        const bool has_local = local_correction(tx, ty, local, radius);
-        const unsigned label = L->label[(size_t)ty * L->cw + tx];
+        const bool has_local = local_correction(centre, local, radius);
         float region = 0.0f, br = 0.0f;
         bool bhr = false;
         const bool has_region =
@@ -838,64 +880,95 @@ struct Site {
 };
 
 // opposed.rs:97-162: one thread per (cell, channel) with a suspected site of
-// that channel; the estimate is computed once and applied to each such site.
+// that channel (two cells per thread); the estimate is computed once and
+// applied to each such site. Blocks without a suspected cell of their channel
+// leave before staging anything.
 __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
                                                      float l0, float l1, float l2, int has_ceiling,
                                                      float c0, float c1, float c2,
                                                      DeviceStats* stats) {
+    __shared__ float4 s_cell[WY * WX];
+    __shared__ float s_pref[WY * WX];
+    __shared__ float s_plr[WY * WX];
     __shared__ int s_keys[MAX_LOCAL * THREADS];
     const float lv[3] = {l0, l1, l2};
     const float ceiling[3] = {c0, c1, c2};
     const int channel = blockIdx.z;
-    const int tx = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int ty = blockIdx.y * (THREADS / 64) + (threadIdx.x >> 6);
+    const int bx0 = blockIdx.x * SX, by0 = blockIdx.y * SY;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    constexpr int ROWS_PER_THREAD = SY / (THREADS / 64);
+    bool mine = false;
+#pragma unroll
+    for (int q = 0; q < ROWS_PER_THREAD; ++q) {
+        const int cx = bx0 + tx, cy = by0 + ty + 4 * q;
+        if (cx < L.cw && cy < L.ch)
+            mine |= (flags_of(L.cell[(size_t)cy * L.cw + cx]) >> (channel + 3)) & 1u;
+    }
+    if (!__syncthreads_or(mine)) return;
+    const size_t plane = (size_t)channel * L.cells;
+    for (int k = threadIdx.x; k < WY * WX; k += THREADS) {
+        const int x = bx0 - SR + k % WX, y = by0 - SR + k / WX;
+        float4 cell = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float pr = 0.0f, pl = 0.0f;
+        if (x >= 0 && y >= 0 && x < L.cw && y < L.ch) {
+            const size_t j = (size_t)y * L.cw + x;
+            cell = L.cell[j];
+            pr = L.pair_ref[plane + j];
+            pl = L.pair_lr[plane + j];
+        }
+        s_cell[k] = cell;
+        s_pref[k] = pr;
+        s_plr[k] = pl;
+    }
+    __syncthreads();
     unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
-    if (tx < L.cw && ty < L.ch) {
-        const float4 target = L.cell[(size_t)ty * L.cw + tx];
+    for (int q = 0; q < ROWS_PER_THREAD; ++q) {
+        const int cx = bx0 + tx, cy = by0 + ty + 4 * q;
+        if (cx >= L.cw || cy >= L.ch) continue;
+        const int centre = (ty + 4 * q + SR) * WX + tx + SR;
+        const float4 target = s_cell[centre];
         const unsigned f = flags_of(target);
-        if (f & (1u << (channel + 3))) {
-            int support_count = 0;
-            for (int o = 0; o < 3; ++o)
-                if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
-            float est = 0.0f;
-            bool has = false;
-            if (support_count != 0) {
-                Site site{&L, lv, channel, support_mask(target, channel, lv), false, false, 0.0f,
-                          s_keys + threadIdx.x};
-                site.two = __popc(site.support) == 2;
-                const size_t pi = (size_t)channel * L.cells + (size_t)ty * L.cw + tx;
-                site.ht = site.two && support_log_ratio(site.support, L.pair_lr[pi], site.tr);
-                has = site.estimate(target, tx, ty, est);
-            }
-            const float level = lv[channel];
-            for (int dy = 0; dy < 2; ++dy)
-                for (int dx = 0; dx < 2; ++dx) {
-                    const int x = 2 * tx + dx, y = 2 * ty + dy;
-                    if (x >= L.w || y >= L.h || color_at(pattern, x, y) != channel) continue;
-                    float* sp = m + (size_t)y * L.w + x;
-                    const float sample = *sp;
-                    if (!(sample >= level)) continue;
-                    bool neutralise = false;
-                    if (!has) {
-                        fallback += 1;
-                        unsupported += support_count == 0;
-                        neutralise = true;
+        if (!(f & (1u << (channel + 3)))) continue;
+        int support_count = 0;
+        for (int o = 0; o < 3; ++o)
+            if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
+        float est = 0.0f;
+        bool has = false;
+        if (support_count != 0) {
+            Site site{&L, lv, s_cell, s_pref, s_plr, channel, support_mask(target, channel, lv),
+                      false, false, 0.0f, s_keys + threadIdx.x};
+            site.two = __popc(site.support) == 2;
+            site.ht = site.two && support_log_ratio(site.support, s_plr[centre], site.tr);
+            has = site.estimate(target, centre, L.label[(size_t)cy * L.cw + cx], est);
+        }
+        const float level = lv[channel];
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int x = 2 * cx + dx, y = 2 * cy + dy;
+                if (x >= L.w || y >= L.h || color_at(pattern, x, y) != channel) continue;
+                float* sp = m + (size_t)y * L.w + x;
+                const float sample = *sp;
+                if (!(sample >= level)) continue;
+                bool neutralise = false;
+                if (!has) {
+                    fallback += 1;
+                    unsupported += support_count == 0;
+                    neutralise = true;
+                } else {
+                    reconstructed += 1;
+                    const float bounded = fminf(est, 4.0f * level);
+                    if (bounded > sample) {
+                        *sp = bounded;
+                        changed += 1;
                     } else {
-                        reconstructed += 1;
-                        const float bounded = fminf(est, 4.0f * level);
-                        if (bounded > sample) {
-                            *sp = bounded;
-                            changed += 1;
-                        } else {
-                            neutralise = true;
-                        }
-                    }
-                    if (neutralise && has_ceiling && sample > ceiling[channel]) {
-                        *sp = ceiling[channel];
-                        neutralised += 1;
+                        neutralise = true;
                     }
                 }
-        }
+                if (neutralise && has_ceiling && sample > ceiling[channel]) {
+                    *sp = ceiling[channel];
+                    neutralised += 1;
+                }
+            }
     }
     unsigned v[5] = {reconstructed, changed, fallback, unsupported, neutralised};
     for (int k = 0; k < 5; ++k) {
@@ -997,9 +1070,9 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     RHD_TIMED(prof, s, "opp_contrib", 0,
               hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
     RHD_TIMED(prof, s, "opp_sums", 0,
-              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(64), 0, s, L));
+              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(128), 0, s, L));
     RHD_TIMED(prof, s, "opp_sites", 8 * px + 16 * cells,
-              hipLaunchKernelGGL(opp_sites, dim3((L.cw + 63) / 64, (L.ch + 3) / 4, 3),
+              hipLaunchKernelGGL(opp_sites, dim3((L.cw + SX - 1) / SX, (L.ch + SY - 1) / SY, 3),
                                  dim3(THREADS), 0, s, a.mosaic, L, a.pattern, l0, l1, l2,
                                  a.has_ceiling, a.ceiling[0], a.ceiling[1], a.ceiling[2],
                                  a.stats));
