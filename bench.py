@@ -36,7 +36,23 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain device copy of 1 GiB (read) into 1 GiB (write), the "
+                        "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args()
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """A plain f32 copy of known size: its FETCH_SIZE and WRITE_SIZE per
+    dispatch, against nbytes, give the counters' scale on this device."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "dispatches": 3}))
 
 
 def workload():
@@ -83,6 +99,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not supported "
               "(frames shard by frame id across GPUs with no exchange).", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     from rohditor_amd import _lib, sensor
 
     spec, frame, recipe, options = workload()

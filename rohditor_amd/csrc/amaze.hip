@@ -72,9 +72,10 @@ __device__ __forceinline__ int gae(int pattern, int row) { return fc(pattern, ro
 // ---------------------------------------------------------------- load
 // mod.rs:171-194 (populate) plus the zeroed state of scratch.rs:113-145.
 __global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int cc = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
     const int sx = mirror_whole(tx * VALID - BORDER + cc, a.width);
     const int sy = mirror_whole(ty * VALID - BORDER + rr, a.height);
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
         t.full(HALT)[i] = 0.0f;
     }
     if (rr == 0 && cc == 0) {
-        int* area = a.areas + 4 * (size_t)blockIdx.z;
+        int* area = a.areas + 4 * (size_t)blk.z;
         area[0] = INT_MAX;
         area[1] = -1;
         area[2] = INT_MAX;
@@ -110,9 +111,10 @@ __global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
 // ---------------------------------------------------------------- gradients
 // green.rs:13-31 and diagonal.rs:15-42.
 __global__ __launch_bounds__(256) void amz_gradients(AmazeArgs a) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int cc = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const float* cfa = t.full(CFA);
     if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
         const int i = rr * TS + cc;
@@ -155,9 +157,10 @@ __device__ __forceinline__ float adaptive_or(float ratio, float centre, float fa
 
 // green.rs:46-137
 __global__ __launch_bounds__(256) void amz_colour_differences(AmazeArgs a, float clip_pt8) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int cc = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     if (rr < 4 || rr >= TS - 4 || cc < 4 || cc >= TS - 4) return;
     const float* cfa = t.full(CFA);
     const float* w0 = t.full(DW0);
@@ -255,8 +258,9 @@ __device__ __forceinline__ float spread(float a, float b, float c) {
 // processed four at a time with the next four rows' inputs loaded up front,
 // so the loop waits for memory once per four rows.
 __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int cc = blockIdx.x * 64 + threadIdx.x;
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int cc = blk.x * 64 + threadIdx.x;
     if (cc < 4 || cc >= TS - 4) return;
     float* __restrict__ vcd = t.full(VCD);
     const float* __restrict__ alt = t.full(VALT);
@@ -328,8 +332,9 @@ __device__ __forceinline__ void bh_stage(float* lds, const float* __restrict__ p
 }
 
 __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int r0 = 4 + blockIdx.x * BH_ROWS;
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int r0 = 4 + blk.x * BH_ROWS;
     const int lane = threadIdx.x;
     const int rr = r0 + lane;
     __shared__ float s_hcd[BH_ROWS * BH_PITCH];
@@ -429,9 +434,10 @@ __device__ __forceinline__ float bound_towards(float estimate, float centre, flo
 }
 
 __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int p = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
     const int i = rr * TS + cc;
     bool flagged = false;
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
         const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
         const int c_lo = __shfl(cc, first, 64), c_hi = __shfl(cc, last, 64);
         if ((threadIdx.x & 63) == 0) {
-            int* area = a.areas + 4 * (size_t)blockIdx.z;
+            int* area = a.areas + 4 * (size_t)blk.z;
             atomicMin(area + 0, rr);
             atomicMax(area + 1, rr);
             atomicMin(area + 2, c_lo);
@@ -564,11 +570,12 @@ __device__ __forceinline__ bool in_area(const Area& ar, int pattern, int rr, int
 
 // nyquist.rs:100-118: neighbourhood vote.
 __global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
+    const uint3 blk = xcd_block();
     Area ar;
-    if (!tile_area(a, blockIdx.z, ar)) return;
-    const Tile t = tile_of(a, blockIdx.z);
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (!tile_area(a, blk.z, ar)) return;
+    const Tile t = tile_of(a, blk.z);
+    const int p = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
     if (!in_area(ar, a.pattern, rr, cc)) return;
     const int i = rr * TS + cc;
@@ -581,11 +588,12 @@ __global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
 
 // nyquist.rs:120-152: area interpolation of hvwt.
 __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
+    const uint3 blk = xcd_block();
     Area ar;
-    if (!tile_area(a, blockIdx.z, ar)) return;
-    const Tile t = tile_of(a, blockIdx.z);
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (!tile_area(a, blk.z, ar)) return;
+    const Tile t = tile_of(a, blk.z);
+    const int p = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
     if (!in_area(ar, a.pattern, rr, cc)) return;
     const int i = rr * TS + cc;
@@ -622,7 +630,8 @@ __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
 // workgroup per tile; thread p owns pair p; the previous row's final weights
 // live in LDS.
 __global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
-    const Tile t = tile_of(a, blockIdx.x);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.x);
     const int p = threadIdx.x;
     __shared__ float hv_prev[TSH];
     __shared__ float pm_prev[TSH];
@@ -734,9 +743,10 @@ __device__ __forceinline__ void red_offset(int pattern, int& ey, int& ex) {
 }
 
 __global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int p = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
     const int i = rr * TS + cc;
     const int h = i >> 1;
@@ -744,7 +754,7 @@ __global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
     float* dgrb0 = t.half(DGRB0);
     float* grb = t.half(GRB);
     Area ar;
-    if (tile_area(a, blockIdx.z, ar) && in_area(ar, a.pattern, rr, cc) && t.nyq2()[h] != 0) {
+    if (tile_area(a, blk.z, ar) && in_area(ar, a.pattern, rr, cc) && t.nyq2()[h] != 0) {
         auto average = [&](const float* pl) {
             return EPSSQ + (GQUINC[0] * pl[h] +
                             GQUINC[1] * (pl[(i - TS - 1) >> 1] + pl[(i - TS + 1) >> 1] +
@@ -794,9 +804,10 @@ __global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
 // ---------------------------------------------------------------- chroma
 // chroma.rs:34-96
 __global__ __launch_bounds__(256) void amz_chroma(AmazeArgs a) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int p = blk.x * 64 + (threadIdx.x & 63);
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
     if (rr < 14 || rr >= TS - 14) return;
     const int start = 14 + gae(a.pattern, rr);
     const int cc = 2 * p + gae(a.pattern, rr);
@@ -827,9 +838,10 @@ __global__ __launch_bounds__(256) void amz_chroma(AmazeArgs a) {
 // ---------------------------------------------------------------- finish + write
 // chroma.rs:103-147 (finish_planes) fused with mod.rs:196-273 (write_tile).
 __global__ __launch_bounds__(256) void amz_finish(AmazeArgs a) {
-    const Tile t = tile_of(a, blockIdx.z);
-    const int cc = BORDER + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rr = BORDER + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const int cc = BORDER + blk.x * 64 + (threadIdx.x & 63);
+    const int rr = BORDER + blk.y * 4 + (threadIdx.x >> 6);
     if (cc >= TS - BORDER || rr >= TS - BORDER) return;
     const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
     const int gx = tx * VALID + cc - BORDER, gy = ty * VALID + rr - BORDER;

@@ -94,19 +94,24 @@ __device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, 
 // Copies a (rows x cols) window of a global plane into LDS. The trip counts
 // are compile-time bounds (MAXROWS x MAXCOLS), so every load of the window is
 // issued before the first LDS store and the memory latency is paid once.
+// A wavefront stages whole rows, so each row's address is wave-uniform and
+// formed in scalar registers.
 template <int MAXROWS, int MAXCOLS>
 __device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ src,
                                       size_t src_pitch, int rows, int cols) {
     constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
-    const int ty = threadIdx.x >> 6, tx = threadIdx.x & 63;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
     float v[RI * CI];
 #pragma unroll
-    for (int i = 0; i < RI; ++i)
+    for (int i = 0; i < RI; ++i) {
+        const int r = ty + 4 * i;
+        const float* row = src + (size_t)r * src_pitch;
 #pragma unroll
         for (int j = 0; j < CI; ++j) {
-            const int r = ty + 4 * i, c = tx + 64 * j;
-            v[i * CI + j] = (r < rows && c < cols) ? src[(size_t)r * src_pitch + c] : 0.0f;
+            const int c = tx + 64 * j;
+            v[i * CI + j] = (r < rows && c < cols) ? row[c] : 0.0f;
         }
+    }
 #pragma unroll
     for (int i = 0; i < RI; ++i)
 #pragma unroll
@@ -123,7 +128,8 @@ template <int R, bool FAST>
 __device__ void blur_tile(const float* __restrict__ in, const float* __restrict__ aux,
                           float* __restrict__ out, int width, int height, const Weights& k,
                           int mode, float noise_protection, float* s_in, float* s_h) {
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
     const int cx0 = max(0, x0 - R), cx1 = min(width, x0 + TW + R);
     const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
     const int pw = cx1 - cx0, ph = ry1 - ry0;
@@ -156,7 +162,8 @@ __global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in
                                                     float noise_protection) {
     __shared__ float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
     __shared__ float s_h[(TH + 2 * MAXR) * TW];
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool fast = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
     if (fast)
         blur_tile<R, true>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
@@ -183,7 +190,8 @@ __device__ void iterate_tile(const float* __restrict__ est_in, const float* __re
                              float* s_e, float* s_h, float* s_g) {
     using Win = IterWindows<R>;
     constexpr int EW = Win::EW, HW = Win::HW, GW = Win::GW;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
     const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
     const int ex0 = max(0, x0 - 2 * R), ex1 = min(width, x0 + TW + 2 * R);
     const int ey0 = max(0, y0 - 2 * R), ey1 = min(height, y0 + TH + 2 * R);
@@ -216,55 +224,122 @@ __device__ void iterate_tile(const float* __restrict__ est_in, const float* __re
 
 // The same iteration for a tile whose windows lie inside the image: every
 // horizontal tap is interior (the sum starts at 0.0), no mirroring, and all
-// LDS offsets are compile-time constants. The guide values of the tile's
-// outputs are loaded with the windows, so the final clamp waits on nothing.
+// LDS offsets are compile-time constants. Each thread computes strips of 4
+// outputs (4 columns of a row in the horizontal passes, read as float4
+// chunks; 4 rows of a column in the vertical ones), so loads, addressing and
+// loop control are shared by 4 results; every output keeps the reference's
+// tap order. Strips that reach past a window's edge compute throw-away
+// values from the padding and are not stored. The guide values of the tile's
+// outputs are kept in registers before the ratio overwrites the guide window.
+template <int R>
+struct InteriorWindows {
+    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;   // estimate
+    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;   // guide / ratio
+    static constexpr int P = (GW + 3) / 4 * 4;              // pitch of s_h and s_g
+    static constexpr int E_FLOATS = EH * EW + 8;
+    static constexpr int H_FLOATS = (EH + 4) * P;
+    static constexpr int G_FLOATS = GH * P;
+    static constexpr int CHUNKS = (4 + 2 * R + 3) / 4;      // float4 reads per strip
+};
+
+// 4 horizontal outputs from the row segment starting at `src` (16-byte aligned).
+template <int R>
+__device__ __forceinline__ float4 hstrip(const float* src, const Weights& k) {
+    constexpr int N = InteriorWindows<R>::CHUNKS * 4;
+    float x[N];
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(src + 4 * q);
+        x[4 * q] = v.x;
+        x[4 * q + 1] = v.y;
+        x[4 * q + 2] = v.z;
+        x[4 * q + 3] = v.w;
+    }
+    float o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float v = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
+        o[q] = v;
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// 4 vertical outputs down the column starting at `src`, rows `pitch` apart.
+template <int R, int PITCH>
+__device__ __forceinline__ void vstrip(const float* src, const Weights& k, float o[4]) {
+    float x[4 + 2 * R];
+#pragma unroll
+    for (int i = 0; i < 4 + 2 * R; ++i) x[i] = src[i * PITCH];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float v = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
+        o[q] = v;
+    }
+}
+
 template <int R>
 __device__ void iterate_interior(const float* __restrict__ est_in, const float* __restrict__ guide,
                                  float* __restrict__ est_out, int width, const Weights& k,
                                  float* s_e, float* s_h, float* s_g) {
-    using Win = IterWindows<R>;
-    constexpr int EW = Win::EW, EH = Win::EH, HW = Win::HW, GW = Win::GW, GH = Win::GH;
-    constexpr int OUT_PER_THREAD = TW * TH / THREADS;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    float g_out[OUT_PER_THREAD];
-#pragma unroll
-    for (int q = 0; q < OUT_PER_THREAD; ++q)
-        g_out[q] = guide[(size_t)(y0 + ty + 4 * q) * width + x0 + tx];
+    using Win = InteriorWindows<R>;
+    constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
+    constexpr int NS1 = (GW + 3) / 4;   // column strips of the estimate's blur
+    constexpr int NR2 = (GH + 3) / 4;   // row strips of the blurred estimate
+    constexpr int NS3 = TW / 4;         // column strips of the ratio's blur
+    constexpr int NR4 = TH / 4;         // row strips of the output
+    constexpr int OUT_STRIPS = NR4 * TW / THREADS;
+    static_assert(TW == 64, "output strips map one wavefront to one tile row");
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
     stage<EH, EW>(s_e, EW, est_in + (size_t)(y0 - 2 * R) * width + x0 - 2 * R, width, EH, EW);
-    stage<GH, GW>(s_g, GW, guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
+    stage<GH, GW>(s_g, P, guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
     __syncthreads();
-    FOR_WINDOW(EH, GW, r, c) {
-        float v = 0.0f;
+    float g_out[OUT_STRIPS][4];
 #pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) v += s_e[r * EW + c + t] * k.w[t];
-        s_h[r * HW + c] = v;
+    for (int q = 0; q < OUT_STRIPS; ++q) {
+        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) g_out[q][i] = s_g[(r0 + i + R) * P + c + R];
+    }
+    for (int item = threadIdx.x; item < EH * NS1; item += THREADS) {
+        const int r = item / NS1, j = item % NS1;
+        *reinterpret_cast<float4*>(s_h + r * P + 4 * j) = hstrip<R>(s_e + r * EW + 4 * j, k);
     }
     __syncthreads();
-    FOR_WINDOW(GH, GW, r, c) {
-        float b = 0.0f;
+    for (int item = threadIdx.x; item < NR2 * GW; item += THREADS) {
+        const int c = item % GW, r0 = 4 * (item / GW);
+        float b[4];
+        vstrip<R, P>(s_h + r0 * P + c, k, b);
 #pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) b += s_h[(r + t) * HW + c] * k.w[t];
-        s_g[r * GW + c] = s_g[r * GW + c] / num_max(b, FLOOR);
+        for (int i = 0; i < 4; ++i)
+            if (r0 + i < GH) s_g[(r0 + i) * P + c] = s_g[(r0 + i) * P + c] / num_max(b[i], FLOOR);
     }
     __syncthreads();
-    FOR_WINDOW(GH, TW, r, c) {
-        float v = 0.0f;
-#pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) v += s_g[r * GW + c + t] * k.w[t];
-        s_h[r * HW + c] = v;
+    for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
+        const int r = item / NS3, j = item % NS3;
+        *reinterpret_cast<float4*>(s_h + r * P + 4 * j) = hstrip<R>(s_g + r * P + 4 * j, k);
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < OUT_PER_THREAD; ++q) {
-        const int r = ty + 4 * q, c = tx;
-        float b = 0.0f;
+    for (int q = 0; q < OUT_STRIPS; ++q) {
+        // TW == 64: a wavefront's strips share their rows.
+        const int c = threadIdx.x & 63;
+        const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+        float b[4];
+        vstrip<R, P>(s_h + r0 * P + c, k, b);
+        float* out_row = est_out + (size_t)(y0 + r0) * width + x0;
 #pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) b += s_h[(r + t) * HW + c] * k.w[t];
-        const float g = g_out[q];
-        const float v = s_e[(r + 2 * R) * EW + c + 2 * R] * b;
-        const float lo = g * 0.5f, hi = g * 2.0f;
-        est_out[(size_t)(y0 + r) * width + x0 + c] = v < lo ? lo : (v > hi ? hi : v);
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + i;
+            const float g = g_out[q][i];
+            const float v = s_e[(r + 2 * R) * EW + c + 2 * R] * b[i];
+            const float lo = g * 0.5f, hi = g * 2.0f;
+            out_row[(size_t)i * width + c] = v < lo ? lo : (v > hi ? hi : v);
+        }
     }
 }
 
@@ -274,10 +349,15 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
                                                        float* __restrict__ est_out, int width,
                                                        int height, Weights k) {
     using Win = IterWindows<R>;
-    __shared__ float s_e[Win::EH * Win::EW];
-    __shared__ float s_h[Win::EH * Win::HW];
-    __shared__ float s_g[Win::GH * Win::GW];
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    using IWin = InteriorWindows<R>;
+    __shared__ __align__(16) float s_e[IWin::E_FLOATS];
+    __shared__ __align__(16) float s_h[IWin::H_FLOATS];
+    __shared__ __align__(16) float s_g[IWin::G_FLOATS];
+    static_assert(IWin::E_FLOATS >= Win::EH * Win::EW && IWin::H_FLOATS >= Win::EH * Win::HW &&
+                      IWin::G_FLOATS >= Win::GH * Win::GW,
+                  "the border path's windows fit the interior path's arrays");
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool interior =
         x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
     if (interior)

@@ -65,8 +65,9 @@ __device__ __forceinline__ void bilinear_pixel(const Src& s, int x, int y, float
 __global__ __launch_bounds__(256) void bilinear_kernel(Src s, float g0, float g1, float g2,
                                                        float* __restrict__ out,
                                                        DeviceStats* stats) {
-    int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    int x = blk.x * 64 + (threadIdx.x & 63);
+    int y = blk.y * 4 + (threadIdx.x >> 6);
     if (x >= s.w || y >= s.h) return;
     float rgb[3];
     bilinear_pixel(s, x, y, rgb);
@@ -103,8 +104,9 @@ __device__ __forceinline__ float mhc_apply(const Src& s, int x, int y, int which
 
 __global__ __launch_bounds__(256) void mhc_kernel(Src s, float* __restrict__ out,
                                                   DeviceStats* stats) {
-    int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint3 blk = xcd_block();
+    int x = blk.x * 64 + (threadIdx.x & 63);
+    int y = blk.y * 4 + (threadIdx.x >> 6);
     if (x >= s.w || y >= s.h) return;
     float rgb[3];
     bool border = x < 2 || y < 2 || x + 2 >= s.w || y + 2 >= s.h;

@@ -16,6 +16,8 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int PER_THREAD = 4;
+// Row blocks per launch: each block strides over height / ROW_BLOCKS rows.
+constexpr int ROW_BLOCKS = 256;
 
 struct Counters {
     unsigned c[6];
@@ -57,24 +59,55 @@ __device__ __forceinline__ void classify(float& v, int c, int mode, const float 
     }
 }
 
+// One thread per 4 consecutive samples of a row, striding over rows so that
+// each block flushes its counters once. The level table is staged in LDS and
+// indexed by a phase that is stepped, not recomputed; rows whose u16 source
+// is 8-byte aligned load 4 samples at once, and rows of a width that is a
+// multiple of 4 store 4 floats at once.
 __global__ __launch_bounds__(BLOCK) void normalize_kernel(NormalizeArgs a, float* __restrict__ out,
                                                           DeviceStats* stats) {
-    const int y = blockIdx.y;
+    __shared__ float s_black[256], s_range[256];
+    const int px = a.table.period_x, entries = px * a.table.period_y;
+    if ((int)threadIdx.x < entries) {
+        s_black[threadIdx.x] = a.table.black[threadIdx.x];
+        s_range[threadIdx.x] = a.table.range[threadIdx.x];
+    }
+    __syncthreads();
     const int x0 = (blockIdx.x * BLOCK + threadIdx.x) * PER_THREAD;
     Counters k = {{0, 0, 0, 0, 0, 0}};
-    const int sy = a.crop_y + y;
-    const int ty = sy % a.table.period_y;
-    const uint16_t* src = a.mosaic + (uint64_t)sy * a.row_stride + a.crop_x;
-    float* dst = out + (size_t)y * a.width;
+    for (int y = blockIdx.y; y < a.height && x0 < a.width; y += gridDim.y) {
+        const int sy = a.crop_y + y;
+        const int row_t = (sy % a.table.period_y) * px;
+        int phase = (a.crop_x + x0) % px;
+        const uint64_t first = (uint64_t)sy * a.row_stride + a.crop_x;
+        const uint16_t* src = a.mosaic + first;
+        float* dst = out + (size_t)y * a.width;
+        const bool full = x0 + PER_THREAD <= a.width;
+        uint16_t raw[PER_THREAD];
+        if (full && (first & 3) == 0) {
+            const ushort4 q = *reinterpret_cast<const ushort4*>(src + x0);
+            raw[0] = q.x;
+            raw[1] = q.y;
+            raw[2] = q.z;
+            raw[3] = q.w;
+        } else {
 #pragma unroll
-    for (int i = 0; i < PER_THREAD; ++i) {
-        int x = x0 + i;
-        if (x < a.width) {
-            int sx = a.crop_x + x;
-            int t = ty * a.table.period_x + sx % a.table.period_x;
-            float v = ((float)src[x] - a.table.black[t]) / a.table.range[t];
-            classify(v, color_at(a.crop_pattern, x, y), a.mode, a.levels, k);
-            dst[x] = v;
+            for (int i = 0; i < PER_THREAD; ++i) raw[i] = x0 + i < a.width ? src[x0 + i] : 0;
+        }
+        float v[PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < PER_THREAD; ++i) {
+            const int t = row_t + phase;
+            v[i] = ((float)raw[i] - s_black[t]) / s_range[t];
+            if (x0 + i < a.width) classify(v[i], color_at(a.crop_pattern, x0 + i, y), a.mode, a.levels, k);
+            phase = phase + 1 == px ? 0 : phase + 1;
+        }
+        if (full && (a.width & 3) == 0) {
+            *reinterpret_cast<float4*>(dst + x0) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < PER_THREAD; ++i)
+                if (x0 + i < a.width) dst[x0 + i] = v[i];
         }
     }
     if (a.mode == 1) flush_counters(k, 6, stats->clip);
@@ -86,24 +119,25 @@ __global__ __launch_bounds__(BLOCK) void classify_kernel(float* __restrict__ pla
                                                          int height, int pattern, int mode,
                                                          float l0, float l1, float l2,
                                                          DeviceStats* stats) {
-    const int y = blockIdx.y;
     const int x0 = (blockIdx.x * BLOCK + threadIdx.x) * PER_THREAD;
     const float levels[3] = {l0, l1, l2};
     Counters k = {{0, 0, 0, 0, 0, 0}};
     unsigned bad = 0;
-    float* row = plane + (size_t)y * width;
+    for (int y = blockIdx.y; y < height; y += gridDim.y) {
+        float* row = plane + (size_t)y * width;
 #pragma unroll
-    for (int i = 0; i < PER_THREAD; ++i) {
-        int x = x0 + i;
-        if (x < width) {
-            float v = row[x];
-            if (!is_finite(v)) {
-                bad = 1;
-                continue;
+        for (int i = 0; i < PER_THREAD; ++i) {
+            int x = x0 + i;
+            if (x < width) {
+                float v = row[x];
+                if (!is_finite(v)) {
+                    bad = 1;
+                    continue;
+                }
+                float before = v;
+                classify(v, color_at(pattern, x, y), mode, levels, k);
+                if (v != before) row[x] = v;
             }
-            float before = v;
-            classify(v, color_at(pattern, x, y), mode, levels, k);
-            if (v != before) row[x] = v;
         }
     }
     if (bad) atomicOr(&stats->nonfinite, 1u);
@@ -114,7 +148,7 @@ __global__ __launch_bounds__(BLOCK) void classify_kernel(float* __restrict__ pla
 
 hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
                             hipStream_t s, Profiler* prof) {
-    dim3 grid((a.width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), a.height);
+    dim3 grid((a.width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), std::min(a.height, ROW_BLOCKS));
     RHD_TIMED(prof, s, "normalize", 6ull * a.width * a.height,
               hipLaunchKernelGGL(normalize_kernel, grid, dim3(BLOCK), 0, s, a, out, stats));
     return hipGetLastError();
@@ -122,7 +156,7 @@ hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* sta
 
 hipError_t launch_clip(float* plane, int width, int height, int pattern, const float limits[3],
                        DeviceStats* stats, hipStream_t s) {
-    dim3 grid((width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), height);
+    dim3 grid((width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), std::min(height, ROW_BLOCKS));
     hipLaunchKernelGGL(classify_kernel, grid, dim3(BLOCK), 0, s, plane, width, height, pattern, 1,
                        limits[0], limits[1], limits[2], stats);
     return hipGetLastError();
@@ -130,7 +164,7 @@ hipError_t launch_clip(float* plane, int width, int height, int pattern, const f
 
 hipError_t launch_detect(const float* plane, int width, int height, int pattern,
                          const float levels[3], DeviceStats* stats, hipStream_t s) {
-    dim3 grid((width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), height);
+    dim3 grid((width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), std::min(height, ROW_BLOCKS));
     hipLaunchKernelGGL(classify_kernel, grid, dim3(BLOCK), 0, s, const_cast<float*>(plane), width,
                        height, pattern, 2, levels[0], levels[1], levels[2], stats);
     return hipGetLastError();

@@ -64,6 +64,7 @@ struct Layout {
     unsigned* region_first;
     unsigned* region_last;
     float4* corrections;           // per (group, slot): offset, ratio, valid, has_ratio
+    unsigned* site_counts;         // per opp_sites block: its five statistics
     unsigned* scalars;             // [0] region count, [1] ring count, [2] any wanted
 };
 
@@ -250,7 +251,8 @@ constexpr int LHX = LX + 2, LHY = LY + 2;
 __global__ __launch_bounds__(THREADS) void opp_label_tiles(Layout L) {
     __shared__ unsigned char s_clip[LHY * LHX];
     __shared__ unsigned s_par[LY * LX];
-    const int x0 = blockIdx.x * LX, y0 = blockIdx.y * LY;
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * LX, y0 = blk.y * LY;
     for (int k = threadIdx.x; k < LHY * LHX; k += THREADS) {
         const int x = x0 + k % LHX - 1, y = y0 + k / LHX - 1;
         bool clipped = false;
@@ -589,9 +591,12 @@ __device__ __forceinline__ void sum_load(const Layout& L, unsigned g, unsigned b
 // Staging buffers of one batch: per slot the exact f64 terms it adds, the
 // number of contributing entries, and whether any entry of the batch belongs
 // to the group.
+// Slot rows are SUM_PITCH doubles apart (2 more than a batch) so that the
+// nine lanes reading them start in different LDS banks.
+constexpr int SUM_PITCH = SUM_BATCH + 2;
 struct SumStage {
-    double a[SLOTS * SUM_BATCH];
-    double b[SLOTS * SUM_BATCH];
+    double a[SLOTS * SUM_PITCH];
+    double b[SLOTS * SUM_PITCH];
     unsigned count[SLOTS];
     unsigned staged;
 };
@@ -642,8 +647,8 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
                             ta = take ? (double)v : 0.0;
                             tb = take && ratio ? (double)b.rat[q][t / 3] : 0.0;
                         }
-                        out.a[t * SUM_BATCH + e] = ta;
-                        out.b[t * SUM_BATCH + e] = tb;
+                        out.a[t * SUM_PITCH + e] = ta;
+                        out.b[t * SUM_PITCH + e] = tb;
                         counts[t] += __popcll(__ballot(take));
                     }
                 }
@@ -661,8 +666,8 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
             const SumStage& in = st[(i - 1) & 1];
             if (in.staged) {
                 count += in.count[lane];
-                const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_BATCH);
-                const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_BATCH);
+                const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_PITCH);
+                const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_PITCH);
                 for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
                     double2 a[SUM_UNROLL], c2[SUM_UNROLL];
 #pragma unroll
@@ -893,8 +898,9 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     __shared__ int s_keys[MAX_LOCAL * THREADS];
     const float lv[3] = {l0, l1, l2};
     const float ceiling[3] = {c0, c1, c2};
-    const int channel = blockIdx.z;
-    const int bx0 = blockIdx.x * SX, by0 = blockIdx.y * SY;
+    const uint3 blk = xcd_block();
+    const int channel = blk.z;
+    const int bx0 = blk.x * SX, by0 = blk.y * SY;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     constexpr int ROWS_PER_THREAD = SY / (THREADS / 64);
     bool mine = false;
@@ -904,7 +910,12 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
         if (cx < L.cw && cy < L.ch)
             mine |= (flags_of(L.cell[(size_t)cy * L.cw + cx]) >> (channel + 3)) & 1u;
     }
-    if (!__syncthreads_or(mine)) return;
+    unsigned* counts = L.site_counts +
+                       5 * (size_t)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    if (!__syncthreads_or(mine)) {
+        if (threadIdx.x < 5) counts[threadIdx.x] = 0;
+        return;
+    }
     const size_t plane = (size_t)channel * L.cells;
     for (int k = threadIdx.x; k < WY * WX; k += THREADS) {
         const int x = bx0 - SR + k % WX, y = by0 - SR + k / WX;
@@ -970,11 +981,36 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
                 }
             }
     }
+    // Per-block totals go to the block's own slot (no shared counters);
+    // opp_site_stats adds them up.
+    __shared__ unsigned s_part[THREADS / 64][5];
     unsigned v[5] = {reconstructed, changed, fallback, unsupported, neutralised};
     for (int k = 0; k < 5; ++k) {
-        unsigned t = wave_sum(v[k]);
-        if ((threadIdx.x & 63) == 0 && t) atomicAdd(stats->opposed + k, (unsigned long long)t);
+        const unsigned t = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6][k] = t;
     }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned t = 0;
+        for (int w = 0; w < THREADS / 64; ++w) t += s_part[w][threadIdx.x];
+        counts[threadIdx.x] = t;
+    }
+}
+
+// Sum of the opp_sites block totals into the develop statistics.
+__global__ __launch_bounds__(THREADS) void opp_site_stats(Layout L, unsigned blocks,
+                                                          DeviceStats* stats) {
+    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    for (unsigned b = blockIdx.x * THREADS + threadIdx.x; b < blocks; b += gridDim.x * THREADS)
+        for (int k = 0; k < 5; ++k) v[k] += L.site_counts[5 * (size_t)b + k];
+    for (int k = 0; k < 5; ++k) {
+        const unsigned long long t = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(stats->opposed + k, t);
+    }
+}
+
+size_t site_blocks(const Layout& L) {
+    return (size_t)((L.cw + SX - 1) / SX) * ((L.ch + SY - 1) / SY) * 3;
 }
 
 Layout layout_of(const OpposedArgs& a) {
@@ -1010,6 +1046,7 @@ Layout layout_of(const OpposedArgs& a) {
     L.region_first = (unsigned*)take(cap * 4);
     L.region_last = (unsigned*)take(cap * 4);
     L.corrections = (float4*)take(cap * SLOTS * sizeof(float4));
+    L.site_counts = (unsigned*)take(site_blocks(L) * 5 * sizeof(unsigned));
     L.scalars = (unsigned*)take(64);
     return L;
 }
@@ -1076,6 +1113,9 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
                                  dim3(THREADS), 0, s, a.mosaic, L, a.pattern, l0, l1, l2,
                                  a.has_ceiling, a.ceiling[0], a.ceiling[1], a.ceiling[2],
                                  a.stats));
+    RHD_TIMED(prof, s, "opp_site_stats", 20 * site_blocks(L),
+              hipLaunchKernelGGL(opp_site_stats, dim3(64), dim3(THREADS), 0, s, L,
+                                 (unsigned)site_blocks(L), a.stats));
     return hipGetLastError();
 }
 

@@ -84,9 +84,10 @@ __device__ __forceinline__ float interpolate(float d, float h, float v) {
 }
 
 #define RCD_SITE                                                   \
-    const Tile t = tile_of(a, blockIdx.z);                         \
-    const int col = blockIdx.x * 64 + (threadIdx.x & 63);          \
-    const int row = blockIdx.y * 4 + (threadIdx.x >> 6);           \
+    const uint3 blk = xcd_block();                                 \
+    const Tile t = tile_of(a, blk.z);                              \
+    const int col = blk.x * 64 + (threadIdx.x & 63);               \
+    const int row = blk.y * 4 + (threadIdx.x >> 6);                \
     if (col >= TS || row >= TS) return;                            \
     const int i = row * TS + col;                                  \
     const bool rb = (color_at(a.pattern, col, row) & 1) == 0;

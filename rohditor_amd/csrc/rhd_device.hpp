@@ -54,6 +54,21 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
     return v;
 }
 
+// MI355X dispatches workgroups to its 8 XCDs round-robin by linear workgroup
+// id, and each XCD has its own L2. xcd_block() renumbers the grid so that XCD
+// k walks the k-th contiguous eighth of it in order: blocks that share
+// stencil halos (the rows of one tile, neighbouring tiles) then run on the
+// same XCD and find the halo in its L2. A bijection on the grid; the last
+// (count mod 8) blocks keep their own id.
+constexpr unsigned XCDS = 8;
+__device__ __forceinline__ uint3 xcd_block() {
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned per = gx * gy * gz / XCDS;
+    const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned v = b < per * XCDS ? (b % XCDS) * per + b / XCDS : b;
+    return make_uint3(v % gx, (v / gx) % gy, v / (gx * gy));
+}
+
 // Device-side counters of one develop call; read back once at the end.
 struct DeviceStats {
     unsigned long long clip[6];     // affected, changed, over-white, affected by channel
