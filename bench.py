@@ -39,7 +39,82 @@ def parse():
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain device copy of 1 GiB (read) into 1 GiB (write), the "
                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
+    p.add_argument("--rocprof-summary", metavar="RESULTS_DB",
+                   help="print the per-kernel table of a rocprofv3 --kernel-trace --stats "
+                        "results database and exit (no GPU needed)")
+    p.add_argument("--traffic-summary", nargs=4,
+                   metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                   help="print per-kernel HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE "
+                        "and WRITE_SIZE counter CSVs of this bench and of --calibrate-copy, "
+                        "scaled by the copy kernel's known bytes, and exit (no GPU needed)")
     return p.parse_args()
+
+
+# Per-kernel HBM traffic measured with rocprofv3 counters (--traffic-summary
+# output); the live roofline reports the dominant kernel's row when present.
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_develop_24mp.txt"
+COPY_BYTES = 1 << 30
+
+
+def short_kernel_name(name):
+    """`rhd::(anonymous namespace)::cap_iterate<3>(...)` -> `cap_iterate<3>`."""
+    import re
+    m = re.search(r"::(\w+)(<\d+>)?\(", name)
+    return (m.group(1) + (m.group(2) or "")) if m else name.split("(")[0][:48]
+
+
+def rocprof_summary(db):
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = list(con.execute(
+        "select name, total_calls, total_duration, average, percentage from top_kernels"))
+    print(f"{'kernel':28s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'%':>6s}")
+    for name, calls, total, avg, pct in rows:
+        # top_kernels holds microseconds.
+        print(f"{short_kernel_name(name):28s} {calls:6d} {total / 1e3:10.3f} {avg / 1e3:9.4f} "
+              f"{pct:6.2f}")
+
+
+def counter_means(path, counter):
+    """{kernel: (dispatches, mean counter value)} from a rocprofv3 counter CSV."""
+    import csv
+    acc = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            k = short_kernel_name(row["Kernel_Name"])
+            n, total = acc.get(k, (0, 0.0))
+            acc[k] = (n + 1, total + float(row["Counter_Value"]))
+    return {k: (n, total / n) for k, (n, total) in acc.items()}
+
+
+def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
+    fetch = counter_means(fetch_csv, "FETCH_SIZE")
+    write = counter_means(write_csv, "WRITE_SIZE")
+    copy_f = counter_means(copy_fetch_csv, "FETCH_SIZE")["__amd_rocclr_copyBuffer"][1]
+    copy_w = counter_means(copy_write_csv, "WRITE_SIZE")["__amd_rocclr_copyBuffer"][1]
+    sf, sw = COPY_BYTES / (copy_f * 1024), COPY_BYTES / (copy_w * 1024)
+    print(f"# copy kernel: FETCH_SIZE {copy_f:.1f} KiB, WRITE_SIZE {copy_w:.1f} KiB per dispatch "
+          f"for {COPY_BYTES} bytes each way")
+    print(f"# scale: fetch x{sf:.4f}, write x{sw:.4f} (bytes per counted byte)")
+    print(f"{'kernel':28s} {'dispatches':>10s} {'fetch_bytes':>14s} {'write_bytes':>14s} "
+          f"{'hbm_bytes':>14s}")
+    for k, (n, f) in fetch.items():
+        fb = f * 1024 * sf
+        wb = write.get(k, (0, 0.0))[1] * 1024 * sw
+        print(f"{k:28s} {n:10d} {fb:14.0f} {wb:14.0f} {fb + wb:14.0f}")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed traffic summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 5 and not line.startswith("#") and parts[0] == kernel:
+            return float(parts[4])
+    return None
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -102,6 +177,12 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary)
+        return
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary)
+        return
     from rohditor_amd import _lib, sensor
 
     spec, frame, recipe, options = workload()
@@ -142,8 +223,15 @@ def main():
     dom_ms, dom_launches, dom_bytes = prof[dominant]
     avg_ms = dom_ms / max(dom_launches, 1)
     achieved = dom_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # rocprof names the dominant kernel's instantiation (cap_iterate<3> for
+    # sigma 0.7); the traffic file is keyed by that name.
+    traffic = None
+    for name in (dominant, dominant + "<3>"):
+        traffic = traffic if traffic is not None else measured_traffic(name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": None if traffic is None else int(traffic),
+                "traffic_source": None if traffic is None else str(TRAFFIC_FILE.relative_to(ROOT)),
                 "kernel": dominant, "avg_launch_ms": round(avg_ms, 4),
                 "launches_per_step": dom_launches / args.steps,
                 "algorithmic_bytes_per_launch": int(dom_bytes),
