@@ -36,6 +36,9 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-served", action="store_true")
+    p.add_argument("--served-lanes", type=int, default=3)
+    p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain device copy of 1 GiB (read) into 1 GiB (write), the "
                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -239,6 +242,7 @@ def main():
     stage = {"stage_GBps": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9, 1),
              "stage_frac": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
 
+    served_line = None if args.no_served else served(frame, desc, detail, value, args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(frame, recipe, options)
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -256,9 +260,52 @@ def main():
         "stage_roofline": stage,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(
             prof.items(), key=lambda kv: -kv[1][0])},
+        "served": served_line,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def served(frame, desc, detail, plain_value, args):
+    """The same develop through the serving loop (include/rhd.h rhd_server_*):
+    a closed loop keeping 2 x lanes requests in flight, each on its own
+    resident frame id (so newest-wins never coalesces them), no base cache
+    (every request runs the whole stage). Latency is submit -> RGB ready."""
+    from rohditor_amd import sensor
+    lanes, frames = args.served_lanes, 4 * args.served_lanes
+    in_flight, total = 2 * lanes, args.served_requests
+    proc = sensor.GpuSensorProcessor(0)
+    server = sensor.GpuServer(proc, lanes=lanes)
+    for f in range(frames):
+        server.put_frame(f, frame)
+    mp = desc.crop.width * desc.crop.height / 1e6
+
+    def run(n):
+        pending, nxt, done = [], 0, 0
+        while done < n:
+            while len(pending) < in_flight and nxt < n:
+                pending.append(server.submit(nxt % frames, desc, detail))
+                nxt += 1
+            r = server.wait(pending.pop(0), keep=False)
+            if not r.ok:
+                raise RuntimeError(f"served develop failed with status {r.status}")
+            done += 1
+
+    run(2 * in_flight)  # warm-up: every lane's scratch and the device pool
+    server.reset_stats()
+    t0 = time.perf_counter()
+    run(total)
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    server.close()
+    proc.close()
+    value = mp * total / wall
+    return {"value": round(value, 2), "unit": "MP/s", "vs_plain": round(value / plain_value, 3),
+            "latency_ms": {"p50": round(st.p50_ms, 3), "p99": round(st.p99_ms, 3),
+                           "mean": round(st.mean_ms, 3)},
+            "requests": total, "lanes": lanes, "in_flight": in_flight,
+            "workload": "closed loop over resident copies of the same frame and recipe, "
+                        "no base cache"}
 
 
 def profile_table(proc):

@@ -224,6 +224,68 @@ uint32_t rhd_profile_count(rhd_context* ctx);
 rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,
                              uint64_t* launches, uint64_t* bytes_per_launch);
 
+/* ---------------------------------------------------------------------------
+ * Serving loop: the counterpart of the engine worker's develop loop
+ * (crates/engine/src/coordinator/worker.rs:723-800) with its newest-wins
+ * mailbox (crates/engine/src/coordinator/scheduler.rs:44-173) and the
+ * resident caches of SpatialCache (crates/engine/src/preview_cache/
+ * spatial.rs:322-611), as continuous batching on one GPU. Requests name a
+ * frame made resident with rhd_server_put_mosaic; up to `lanes` run at once,
+ * each on its own HIP stream, admitted in arrival order while the device
+ * bytes of the running ones stay within `admission_bytes`. A newer request
+ * for a frame coalesces that frame's queued requests (they complete as
+ * Cancelled) and cancels its running one at the next stage boundary.
+ * Demosaiced bases (camera RGB before capture sharpening) are cached per
+ * frame and sensor recipe, so a re-develop with the same sensor settings only
+ * runs capture sharpening. The server holds a reference on its context and
+ * uses lanes 1..lanes and 7; do not call the staged API on the same context
+ * from other threads meanwhile.
+ * ------------------------------------------------------------------------- */
+typedef struct rhd_server rhd_server;
+
+typedef struct rhd_server_config {
+    uint32_t lanes;            /* concurrent develops, 1..6; 0 selects 3 */
+    uint64_t admission_bytes;  /* device bytes running requests may hold; 0 = half the budget */
+    uint64_t base_cache_bytes; /* device bytes of cached demosaiced bases; 0 disables */
+} rhd_server_config;
+
+typedef struct rhd_request_info {
+    int32_t status;            /* rhd_status of the request */
+    uint32_t lane;             /* lane it ran on (0 if it never ran) */
+    int32_t base_cache_hit;    /* 1 if it reused a cached demosaiced base */
+    double queued_ms, run_ms;  /* submit -> start, start -> RGB ready */
+    double latency_ms;         /* submit -> RGB ready on the device */
+    double megapixels;
+    rhd_metrics metrics;       /* stage metrics (zero for a base hit) */
+} rhd_request_info;
+
+typedef struct rhd_server_stats {
+    uint64_t submitted, completed, cancelled, coalesced, failed, base_hits;
+    double p50_ms, p99_ms, mean_ms; /* latency of completed requests */
+    double megapixels;              /* sensor megapixels of completed requests */
+    double span_ms;                 /* first submit -> last completion */
+    uint64_t base_cache_bytes, running_bytes;
+} rhd_server_stats;
+
+rhd_status rhd_server_create(rhd_context* ctx, const rhd_server_config* config, rhd_server** out);
+/* Cancels queued requests, waits for running ones, frees what the server holds. */
+void rhd_server_destroy(rhd_server* server);
+/* Makes a decoded frame resident under `frame_id` (replacing a previous one). */
+rhd_status rhd_server_put_mosaic(rhd_server* server, uint64_t frame_id, const uint16_t* host,
+                                 uint64_t row_stride, uint32_t width, uint32_t height);
+rhd_status rhd_server_drop_mosaic(rhd_server* server, uint64_t frame_id);
+/* Queues a develop of a resident frame; the descriptions are copied. */
+rhd_status rhd_server_submit(rhd_server* server, uint64_t frame_id, const rhd_sensor_desc* desc,
+                             const rhd_detail_desc* detail, uint64_t* request_id);
+/* Waits up to `timeout_ms` (< 0: forever) for a request. Returns
+ * Synchronization if it is still pending (it stays collectable); otherwise
+ * its status, and on success hands over the developed source (or releases
+ * it when `out` is NULL). A request is collected once. */
+rhd_status rhd_server_wait(rhd_server* server, uint64_t request_id, int32_t timeout_ms,
+                           rhd_source** out, rhd_request_info* info);
+rhd_status rhd_server_read_stats(rhd_server* server, rhd_server_stats* out);
+rhd_status rhd_server_reset_stats(rhd_server* server);
+
 #ifdef __cplusplus
 }
 #endif

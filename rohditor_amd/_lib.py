@@ -61,6 +61,27 @@ class Metrics(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("lanes", c_uint32), ("admission_bytes", c_uint64),
+                ("base_cache_bytes", c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", c_int32), ("lane", c_uint32), ("base_cache_hit", c_int32),
+                ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("megapixels", ctypes.c_double),
+                ("metrics", Metrics)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("cancelled", c_uint64),
+                ("coalesced", c_uint64), ("failed", c_uint64), ("base_hits", c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("megapixels", ctypes.c_double),
+                ("span_ms", ctypes.c_double), ("base_cache_bytes", c_uint64),
+                ("running_bytes", c_uint64)]
+
+
 # Every symbol include/rhd.h declares, with (restype, argtypes).
 _P = c_void_p
 SIGNATURES = {
@@ -98,6 +119,15 @@ SIGNATURES = {
     "rhd_profile_count": (c_uint32, [_P]),
     "rhd_profile_entry": (c_int, [_P, c_uint32, POINTER(c_char_p), POINTER(ctypes.c_double),
                                   POINTER(c_uint64), POINTER(c_uint64)]),
+    "rhd_server_create": (c_int, [_P, POINTER(ServerConfig), POINTER(_P)]),
+    "rhd_server_destroy": (None, [_P]),
+    "rhd_server_put_mosaic": (c_int, [_P, c_uint64, _P, c_uint64, c_uint32, c_uint32]),
+    "rhd_server_drop_mosaic": (c_int, [_P, c_uint64]),
+    "rhd_server_submit": (c_int, [_P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc),
+                                  POINTER(c_uint64)]),
+    "rhd_server_wait": (c_int, [_P, c_uint64, c_int32, POINTER(_P), POINTER(RequestInfo)]),
+    "rhd_server_read_stats": (c_int, [_P, POINTER(ServerStats)]),
+    "rhd_server_reset_stats": (c_int, [_P]),
 }
 
 _lib = None

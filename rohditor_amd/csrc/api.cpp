@@ -17,51 +17,29 @@
 #include <string>
 #include <vector>
 
+#include "rhd_context.hpp"
 #include "rhd_internal.hpp"
 
 using namespace rhd;
 
-namespace {
-
-constexpr uint64_t DEFAULT_BUDGET = 64ull << 30;  // 64 GiB of the 288 GiB HBM3E
-constexpr int MAX_TILES_PER_LAUNCH = 2048;
-
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace
-
-struct rhd_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::atomic<uint64_t> budget{DEFAULT_BUDGET};
-    std::atomic<uint64_t> reserved{0};
-    std::string last_error;
-    DeviceStats* d_stats = nullptr;
-    DeviceStats* h_stats = nullptr;
-    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed;
-    hipEvent_t ev[5] = {};
-    bool profiling = false;
-    Profiler profiler;
-    std::atomic<int> refs{1};  // rhd_create's reference plus one per live handle
-};
-
-namespace {
-void ctx_unref(rhd_context* ctx) {
+namespace rhd {
+void context_unref(rhd_context* ctx) {
     if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
-    for (auto& ev : ctx->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (ctx->d_stats) (void)hipFree(ctx->d_stats);
-    if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (Lane& L : ctx->lanes) {
+        if (!L.open) continue;
+        for (auto& ev : L.ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (L.d_stats) (void)hipFree(L.d_stats);
+        if (L.h_stats) (void)hipHostFree(L.h_stats);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    for (const PooledBlock& b : ctx->pool) (void)hipFree(b.ptr);
     delete ctx;
 }
-}  // namespace
+}  // namespace rhd
 
-#define PROF(ctx) ((ctx)->profiling ? &(ctx)->profiler : nullptr)
+#define PROF(L) ((L).profiling ? &(L).profiler : nullptr)
 
 namespace rhd {
 
@@ -125,33 +103,13 @@ Profiler::~Profiler() {
 
 }  // namespace rhd
 
-struct rhd_mosaic {
-    rhd_context* ctx;
-    uint16_t* d;
-    uint64_t row_stride;
-    uint32_t width, height;
-    uint64_t bytes;
-};
-
-struct rhd_plane {
-    rhd_context* ctx;
-    float* d;
-    uint32_t width, height;
-    int32_t pattern;
-    uint64_t bytes;
-};
-
-struct rhd_source {
-    rhd_context* ctx;
-    float* d;
-    uint32_t width, height;
-    uint64_t bytes;
-};
-
-namespace {
+namespace rhd {
 
 rhd_status fail(rhd_context* ctx, rhd_status status, const std::string& msg) {
-    if (ctx) ctx->last_error = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->last_error = msg;
+    }
     return status;
 }
 
@@ -181,33 +139,82 @@ rhd_status reserve(rhd_context* ctx, uint64_t bytes) {
 }
 void unreserve(rhd_context* ctx, uint64_t bytes) { ctx->reserved.fetch_sub(bytes); }
 
-void free_buffer(rhd_context* ctx, DeviceBuffer& b) {
-    if (b.ptr) {
-        (void)hipFree(b.ptr);
-        unreserve(ctx, b.bytes);
+// Frees pooled blocks (largest first) until the pool plus live bytes plus
+// `need` fit the budget, or all of them when `all`.
+void pool_trim(rhd_context* ctx, uint64_t need, bool all) {
+    std::vector<PooledBlock> victims;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        std::sort(ctx->pool.begin(), ctx->pool.end(),
+                  [](const PooledBlock& a, const PooledBlock& b) { return a.bytes < b.bytes; });
+        while (!ctx->pool.empty() &&
+               (all || ctx->reserved.load() + ctx->pooled + need > ctx->budget.load())) {
+            victims.push_back(ctx->pool.back());
+            ctx->pooled -= ctx->pool.back().bytes;
+            ctx->pool.pop_back();
+        }
     }
+    for (const PooledBlock& b : victims) (void)hipFree(b.ptr);
+}
+
+void* pool_take(rhd_context* ctx, uint64_t bytes) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (size_t i = 0; i < ctx->pool.size(); ++i)
+        if (ctx->pool[i].bytes == bytes) {
+            void* p = ctx->pool[i].ptr;
+            ctx->pool[i] = ctx->pool.back();
+            ctx->pool.pop_back();
+            ctx->pooled -= bytes;
+            return p;
+        }
+    return nullptr;
+}
+
+// Returns a block to the pool. Every entry point finishes its own work
+// before returning, so a released block has no work pending on any lane.
+void device_free(rhd_context* ctx, void* p, uint64_t bytes) {
+    if (!p) return;
+    unreserve(ctx, bytes);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->pool.push_back(PooledBlock{p, bytes});
+    ctx->pooled += bytes;
+}
+
+void free_buffer(rhd_context* ctx, DeviceBuffer& b) {
+    device_free(ctx, b.ptr, b.bytes);
     b.ptr = nullptr;
     b.bytes = 0;
 }
 
-void drop_scratch(rhd_context* ctx) {
-    (void)hipStreamSynchronize(ctx->stream);
-    free_buffer(ctx, ctx->amaze);
-    free_buffer(ctx, ctx->amaze_areas);
-    free_buffer(ctx, ctx->rcd);
-    free_buffer(ctx, ctx->capture);
-    free_buffer(ctx, ctx->opposed);
+void drop_scratch(rhd_context* ctx, Lane& L) {
+    if (!L.open) return;
+    (void)hipStreamSynchronize(L.stream);
+    free_buffer(ctx, L.amaze);
+    free_buffer(ctx, L.amaze_areas);
+    free_buffer(ctx, L.rcd);
+    free_buffer(ctx, L.capture);
+    free_buffer(ctx, L.opposed);
 }
 
-// Reserve and allocate, dropping cached scratch once if the budget is short.
-rhd_status device_alloc(rhd_context* ctx, uint64_t bytes, void** out) {
+// Reserve and allocate: a pooled block of the same size if there is one,
+// else a new one. When the budget is short, the lane's cached scratch is
+// dropped once; pooled blocks are freed as needed to keep the device's
+// footprint within the budget.
+rhd_status device_alloc(rhd_context* ctx, Lane* L, uint64_t bytes, void** out) {
     rhd_status st = reserve(ctx, bytes);
-    if (st == RHD_OVER_BUDGET) {
-        drop_scratch(ctx);
+    if (st == RHD_OVER_BUDGET && L) {
+        drop_scratch(ctx, *L);
         st = reserve(ctx, bytes);
     }
     if (st != RHD_OK) return st;
+    if ((*out = pool_take(ctx, bytes)) != nullptr) return RHD_OK;
+    pool_trim(ctx, bytes, false);
     hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim(ctx, 0, true);
+        e = hipMalloc(out, bytes ? bytes : 1);
+    }
     if (e != hipSuccess) {
         unreserve(ctx, bytes);
         (void)hipGetLastError();
@@ -217,17 +224,45 @@ rhd_status device_alloc(rhd_context* ctx, uint64_t bytes, void** out) {
     return RHD_OK;
 }
 
-rhd_status ensure_scratch(rhd_context* ctx, DeviceBuffer& b, uint64_t bytes) {
+rhd_status ensure_scratch(rhd_context* ctx, Lane& L, DeviceBuffer& b, uint64_t bytes) {
     if (b.bytes >= bytes) return RHD_OK;
     if (b.ptr) {
-        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(L.stream);
         free_buffer(ctx, b);
     }
     void* p = nullptr;
-    rhd_status st = device_alloc(ctx, bytes, &p);
+    rhd_status st = device_alloc(ctx, &L, bytes, &p);
     if (st != RHD_OK) return st;
     b.ptr = p;
     b.bytes = bytes;
+    return RHD_OK;
+}
+
+// Creates lane `i` on first use.
+rhd_status lane_open(rhd_context* ctx, int i, Lane** out) {
+    if (i < 0 || i >= MAX_LANES) return fail(ctx, RHD_INVALID_INPUT, "no such lane");
+    Lane& L = ctx->lanes[i];
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!L.open) {
+        L.index = i;
+        bool ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipMalloc(&L.d_stats, sizeof(DeviceStats)) == hipSuccess &&
+                  hipHostMalloc((void**)&L.h_stats, sizeof(DeviceStats), hipHostMallocDefault) ==
+                      hipSuccess;
+        for (auto& ev : L.ev) ok = ok && hipEventCreate(&ev) == hipSuccess;
+        if (!ok) {
+            for (auto& ev : L.ev)
+                if (ev) (void)hipEventDestroy(ev);
+            if (L.d_stats) (void)hipFree(L.d_stats);
+            if (L.h_stats) (void)hipHostFree(L.h_stats);
+            if (L.stream) (void)hipStreamDestroy(L.stream);
+            L = Lane{};
+            ctx->last_error = "could not create a stream, events or statistics for a lane";
+            return RHD_UNSUPPORTED;
+        }
+        L.open = true;
+    }
+    *out = &L;
     return RHD_OK;
 }
 
@@ -355,38 +390,41 @@ int crop_pattern(const rhd_sensor_desc* d) {
     return d->sensor_pattern;
 }
 
-rhd_status new_plane(rhd_context* ctx, uint32_t w, uint32_t h, int32_t pattern, rhd_plane** out) {
+rhd_status new_plane(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, int32_t pattern,
+                     rhd_plane** out) {
     uint64_t bytes = (uint64_t)w * h * sizeof(float);
     void* p = nullptr;
-    rhd_status st = device_alloc(ctx, bytes, &p);
+    rhd_status st = device_alloc(ctx, L, bytes, &p);
     if (st != RHD_OK) return st;
     ctx->refs.fetch_add(1);
     *out = new rhd_plane{ctx, (float*)p, w, h, pattern, bytes};
     return RHD_OK;
 }
 
-rhd_status new_source(rhd_context* ctx, uint32_t w, uint32_t h, rhd_source** out) {
+rhd_status new_source(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, rhd_source** out) {
     uint64_t bytes = (uint64_t)w * h * 3 * sizeof(float);
     void* p = nullptr;
-    rhd_status st = device_alloc(ctx, bytes, &p);
+    rhd_status st = device_alloc(ctx, L, bytes, &p);
     if (st != RHD_OK) return st;
     ctx->refs.fetch_add(1);
     *out = new rhd_source{ctx, (float*)p, w, h, bytes};
     return RHD_OK;
 }
 
-rhd_status reset_stats(rhd_context* ctx) {
-    RHD_HIP(ctx, hipMemsetAsync(ctx->d_stats, 0, sizeof(DeviceStats), ctx->stream));
+// A fill kernel, not hipMemsetAsync, so that the sequence can be captured
+// into a HIP graph.
+rhd_status reset_stats(rhd_context* ctx, Lane& L) {
+    RHD_HIP(ctx, launch_zero(L.d_stats, sizeof(DeviceStats), L.stream));
     return RHD_OK;
 }
 
 // Reads the device counters back and turns the non-finite flags into the
 // reference's errors.
-rhd_status finish_stats(rhd_context* ctx, rhd_metrics* m, int highlight) {
-    RHD_HIP(ctx, hipMemcpyAsync(ctx->h_stats, ctx->d_stats, sizeof(DeviceStats),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const DeviceStats& s = *ctx->h_stats;
+rhd_status finish_stats(rhd_context* ctx, Lane& L, rhd_metrics* m, int highlight) {
+    RHD_HIP(ctx, hipMemcpyAsync(L.h_stats, L.d_stats, sizeof(DeviceStats),
+                                hipMemcpyDeviceToHost, L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
+    const DeviceStats& s = *L.h_stats;
     if (m) {
         m->readback_bytes += sizeof(DeviceStats);
         m->highlight_method = highlight;
@@ -426,9 +464,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // Highlight on a plane already on the stream (normalize may have fused Clip
 // and the detection counts). Opposed always runs here.
-rhd_status run_opposed(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d) {
+rhd_status run_opposed(rhd_context* ctx, Lane& L, rhd_plane* plane, const rhd_sensor_desc* d) {
     size_t need = opposed_scratch_bytes((int)plane->width, (int)plane->height);
-    rhd_status st = ensure_scratch(ctx, ctx->opposed, need);
+    rhd_status st = ensure_scratch(ctx, L, L.opposed, need);
     if (st != RHD_OK) return st;
     OpposedArgs a{};
     a.mosaic = plane->d;
@@ -440,10 +478,10 @@ rhd_status run_opposed(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc
         a.ceiling[c] = d->neutral_ceiling[c];
     }
     a.has_ceiling = d->has_neutral_ceiling;
-    a.scratch = ctx->opposed.ptr;
-    a.scratch_bytes = ctx->opposed.bytes;
-    a.stats = ctx->d_stats;
-    hipError_t e = launch_opposed(a, ctx->stream, PROF(ctx));
+    a.scratch = L.opposed.ptr;
+    a.scratch_bytes = L.opposed.bytes;
+    a.stats = L.d_stats;
+    hipError_t e = launch_opposed(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "opposed");
     return RHD_OK;
 }
@@ -468,31 +506,31 @@ Balance resolve_balance(const rhd_sensor_desc* d) {
     return b;
 }
 
-rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sensor_desc* d,
-                        rhd_source* src) {
+rhd_status run_demosaic(rhd_context* ctx, Lane& L, const rhd_plane* plane,
+                        const rhd_sensor_desc* d, rhd_source* src) {
     const int w = (int)plane->width, h = (int)plane->height, pattern = plane->pattern;
     const Balance b = resolve_balance(d);
     hipError_t e = hipSuccess;
     switch (d->demosaic) {
         case RHD_DEMOSAIC_BILINEAR:
-            e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d, ctx->d_stats,
-                                ctx->stream, PROF(ctx));
+            e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d, L.d_stats,
+                                L.stream, PROF(L));
             break;
         case RHD_DEMOSAIC_MHC:
-            e = launch_mhc(plane->d, w, h, pattern, src->d, ctx->d_stats, ctx->stream, PROF(ctx));
+            e = launch_mhc(plane->d, w, h, pattern, src->d, L.d_stats, L.stream, PROF(L));
             break;
         case RHD_DEMOSAIC_AMAZE: {
             if (w <= 32 || h <= 32) {
                 e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
-                                    ctx->d_stats, ctx->stream, PROF(ctx));
+                                    L.d_stats, L.stream, PROF(L));
                 break;
             }
             const int tx = (w + 223) / 224, ty = (h + 223) / 224, tiles = tx * ty;
             const int per_launch = std::min(tiles, MAX_TILES_PER_LAUNCH);
-            rhd_status st = ensure_scratch(ctx, ctx->amaze,
+            rhd_status st = ensure_scratch(ctx, L, L.amaze,
                                            (uint64_t)per_launch * amaze_scratch_bytes_per_tile());
             if (st != RHD_OK) return st;
-            st = ensure_scratch(ctx, ctx->amaze_areas, (uint64_t)per_launch * 4 * sizeof(int));
+            st = ensure_scratch(ctx, L, L.amaze_areas, (uint64_t)per_launch * 4 * sizeof(int));
             if (st != RHD_OK) return st;
             AmazeArgs a{};
             a.mosaic = plane->d;
@@ -506,31 +544,31 @@ rhd_status run_demosaic(rhd_context* ctx, const rhd_plane* plane, const rhd_sens
                 a.inverse[c] = b.inverse[c];
             }
             a.clip_point = b.clip_point;
-            a.scratch = (float*)ctx->amaze.ptr;
-            a.areas = (int*)ctx->amaze_areas.ptr;
+            a.scratch = (float*)L.amaze.ptr;
+            a.areas = (int*)L.amaze_areas.ptr;
             a.out = src->d;
-            a.stats = ctx->d_stats;
+            a.stats = L.d_stats;
             for (int base = 0; base < tiles && e == hipSuccess; base += per_launch) {
                 a.tile_base = base;
-                e = launch_amaze(a, std::min(per_launch, tiles - base), ctx->stream, PROF(ctx));
+                e = launch_amaze(a, std::min(per_launch, tiles - base), L.stream, PROF(L));
             }
             break;
         }
         case RHD_DEMOSAIC_RCD: {
             if (w <= 24 || h <= 24) {
                 e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d,
-                                    ctx->d_stats, ctx->stream, PROF(ctx));
+                                    L.d_stats, L.stream, PROF(L));
                 break;
             }
             const int tx = (w + 173) / 174, ty = (h + 173) / 174, tiles = tx * ty;
             const int per_launch = std::min(tiles, MAX_TILES_PER_LAUNCH);
             rhd_status st =
-                ensure_scratch(ctx, ctx->rcd, (uint64_t)per_launch * rcd_scratch_bytes_per_tile());
+                ensure_scratch(ctx, L, L.rcd, (uint64_t)per_launch * rcd_scratch_bytes_per_tile());
             if (st != RHD_OK) return st;
             for (int base = 0; base < tiles && e == hipSuccess; base += per_launch)
-                e = launch_rcd(plane->d, w, h, pattern, b.forward, b.inverse, (float*)ctx->rcd.ptr,
-                               base, std::min(per_launch, tiles - base), src->d, ctx->d_stats,
-                               ctx->stream, PROF(ctx));
+                e = launch_rcd(plane->d, w, h, pattern, b.forward, b.inverse, (float*)L.rcd.ptr,
+                               base, std::min(per_launch, tiles - base), src->d, L.d_stats,
+                               L.stream, PROF(L));
             break;
         }
         default:
@@ -555,9 +593,9 @@ int gaussian_weights(float sigma, float* w) {
     return radius;
 }
 
-rhd_status run_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc* dd) {
+rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd) {
     const int w = (int)src->width, h = (int)src->height;
-    rhd_status st = ensure_scratch(ctx, ctx->capture, capture_scratch_bytes(w, h));
+    rhd_status st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
     if (st != RHD_OK) return st;
     CaptureArgs a{};
     a.rgb = src->d;
@@ -568,15 +606,15 @@ rhd_status run_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
     for (int c = 0; c < 3; ++c) a.ceilings[c] = dd->ceilings[c];
     a.radius = gaussian_weights(dd->sigma, a.weights);
     a.iterations = dd->iterations;
-    float* base = (float*)ctx->capture.ptr;
+    float* base = (float*)L.capture.ptr;
     size_t n = (size_t)w * h;
     a.guide = base;
     a.mask = base + n;
     a.est_a = base + 2 * n;
     a.est_b = base + 3 * n;
     a.tmp = base + 4 * n;
-    a.stats = ctx->d_stats;
-    hipError_t e = launch_capture(a, ctx->stream, PROF(ctx));
+    a.stats = L.d_stats;
+    hipError_t e = launch_capture(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "capture");
     return RHD_OK;
 }
@@ -587,7 +625,122 @@ rhd_status check_handle_ctx(rhd_context* ctx, const void* handle_ctx) {
     return RHD_OK;
 }
 
-}  // namespace
+rhd_status mosaic_upload(rhd_context* ctx, Lane& L, const uint16_t* host, uint64_t row_stride,
+                         uint32_t width, uint32_t height, rhd_mosaic** out) {
+    *out = nullptr;
+    if (!host || width == 0 || height == 0 || row_stride < width)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "decoded row stride is shorter than the sensor width");
+    (void)hipSetDevice(ctx->device);
+    uint64_t bytes = row_stride * height * sizeof(uint16_t);
+    void* p = nullptr;
+    rhd_status st = device_alloc(ctx, &L, bytes, &p);
+    if (st != RHD_OK) return st;
+    hipError_t e = hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+    if (e != hipSuccess) {
+        device_free(ctx, p, bytes);
+        return hip_fail(ctx, e, "mosaic upload");
+    }
+    ctx->refs.fetch_add(1);
+    *out = new rhd_mosaic{ctx, (uint16_t*)p, row_stride, width, height, bytes};
+    return RHD_OK;
+}
+
+// The whole stage on one lane: normalize (with Clip or the Opposed detection
+// fused), Opposed, demosaic and capture, one statistics readback at the end
+// (crates/gpu/src/sensor/develop.rs:68-142).
+rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
+                      const rhd_sensor_desc* d, const rhd_detail_desc* dd,
+                      const volatile uint8_t* cancel, rhd_source** out, rhd_metrics* m,
+                      rhd_source** base) {
+    *out = nullptr;
+    if (base) *base = nullptr;
+    rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK) st = validate_desc(ctx, d);
+    if (st == RHD_OK) st = validate_detail(ctx, dd);
+    if (st == RHD_OK && !rhd_supports(d, dd))
+        st = fail(ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
+    if (st != RHD_OK) return st;
+    if (mosaic->width != d->sensor_width || mosaic->height != d->sensor_height ||
+        mosaic->row_stride != d->row_stride)
+        return fail(ctx, RHD_INVALID_INPUT, "decoded frame does not match its normalization contract");
+    (void)hipSetDevice(ctx->device);
+    NormalizeArgs a{};
+    if ((st = level_table(ctx, d, a.table)) != RHD_OK) return st;
+    rhd_plane* plane = nullptr;
+    if ((st = new_plane(ctx, &L, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
+        return st;
+    rhd_source* src = nullptr;
+    if ((st = new_source(ctx, &L, plane->width, plane->height, &src)) != RHD_OK) {
+        rhd_plane_release(plane);
+        return st;
+    }
+    a.mosaic = mosaic->d;
+    a.row_stride = mosaic->row_stride;
+    a.crop_x = (int)d->crop_x;
+    a.crop_y = (int)d->crop_y;
+    a.width = (int)d->crop_width;
+    a.height = (int)d->crop_height;
+    a.crop_pattern = plane->pattern;
+    a.mode = d->highlight == RHD_HIGHLIGHT_CLIP ? 1 : (d->highlight == RHD_HIGHLIGHT_OPPOSED ? 2 : 0);
+    const float* lv = d->highlight == RHD_HIGHLIGHT_CLIP ? d->clip_levels : d->detection_levels;
+    for (int c = 0; c < 3; ++c) a.levels[c] = lv[c];
+    auto step = [&](bool ok, rhd_status err_status, const char* what) {
+        if (!ok && st == RHD_OK) st = fail(ctx, err_status, what);
+    };
+    st = reset_stats(ctx, L);
+    step(hipEventRecord(L.ev[0], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) {
+        hipError_t e = launch_normalize(a, plane->d, L.d_stats, L.stream, PROF(L));
+        if (e != hipSuccess) st = hip_fail(ctx, e, "normalize");
+    }
+    step(hipEventRecord(L.ev[1], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK && d->highlight == RHD_HIGHLIGHT_OPPOSED) st = run_opposed(ctx, L, plane, d);
+    step(hipEventRecord(L.ev[2], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src);
+    step(hipEventRecord(L.ev[3], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK && base) {
+        st = new_source(ctx, &L, src->width, src->height, base);
+        if (st == RHD_OK) {
+            hipError_t e = hipMemcpyAsync((*base)->d, src->d, src->bytes, hipMemcpyDeviceToDevice,
+                                          L.stream);
+            if (e != hipSuccess) st = hip_fail(ctx, e, "base copy");
+        }
+    }
+    if (st == RHD_OK && capture_active(dd)) {
+        if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
+        if (st == RHD_OK) st = run_capture(ctx, L, src, dd);
+    }
+    step(hipEventRecord(L.ev[4], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
+    if (st == RHD_OK) st = finish_stats(ctx, L, m, d->highlight);
+    if (st != RHD_OK) {
+        (void)hipStreamSynchronize(L.stream);
+        rhd_plane_release(plane);
+        rhd_source_release(src);
+        if (base && *base) {
+            rhd_source_release(*base);
+            *base = nullptr;
+        }
+        return st;
+    }
+    if (m) {
+        m->normalize_ms = elapsed(L.ev[0], L.ev[1]);
+        m->highlight_ms = elapsed(L.ev[1], L.ev[2]);
+        m->demosaic_ms = elapsed(L.ev[2], L.ev[3]);
+        m->capture_ms = elapsed(L.ev[3], L.ev[4]);
+        m->total_ms = elapsed(L.ev[0], L.ev[4]);
+        m->launches += 4;
+        m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, ctx->reserved.load());
+    }
+    rhd_plane_release(plane);
+    *out = src;
+    return RHD_OK;
+}
+
+}  // namespace rhd
 
 extern "C" {
 
@@ -603,35 +756,31 @@ rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
     rhd_context* ctx = new rhd_context();
     ctx->device = device;
     if (budget_bytes) ctx->budget = budget_bytes;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->d_stats, sizeof(DeviceStats)) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_stats, sizeof(DeviceStats), hipHostMallocDefault) != hipSuccess) {
-        rhd_destroy(ctx);
+    Lane* L = nullptr;
+    if (lane_open(ctx, 0, &L) != RHD_OK) {
+        context_unref(ctx);
         return RHD_UNSUPPORTED;
     }
-    for (auto& ev : ctx->ev)
-        if (hipEventCreate(&ev) != hipSuccess) {
-            rhd_destroy(ctx);
-            return RHD_UNSUPPORTED;
-        }
     *out = ctx;
     return RHD_OK;
 }
 
 // The context outlives rhd_destroy while handles made from it are alive: each
-// handle holds a reference, and the last release frees the stream and stats.
+// handle holds a reference, and the last release frees the lanes and the pool.
 void rhd_destroy(rhd_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    drop_scratch(ctx);
-    ctx_unref(ctx);
+    for (Lane& L : ctx->lanes) drop_scratch(ctx, L);
+    context_unref(ctx);
 }
 
+// Not to be called while other calls on the context are in flight.
 rhd_status rhd_set_budget(rhd_context* ctx, uint64_t budget_bytes) {
     if (!ctx) return RHD_INVALID_INPUT;
     ctx->budget = budget_bytes ? budget_bytes : DEFAULT_BUDGET;
-    if (ctx->reserved.load() > ctx->budget.load()) drop_scratch(ctx);
+    if (ctx->reserved.load() > ctx->budget.load())
+        for (Lane& L : ctx->lanes) drop_scratch(ctx, L);
+    pool_trim(ctx, 0, false);
     return RHD_OK;
 }
 
@@ -674,39 +823,22 @@ int rhd_supports(const rhd_sensor_desc* d, const rhd_detail_desc* dd) {
 rhd_status rhd_mosaic_upload(rhd_context* ctx, const uint16_t* host, uint64_t row_stride,
                              uint32_t width, uint32_t height, rhd_mosaic** out) {
     if (!ctx || !out) return RHD_INVALID_INPUT;
-    *out = nullptr;
-    if (!host || width == 0 || height == 0 || row_stride < width)
-        return fail(ctx, RHD_INVALID_DIMENSIONS, "decoded row stride is shorter than the sensor width");
-    (void)hipSetDevice(ctx->device);
-    uint64_t bytes = row_stride * height * sizeof(uint16_t);
-    void* p = nullptr;
-    rhd_status st = device_alloc(ctx, bytes, &p);
-    if (st != RHD_OK) return st;
-    hipError_t e = hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        (void)hipFree(p);
-        unreserve(ctx, bytes);
-        return hip_fail(ctx, e, "mosaic upload");
-    }
-    ctx->refs.fetch_add(1);
-    *out = new rhd_mosaic{ctx, (uint16_t*)p, row_stride, width, height, bytes};
-    return RHD_OK;
+    return mosaic_upload(ctx, ctx->lanes[0], host, row_stride, width, height, out);
 }
 
 void rhd_mosaic_release(rhd_mosaic* m) {
     if (!m) return;
-    (void)hipStreamSynchronize(m->ctx->stream);
-    (void)hipFree(m->d);
-    unreserve(m->ctx, m->bytes);
-    ctx_unref(m->ctx);
+    rhd_context* ctx = m->ctx;
+    device_free(ctx, m->d, m->bytes);
     delete m;
+    context_unref(ctx);
 }
 
 rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_sensor_desc* d,
                          const volatile uint8_t* cancel, rhd_plane** out, rhd_metrics* m) {
     if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
     *out = nullptr;
+    Lane& L = ctx->lanes[0];
     rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
     if (st != RHD_OK) return st;
     if ((st = validate_desc(ctx, d)) != RHD_OK) return st;
@@ -718,7 +850,7 @@ rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_s
     NormalizeArgs a{};
     if ((st = level_table(ctx, d, a.table)) != RHD_OK) return st;
     rhd_plane* plane = nullptr;
-    if ((st = new_plane(ctx, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
+    if ((st = new_plane(ctx, &L, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
         return st;
     a.mosaic = mosaic->d;
     a.row_stride = mosaic->row_stride;
@@ -728,20 +860,20 @@ rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_s
     a.height = (int)d->crop_height;
     a.crop_pattern = plane->pattern;
     a.mode = 0;
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream, PROF(ctx));
+    RHD_HIP(ctx, hipEventRecord(L.ev[0], L.stream));
+    hipError_t e = launch_normalize(a, plane->d, L.d_stats, L.stream, PROF(L));
     if (e != hipSuccess) {
         rhd_plane_release(plane);
         return hip_fail(ctx, e, "normalize");
     }
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    e = hipEventSynchronize(ctx->ev[1]);
+    RHD_HIP(ctx, hipEventRecord(L.ev[1], L.stream));
+    e = hipEventSynchronize(L.ev[1]);
     if (e != hipSuccess) {
         rhd_plane_release(plane);
         return hip_fail(ctx, e, "normalize");
     }
     if (m) {
-        m->normalize_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+        m->normalize_ms = elapsed(L.ev[0], L.ev[1]);
         m->launches += 1;
         m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, plane->bytes);
     }
@@ -752,34 +884,35 @@ rhd_status rhd_normalize(rhd_context* ctx, const rhd_mosaic* mosaic, const rhd_s
 rhd_status rhd_highlight(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_desc* d,
                          const volatile uint8_t* cancel, rhd_metrics* m) {
     if (!ctx || !plane) return RHD_INVALID_INPUT;
+    Lane& L = ctx->lanes[0];
     rhd_status st = check_handle_ctx(ctx, plane->ctx);
     if (st != RHD_OK) return st;
     if ((st = validate_desc(ctx, d)) != RHD_OK) return st;
     if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
     (void)hipSetDevice(ctx->device);
-    if ((st = reset_stats(ctx)) != RHD_OK) return st;
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if ((st = reset_stats(ctx, L)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(L.ev[0], L.stream));
     hipError_t e = hipSuccess;
     const int w = (int)plane->width, h = (int)plane->height;
     if (d->highlight == RHD_HIGHLIGHT_CLIP) {
-        e = launch_clip(plane->d, w, h, plane->pattern, d->clip_levels, ctx->d_stats, ctx->stream);
+        e = launch_clip(plane->d, w, h, plane->pattern, d->clip_levels, L.d_stats, L.stream);
     } else {
         // Off and Opposed validate every sample (opposed.rs:182-215).
         float levels[3] = {INFINITY, INFINITY, INFINITY};
         e = launch_detect(plane->d, w, h, plane->pattern,
                           d->highlight == RHD_HIGHLIGHT_OPPOSED ? d->detection_levels : levels,
-                          ctx->d_stats, ctx->stream);
+                          L.d_stats, L.stream);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "highlight");
     if (d->highlight == RHD_HIGHLIGHT_OPPOSED) {
         // Samples must be finite before reconstruction reads them.
-        if ((st = finish_stats(ctx, nullptr, d->highlight)) != RHD_OK) return st;
-        if ((st = run_opposed(ctx, plane, d)) != RHD_OK) return st;
+        if ((st = finish_stats(ctx, L, nullptr, d->highlight)) != RHD_OK) return st;
+        if ((st = run_opposed(ctx, L, plane, d)) != RHD_OK) return st;
     }
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if ((st = finish_stats(ctx, m, d->highlight)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(L.ev[1], L.stream));
+    if ((st = finish_stats(ctx, L, m, d->highlight)) != RHD_OK) return st;
     if (m) {
-        m->highlight_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+        m->highlight_ms = elapsed(L.ev[0], L.ev[1]);
         m->launches += 1;
         m->readback_bytes += sizeof(DeviceStats);
     }
@@ -794,6 +927,7 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
         return RHD_INVALID_INPUT;
     }
     *out = nullptr;
+    Lane& L = ctx->lanes[0];
     rhd_status st = check_handle_ctx(ctx, plane->ctx);
     if (st == RHD_OK) st = validate_desc(ctx, d);
     if (st == RHD_OK) st = validate_detail(ctx, dd);
@@ -806,28 +940,29 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
     }
     (void)hipSetDevice(ctx->device);
     rhd_source* src = nullptr;
-    st = new_source(ctx, plane->width, plane->height, &src);
-    if (st == RHD_OK) st = reset_stats(ctx);
-    if (st == RHD_OK && hipEventRecord(ctx->ev[2], ctx->stream) != hipSuccess)
+    st = new_source(ctx, &L, plane->width, plane->height, &src);
+    if (st == RHD_OK) st = reset_stats(ctx, L);
+    if (st == RHD_OK && hipEventRecord(L.ev[2], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK) st = run_demosaic(ctx, plane, d, src);
-    if (st == RHD_OK && hipEventRecord(ctx->ev[3], ctx->stream) != hipSuccess)
+    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src);
+    if (st == RHD_OK && hipEventRecord(L.ev[3], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK && capture_active(dd)) {
         if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_capture(ctx, src, dd);
+        if (st == RHD_OK) st = run_capture(ctx, L, src, dd);
     }
-    if (st == RHD_OK && hipEventRecord(ctx->ev[4], ctx->stream) != hipSuccess)
+    if (st == RHD_OK && hipEventRecord(L.ev[4], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK) st = finish_stats(ctx, nullptr, RHD_HIGHLIGHT_OFF);
+    if (st == RHD_OK) st = finish_stats(ctx, L, nullptr, RHD_HIGHLIGHT_OFF);
+    if (st != RHD_OK) (void)hipStreamSynchronize(L.stream);
     rhd_plane_release(plane);
     if (st != RHD_OK) {
         rhd_source_release(src);
         return st;
     }
     if (m) {
-        m->demosaic_ms = elapsed(ctx->ev[2], ctx->ev[3]);
-        m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+        m->demosaic_ms = elapsed(L.ev[2], L.ev[3]);
+        m->capture_ms = elapsed(L.ev[3], L.ev[4]);
         m->launches += 2;
         m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, ctx->reserved.load());
     }
@@ -838,6 +973,7 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc* dd,
                        const volatile uint8_t* cancel, rhd_metrics* m) {
     if (!ctx || !src) return RHD_INVALID_INPUT;
+    Lane& L = ctx->lanes[0];
     rhd_status st = check_handle_ctx(ctx, src->ctx);
     if (st != RHD_OK) return st;
     if ((st = validate_detail(ctx, dd)) != RHD_OK) return st;
@@ -846,12 +982,15 @@ rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
         return fail(ctx, RHD_UNSUPPORTED_EDITS, "capture radius above 4 taps");
     if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
     (void)hipSetDevice(ctx->device);
-    if ((st = reset_stats(ctx)) != RHD_OK) return st;
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-    if ((st = run_capture(ctx, src, dd)) != RHD_OK) return st;
-    RHD_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if ((st = finish_stats(ctx, nullptr, RHD_HIGHLIGHT_OFF)) != RHD_OK) return st;
-    if (m) m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+    if ((st = reset_stats(ctx, L)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipEventRecord(L.ev[3], L.stream));
+    if ((st = run_capture(ctx, L, src, dd)) != RHD_OK) {
+        (void)hipStreamSynchronize(L.stream);
+        return st;
+    }
+    RHD_HIP(ctx, hipEventRecord(L.ev[4], L.stream));
+    if ((st = finish_stats(ctx, L, nullptr, RHD_HIGHLIGHT_OFF)) != RHD_OK) return st;
+    if (m) m->capture_ms = elapsed(L.ev[3], L.ev[4]);
     return RHD_OK;
 }
 
@@ -860,78 +999,7 @@ rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,
                                 const volatile uint8_t* cancel, rhd_source** out,
                                 rhd_metrics* m) {
     if (!ctx || !out || !mosaic) return RHD_INVALID_INPUT;
-    *out = nullptr;
-    rhd_status st = check_handle_ctx(ctx, mosaic->ctx);
-    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-    if (st == RHD_OK) st = validate_desc(ctx, d);
-    if (st == RHD_OK) st = validate_detail(ctx, dd);
-    if (st == RHD_OK && !rhd_supports(d, dd))
-        st = fail(ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
-    if (st != RHD_OK) return st;
-    if (mosaic->width != d->sensor_width || mosaic->height != d->sensor_height ||
-        mosaic->row_stride != d->row_stride)
-        return fail(ctx, RHD_INVALID_INPUT, "decoded frame does not match its normalization contract");
-    (void)hipSetDevice(ctx->device);
-    NormalizeArgs a{};
-    if ((st = level_table(ctx, d, a.table)) != RHD_OK) return st;
-    rhd_plane* plane = nullptr;
-    if ((st = new_plane(ctx, d->crop_width, d->crop_height, crop_pattern(d), &plane)) != RHD_OK)
-        return st;
-    rhd_source* src = nullptr;
-    if ((st = new_source(ctx, plane->width, plane->height, &src)) != RHD_OK) {
-        rhd_plane_release(plane);
-        return st;
-    }
-    a.mosaic = mosaic->d;
-    a.row_stride = mosaic->row_stride;
-    a.crop_x = (int)d->crop_x;
-    a.crop_y = (int)d->crop_y;
-    a.width = (int)d->crop_width;
-    a.height = (int)d->crop_height;
-    a.crop_pattern = plane->pattern;
-    a.mode = d->highlight == RHD_HIGHLIGHT_CLIP ? 1 : (d->highlight == RHD_HIGHLIGHT_OPPOSED ? 2 : 0);
-    const float* lv = d->highlight == RHD_HIGHLIGHT_CLIP ? d->clip_levels : d->detection_levels;
-    for (int c = 0; c < 3; ++c) a.levels[c] = lv[c];
-    auto step = [&](bool ok, rhd_status err_status, const char* what) {
-        if (!ok && st == RHD_OK) st = fail(ctx, err_status, what);
-    };
-    st = reset_stats(ctx);
-    step(hipEventRecord(ctx->ev[0], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK) {
-        hipError_t e = launch_normalize(a, plane->d, ctx->d_stats, ctx->stream, PROF(ctx));
-        if (e != hipSuccess) st = hip_fail(ctx, e, "normalize");
-    }
-    step(hipEventRecord(ctx->ev[1], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-    if (st == RHD_OK && d->highlight == RHD_HIGHLIGHT_OPPOSED) st = run_opposed(ctx, plane, d);
-    step(hipEventRecord(ctx->ev[2], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-    if (st == RHD_OK) st = run_demosaic(ctx, plane, d, src);
-    step(hipEventRecord(ctx->ev[3], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK && capture_active(dd)) {
-        if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_capture(ctx, src, dd);
-    }
-    step(hipEventRecord(ctx->ev[4], ctx->stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK) st = finish_stats(ctx, m, d->highlight);
-    if (st != RHD_OK) {
-        (void)hipStreamSynchronize(ctx->stream);
-        rhd_plane_release(plane);
-        rhd_source_release(src);
-        return st;
-    }
-    if (m) {
-        m->normalize_ms = elapsed(ctx->ev[0], ctx->ev[1]);
-        m->highlight_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-        m->demosaic_ms = elapsed(ctx->ev[2], ctx->ev[3]);
-        m->capture_ms = elapsed(ctx->ev[3], ctx->ev[4]);
-        m->total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
-        m->launches += 4;
-        m->estimated_gpu_bytes = std::max<uint64_t>(m->estimated_gpu_bytes, ctx->reserved.load());
-    }
-    rhd_plane_release(plane);
-    *out = src;
-    return RHD_OK;
+    return develop_on(ctx, ctx->lanes[0], mosaic, d, dd, cancel, out, m, nullptr);
 }
 
 rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t row_stride,
@@ -951,21 +1019,23 @@ rhd_status rhd_develop(rhd_context* ctx, const uint16_t* host_mosaic, uint64_t r
 
 rhd_status rhd_plane_read(rhd_context* ctx, const rhd_plane* plane, float* dst, uint64_t count) {
     if (!ctx || !plane || !dst) return RHD_INVALID_INPUT;
+    Lane& L = ctx->lanes[0];
     if (count < (uint64_t)plane->width * plane->height)
         return fail(ctx, RHD_READBACK, "destination too small");
     RHD_HIP(ctx, hipMemcpyAsync(dst, plane->d, (size_t)plane->width * plane->height * sizeof(float),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                hipMemcpyDeviceToHost, L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
     return RHD_OK;
 }
 
 rhd_status rhd_plane_write(rhd_context* ctx, rhd_plane* plane, const float* src, uint64_t count) {
     if (!ctx || !plane || !src) return RHD_INVALID_INPUT;
+    Lane& L = ctx->lanes[0];
     if (count < (uint64_t)plane->width * plane->height)
         return fail(ctx, RHD_INVALID_INPUT, "source too small");
     RHD_HIP(ctx, hipMemcpyAsync(plane->d, src, (size_t)plane->width * plane->height * sizeof(float),
-                                hipMemcpyHostToDevice, ctx->stream));
-    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                hipMemcpyHostToDevice, L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
     return RHD_OK;
 }
 
@@ -978,7 +1048,7 @@ rhd_status rhd_plane_upload(rhd_context* ctx, const float* host, uint32_t width,
     if (pattern < 0 || pattern > 3) return fail(ctx, RHD_INVALID_INPUT, "unsupported CFA pattern");
     (void)hipSetDevice(ctx->device);
     rhd_plane* plane = nullptr;
-    rhd_status st = new_plane(ctx, width, height, pattern, &plane);
+    rhd_status st = new_plane(ctx, &ctx->lanes[0], width, height, pattern, &plane);
     if (st != RHD_OK) return st;
     st = rhd_plane_write(ctx, plane, host, (uint64_t)width * height);
     if (st != RHD_OK) {
@@ -998,21 +1068,21 @@ void rhd_plane_dims(const rhd_plane* p, uint32_t* w, uint32_t* h, int32_t* patte
 
 void rhd_plane_release(rhd_plane* p) {
     if (!p) return;
-    (void)hipStreamSynchronize(p->ctx->stream);
-    (void)hipFree(p->d);
-    unreserve(p->ctx, p->bytes);
-    ctx_unref(p->ctx);
+    rhd_context* ctx = p->ctx;
+    device_free(ctx, p->d, p->bytes);
     delete p;
+    context_unref(ctx);
 }
 
 rhd_status rhd_source_read_rgb(rhd_context* ctx, const rhd_source* s, float* dst, uint64_t count) {
     if (!ctx || !s || !dst) return RHD_INVALID_INPUT;
+    Lane& L = ctx->lanes[0];
     const size_t n = (size_t)s->width * s->height;
     if (count < 3 * (uint64_t)n) return fail(ctx, RHD_READBACK, "destination too small");
     std::vector<float> planar(3 * n);
     RHD_HIP(ctx, hipMemcpyAsync(planar.data(), s->d, 3 * n * sizeof(float), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
     for (size_t i = 0; i < n; ++i) {
         dst[3 * i] = planar[i];
         dst[3 * i + 1] = planar[n + i];
@@ -1038,43 +1108,44 @@ void rhd_source_dims(const rhd_source* s, uint32_t* w, uint32_t* h) {
 
 void rhd_source_release(rhd_source* s) {
     if (!s) return;
-    (void)hipStreamSynchronize(s->ctx->stream);
-    (void)hipFree(s->d);
-    unreserve(s->ctx, s->bytes);
-    ctx_unref(s->ctx);
+    rhd_context* ctx = s->ctx;
+    device_free(ctx, s->d, s->bytes);
     delete s;
+    context_unref(ctx);
 }
 
 rhd_status rhd_synchronize(rhd_context* ctx) {
     if (!ctx) return RHD_INVALID_INPUT;
-    RHD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (Lane& L : ctx->lanes)
+        if (L.open) RHD_HIP(ctx, hipStreamSynchronize(L.stream));
     return RHD_OK;
 }
 
 rhd_status rhd_profile_enable(rhd_context* ctx, int enable) {
     if (!ctx) return RHD_INVALID_INPUT;
-    ctx->profiling = enable != 0;
+    ctx->lanes[0].profiling = enable != 0;
     return RHD_OK;
 }
 
 rhd_status rhd_profile_reset(rhd_context* ctx) {
     if (!ctx) return RHD_INVALID_INPUT;
-    ctx->profiler.reset();
+    ctx->lanes[0].profiler.reset();
     return RHD_OK;
 }
 
 uint32_t rhd_profile_count(rhd_context* ctx) {
     if (!ctx) return 0;
-    ctx->profiler.collect();
-    return (uint32_t)ctx->profiler.totals.size();
+    ctx->lanes[0].profiler.collect();
+    return (uint32_t)ctx->lanes[0].profiler.totals.size();
 }
 
 rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,
                              uint64_t* launches, uint64_t* bytes_per_launch) {
     if (!ctx) return RHD_INVALID_INPUT;
-    ctx->profiler.collect();
-    if (index >= ctx->profiler.totals.size()) return fail(ctx, RHD_INVALID_INPUT, "no such entry");
-    const Profiler::Total& t = ctx->profiler.totals[index];
+    Profiler& prof = ctx->lanes[0].profiler;
+    prof.collect();
+    if (index >= prof.totals.size()) return fail(ctx, RHD_INVALID_INPUT, "no such entry");
+    const Profiler::Total& t = prof.totals[index];
     if (name) *name = t.name.c_str();
     if (total_ms) *total_ms = t.ms;
     if (launches) *launches = t.launches;

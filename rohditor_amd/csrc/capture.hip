@@ -421,7 +421,9 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
     RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.mask, a.mask, a.tmp, 0));
     RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.guide, a.tmp, a.mask, 1));
-    hipMemcpyAsync(a.est_a, a.guide, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    hipError_t e = hipMemcpyAsync(a.est_a, a.guide, (size_t)n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
     float* cur = a.est_a;
     float* next = a.est_b;
     for (int it = 0; it < a.iterations; ++it) {

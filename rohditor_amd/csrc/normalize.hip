@@ -146,6 +146,20 @@ __global__ __launch_bounds__(BLOCK) void classify_kernel(float* __restrict__ pla
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(BLOCK) void zero_kernel(unsigned* __restrict__ p, size_t words) {
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < words; i += (size_t)gridDim.x * BLOCK)
+        p[i] = 0u;
+}
+}  // namespace
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t words = bytes / 4;
+    const unsigned blocks = (unsigned)std::min<size_t>((words + BLOCK - 1) / BLOCK, 1024);
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks ? blocks : 1), dim3(BLOCK), 0, s, (unsigned*)p, words);
+    return hipGetLastError();
+}
+
 hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
                             hipStream_t s, Profiler* prof) {
     dim3 grid((a.width + BLOCK * PER_THREAD - 1) / (BLOCK * PER_THREAD), std::min(a.height, ROW_BLOCKS));

@@ -1,0 +1,112 @@
+// Host-side state shared by the librhd translation units: the context with
+// its lanes and device pool, the resident handles, and the internal stage
+// sequence. Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rhd_internal.hpp"
+
+namespace rhd {
+
+constexpr uint64_t DEFAULT_BUDGET = 64ull << 30;  // 64 GiB of the 288 GiB HBM3E
+constexpr int MAX_TILES_PER_LAUNCH = 2048;
+constexpr int MAX_LANES = 8;
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// One independent stream of work inside a context: its own HIP stream,
+// device statistics (with a pinned readback slot), stage events, scratch and
+// profiler. Lane 0 serves the staged API; the others let several develops
+// overlap on the device (the serving loop and the batched path).
+struct Lane {
+    int index = 0;
+    bool open = false;
+    hipStream_t stream = nullptr;
+    DeviceStats* d_stats = nullptr;
+    DeviceStats* h_stats = nullptr;
+    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed;
+    hipEvent_t ev[5] = {};
+    bool profiling = false;
+    Profiler profiler;
+};
+
+struct PooledBlock {
+    void* ptr;
+    uint64_t bytes;
+};
+
+
+}  // namespace rhd
+
+struct rhd_context {
+    int device = 0;
+    std::atomic<uint64_t> budget{rhd::DEFAULT_BUDGET};
+    std::atomic<uint64_t> reserved{0};  // bytes held by live handles and scratch
+    std::mutex mu;                      // last_error, the pool, opening lanes
+    std::string last_error;
+    // Released device blocks kept for reuse, so that steady-state develops
+    // allocate nothing (hipFree would also wait for the whole device).
+    std::vector<rhd::PooledBlock> pool;
+    uint64_t pooled = 0;
+    rhd::Lane lanes[rhd::MAX_LANES];
+    std::atomic<int> refs{1};  // rhd_create's reference plus one per live handle
+};
+
+struct rhd_mosaic {
+    rhd_context* ctx;
+    uint16_t* d;
+    uint64_t row_stride;
+    uint32_t width, height;
+    uint64_t bytes;
+};
+
+struct rhd_plane {
+    rhd_context* ctx;
+    float* d;
+    uint32_t width, height;
+    int32_t pattern;
+    uint64_t bytes;
+};
+
+struct rhd_source {
+    rhd_context* ctx;
+    float* d;
+    uint32_t width, height;
+    uint64_t bytes;
+};
+namespace rhd {
+
+rhd_status fail(rhd_context* ctx, rhd_status status, const std::string& msg);
+// Drops one reference; the last one frees the lanes and the pool.
+void context_unref(rhd_context* ctx);
+// Waits for the lane, then returns its scratch to the pool.
+void drop_scratch(rhd_context* ctx, Lane& L);
+bool cancelled(const volatile uint8_t* flag);
+rhd_status lane_open(rhd_context* ctx, int i, Lane** out);
+rhd_status device_alloc(rhd_context* ctx, Lane* L, uint64_t bytes, void** out);
+void device_free(rhd_context* ctx, void* p, uint64_t bytes);
+bool capture_active(const rhd_detail_desc* dd);
+rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd);
+rhd_status new_source(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, rhd_source** out);
+rhd_status reset_stats(rhd_context* ctx, Lane& L);
+rhd_status finish_stats(rhd_context* ctx, Lane& L, rhd_metrics* m, int highlight);
+rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
+rhd_status mosaic_upload(rhd_context* ctx, Lane& L, const uint16_t* host, uint64_t row_stride,
+                         uint32_t width, uint32_t height, rhd_mosaic** out);
+// The whole stage on one lane; see api.cpp. With `base`, the demosaiced
+// camera RGB before capture sharpening is also copied into it.
+rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
+                      const rhd_sensor_desc* d, const rhd_detail_desc* dd,
+                      const volatile uint8_t* cancel, rhd_source** out, rhd_metrics* m,
+                      rhd_source** base);
+
+}  // namespace rhd

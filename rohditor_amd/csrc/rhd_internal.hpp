@@ -89,6 +89,9 @@ struct AmazeArgs {
 // Launchers (each returns the first HIP error, or hipSuccess).
 hipError_t launch_normalize(const NormalizeArgs& a, float* out, DeviceStats* stats,
                             hipStream_t s, Profiler* prof);
+// Zeroes `bytes` (a multiple of 4) of device memory with a kernel, which a
+// HIP graph may capture.
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_clip(float* plane, int width, int height, int pattern, const float limits[3],
                        DeviceStats* stats, hipStream_t s);
 hipError_t launch_detect(const float* plane, int width, int height, int pattern,

@@ -467,3 +467,91 @@ class GpuSensorProcessor:
             ctypes.byref(detail), cancellation.ptr if cancellation else None, ctypes.byref(ptr),
             ctypes.byref(m)))
         return GpuDevelopedSource(self, ptr), m
+
+
+@dataclass
+class ServedResult:
+    """The outcome of one served request (include/rhd.h rhd_request_info)."""
+    status: int
+    source: Optional[GpuDevelopedSource]
+    lane: int
+    base_cache_hit: bool
+    queued_ms: float
+    run_ms: float
+    latency_ms: float
+    megapixels: float
+
+    @property
+    def ok(self) -> bool:
+        return self.status == _lib.OK
+
+
+class GpuServer:
+    """Serving loop on one GPU: the counterpart of the engine worker's develop
+    loop with its newest-wins mailbox (crates/engine/src/coordinator/
+    worker.rs:723-800, scheduler.rs:44-173) and the resident caches of
+    SpatialCache (crates/engine/src/preview_cache/spatial.rs:322-611).
+
+    Frames are made resident with `put_frame`; `submit` queues a develop and
+    returns a request id; `wait` collects it. Requests for different frames
+    run concurrently on `lanes` HIP streams; a newer request for a frame
+    coalesces its queued ones and cancels its running one.
+    """
+
+    def __init__(self, proc: GpuSensorProcessor, lanes: int = 3, admission_bytes: int = 0,
+                 base_cache_bytes: int = 0):
+        self.proc = proc
+        cfg = _lib.ServerConfig(lanes, admission_bytes, base_cache_bytes)
+        ptr = ctypes.c_void_p()
+        proc._check(proc.lib.rhd_server_create(proc.ctx, ctypes.byref(cfg), ctypes.byref(ptr)))
+        self.ptr = ptr
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.proc.lib.rhd_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def put_frame(self, frame_id: int, frame: RawFrame) -> None:
+        h = frame.info.height
+        buf = np.ascontiguousarray(frame.mosaic[:h, :frame.row_stride])
+        self.proc._check(self.proc.lib.rhd_server_put_mosaic(
+            self.ptr, frame_id, buf.ctypes.data_as(ctypes.c_void_p), frame.row_stride,
+            frame.info.width, h))
+
+    def drop_frame(self, frame_id: int) -> None:
+        self.proc._check(self.proc.lib.rhd_server_drop_mosaic(self.ptr, frame_id))
+
+    def submit(self, frame_id: int, desc: SensorDevelopmentDescription, detail=None) -> int:
+        c = desc.to_c()
+        rid = ctypes.c_uint64()
+        self.proc._check(self.proc.lib.rhd_server_submit(
+            self.ptr, frame_id, ctypes.byref(c), ctypes.byref(detail) if detail else None,
+            ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, request_id: int, timeout_ms: int = -1, keep: bool = True) -> ServedResult:
+        """Collects a request; with `keep` the developed source is returned
+        (release it when done), otherwise it is released here."""
+        ptr = ctypes.c_void_p()
+        info = _lib.RequestInfo()
+        st = self.proc.lib.rhd_server_wait(self.ptr, request_id, timeout_ms,
+                                           ctypes.byref(ptr) if keep else None, ctypes.byref(info))
+        if st == _lib.SYNCHRONIZATION and info.latency_ms == 0.0:
+            raise TimeoutError(f"request {request_id} is still pending")
+        src = GpuDevelopedSource(self.proc, ptr) if keep and ptr.value else None
+        return ServedResult(st, src, info.lane, bool(info.base_cache_hit), info.queued_ms,
+                            info.run_ms, info.latency_ms, info.megapixels)
+
+    def stats(self) -> "_lib.ServerStats":
+        out = _lib.ServerStats()
+        self.proc._check(self.proc.lib.rhd_server_read_stats(self.ptr, ctypes.byref(out)))
+        return out
+
+    def reset_stats(self) -> None:
+        self.proc._check(self.proc.lib.rhd_server_reset_stats(self.ptr))
