@@ -37,6 +37,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-served", action="store_true")
+    p.add_argument("--batch", action="store_true",
+                   help="run the batched workload instead: 32 mosaics of 12-61 MP with mixed CFA "
+                        "phases, demosaic algorithms and RL iterations in one graph launch")
+    p.add_argument("--batch-frames", type=int, default=32)
+    p.add_argument("--batch-lanes", type=int, default=0, help="0: autotune")
     p.add_argument("--served-lanes", type=int, default=3)
     p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--calibrate-copy", action="store_true",
@@ -186,6 +191,9 @@ def main():
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
         return
+    if args.batch:
+        batch_bench(args)
+        return
     from rohditor_amd import _lib, sensor
 
     spec, frame, recipe, options = workload()
@@ -306,6 +314,86 @@ def served(frame, desc, detail, plain_value, args):
             "requests": total, "lanes": lanes, "in_flight": in_flight,
             "workload": "closed loop over resident copies of the same frame and recipe, "
                         "no base cache"}
+
+
+BATCH_SIZES = [(4240, 2832), (6048, 4024), (7008, 4672), (8256, 5504), (9504, 6336)]
+
+
+def batch_bench(args):
+    """The batched workload: `--batch-frames` mosaics with sizes drawn from
+    12, 24, 33, 45 and 61 MP, CFA phase from RGGB/BGGR/GRBG, demosaic from
+    AMaZE/RCD/MHC and RL iterations from 0..20 (seeded), Opposed highlight
+    recovery, developed by one graph launch; against the same frames
+    developed one after another on the plain path."""
+    from rohditor_amd import sensor, synth
+    rng = np.random.default_rng(0x52480300)
+    n = args.batch_frames
+    sizes = rng.integers(0, len(BATCH_SIZES), n)
+    cfas = rng.integers(0, 3, n)
+    algos = rng.integers(0, 3, n)
+    iters = rng.integers(0, 21, n)
+    # One synthetic scene per size (seed 0x52480300 + size index); frames of
+    # a size share its samples and differ in CFA phase and recipe.
+    scenes = {}
+    for k in sorted(set(sizes.tolist())):
+        w, h = BATCH_SIZES[k]
+        scenes[k] = synth.mosaic(synth.SceneSpec(width=w, height=h, seed=0x52480300 + k,
+                                                 blob_fraction=0.05))
+    proc = sensor.GpuSensorProcessor(0, budget_bytes=200 << 30)
+    residents, descs, details, mp = [], [], [], 0.0
+    algo_of = [sensor.DemosaicAlgorithm.AMAZE, sensor.DemosaicAlgorithm.RCD,
+               sensor.DemosaicAlgorithm.MHC]
+    for i in range(n):
+        w, h = BATCH_SIZES[sizes[i]]
+        cfa = ["RGGB", "BGGR", "GRBG"][cfas[i]]
+        frame = synth.frame_from_mosaic(scenes[sizes[i]], cfa)
+        recipe = sensor.EditRecipe(
+            highlights=sensor.HighlightAdjustments(method=sensor.HighlightMethod.OPPOSED),
+            capture=sensor.CaptureSharpening(enabled=True, amount=0.5, radius=0.7,
+                                             noise_protection=0.5),
+            capture_iterations=int(iters[i]))
+        desc = sensor.SensorDevelopmentDescription.from_frame(
+            frame, recipe, sensor.RenderOptions(demosaic=algo_of[algos[i]]))
+        residents.append(proc.upload(frame))
+        descs.append(desc)
+        details.append(sensor.detail_desc(recipe, desc.demosaic_balance))
+        mp += w * h / 1e6
+    # Plain path: one develop after another.
+    for i in range(min(2, n)):  # warm-up
+        proc.develop_resident(residents[i], descs[i], details[i]).release()
+    t0 = time.perf_counter()
+    for i in range(n):
+        proc.develop_resident(residents[i], descs[i], details[i]).release()
+    plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = sensor.GpuBatch(proc, residents, descs, details, lanes=args.batch_lanes)
+    build_s = time.perf_counter() - t0
+    batch.run()  # warm-up
+    runs = []
+    for _ in range(max(args.steps // 10, 3)):
+        t0 = time.perf_counter()
+        statuses, gpu_ms = batch.run()
+        runs.append(time.perf_counter() - t0)
+        if any(s != 0 for s in statuses):
+            raise RuntimeError(f"batched develop failed: {statuses}")
+    wall = float(np.median(runs))
+    line = {
+        "metric": "batched sensor MP/s (Opposed + AMaZE/RCD/MHC + RL 0-20), one graph launch",
+        "value": round(mp / wall, 2), "unit": "MP/s", "n_gpus": 1, "higher_is_better": True,
+        "frames": n, "megapixels": round(mp, 2), "ms_per_batch": round(wall * 1e3, 3),
+        "plain_serial_value": round(mp / plain_s, 2), "vs_plain": round(plain_s / wall, 3),
+        "lanes": batch.lanes, "autotuned": args.batch_lanes == 0,
+        "build_and_tune_s": round(build_s, 3), "runs": len(runs), "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "32 seeded mosaics, sizes from {4240x2832, 6048x4024, 7008x4672, "
+                               "8256x5504, 9504x6336}, CFA RGGB/BGGR/GRBG, demosaic "
+                               "AMaZE/RCD/MHC, Opposed, RL sigma 0.7 with 0-20 iterations"},
+    }
+    batch.close()
+    for r in residents:
+        r.release()
+    proc.close()
+    print(json.dumps(line))
 
 
 def profile_table(proc):

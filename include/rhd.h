@@ -286,6 +286,33 @@ rhd_status rhd_server_wait(rhd_server* server, uint64_t request_id, int32_t time
 rhd_status rhd_server_read_stats(rhd_server* server, rhd_server_stats* out);
 rhd_status rhd_server_reset_stats(rhd_server* server);
 
+/* ---------------------------------------------------------------------------
+ * Batched develop: many resident mosaics of uneven sizes and recipes in one
+ * HIP graph launch. The reference develops one frame per call
+ * (crates/gpu/src/sensor/develop.rs:68-142) for one request at a time
+ * (crates/engine/src/coordinator/worker.rs:723-800); a batch captures the
+ * stage of all its frames once, as `lanes` independent branches of one graph,
+ * and rhd_batch_run replays it. `details` has `count` entries
+ * (capture_enabled = 0 for none); the descriptions are copied; the mosaics
+ * must stay alive while the batch exists. `lanes` = 0 lets the autotuner
+ * choose (every candidate is captured and timed on the batch; the choice is
+ * remembered per batch shape). The batch uses lanes 0..6 of its context; do
+ * not run other work on the context meanwhile.
+ * ------------------------------------------------------------------------- */
+typedef struct rhd_batch rhd_batch;
+
+rhd_status rhd_batch_create(rhd_context* ctx, uint32_t count, const rhd_mosaic* const* mosaics,
+                            const rhd_sensor_desc* descs, const rhd_detail_desc* details,
+                            uint32_t lanes, rhd_batch** out);
+/* Launches the graph once and waits. `statuses`, `metrics` (statistics only)
+ * and `gpu_ms` (device time of the launch) may be NULL. Returns the first
+ * failing frame's status, or Ok. */
+rhd_status rhd_batch_run(rhd_batch* batch, int32_t* statuses, rhd_metrics* metrics, float* gpu_ms);
+/* The i-th frame's RGB source, owned by the batch and rewritten by each run. */
+const rhd_source* rhd_batch_source(const rhd_batch* batch, uint32_t index);
+uint32_t rhd_batch_lanes(const rhd_batch* batch);
+void rhd_batch_destroy(rhd_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,12 @@ SIGNATURES = {
     "rhd_server_wait": (c_int, [_P, c_uint64, c_int32, POINTER(_P), POINTER(RequestInfo)]),
     "rhd_server_read_stats": (c_int, [_P, POINTER(ServerStats)]),
     "rhd_server_reset_stats": (c_int, [_P]),
+    "rhd_batch_create": (c_int, [_P, c_uint32, POINTER(_P), POINTER(SensorDesc),
+                                 POINTER(DetailDesc), c_uint32, POINTER(_P)]),
+    "rhd_batch_run": (c_int, [_P, POINTER(c_int32), POINTER(Metrics), POINTER(c_float)]),
+    "rhd_batch_source": (_P, [_P, c_uint32]),
+    "rhd_batch_lanes": (c_uint32, [_P]),
+    "rhd_batch_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -424,7 +424,10 @@ rhd_status finish_stats(rhd_context* ctx, Lane& L, rhd_metrics* m, int highlight
     RHD_HIP(ctx, hipMemcpyAsync(L.h_stats, L.d_stats, sizeof(DeviceStats),
                                 hipMemcpyDeviceToHost, L.stream));
     RHD_HIP(ctx, hipStreamSynchronize(L.stream));
-    const DeviceStats& s = *L.h_stats;
+    return stats_to_metrics(ctx, *L.h_stats, m, highlight);
+}
+
+rhd_status stats_to_metrics(rhd_context* ctx, const DeviceStats& s, rhd_metrics* m, int highlight) {
     if (m) {
         m->readback_bytes += sizeof(DeviceStats);
         m->highlight_method = highlight;
@@ -643,6 +646,62 @@ rhd_status mosaic_upload(rhd_context* ctx, Lane& L, const uint16_t* host, uint64
     }
     ctx->refs.fetch_add(1);
     *out = new rhd_mosaic{ctx, (uint16_t*)p, row_stride, width, height, bytes};
+    return RHD_OK;
+}
+
+// Grows lane L's scratch to what the stage needs for a frame of `d`.
+rhd_status stage_scratch(rhd_context* ctx, Lane& L, const rhd_sensor_desc* d,
+                         const rhd_detail_desc* dd) {
+    const int w = (int)d->crop_width, h = (int)d->crop_height;
+    rhd_status st = RHD_OK;
+    if (d->highlight == RHD_HIGHLIGHT_OPPOSED)
+        st = ensure_scratch(ctx, L, L.opposed, opposed_scratch_bytes(w, h));
+    if (st == RHD_OK && d->demosaic == RHD_DEMOSAIC_AMAZE && w > 32 && h > 32) {
+        const uint64_t tiles = std::min<uint64_t>(
+            (uint64_t)((w + 223) / 224) * ((h + 223) / 224), MAX_TILES_PER_LAUNCH);
+        st = ensure_scratch(ctx, L, L.amaze, tiles * amaze_scratch_bytes_per_tile());
+        if (st == RHD_OK) st = ensure_scratch(ctx, L, L.amaze_areas, tiles * 4 * sizeof(int));
+    }
+    if (st == RHD_OK && d->demosaic == RHD_DEMOSAIC_RCD && w > 24 && h > 24) {
+        const uint64_t tiles = std::min<uint64_t>(
+            (uint64_t)((w + 173) / 174) * ((h + 173) / 174), MAX_TILES_PER_LAUNCH);
+        st = ensure_scratch(ctx, L, L.rcd, tiles * rcd_scratch_bytes_per_tile());
+    }
+    if (st == RHD_OK && capture_active(dd))
+        st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
+    return st;
+}
+
+// Everything develop_on launches for one frame, without allocating or
+// waiting: `plane` and `src` are given, the lane's scratch must already fit
+// (stage_scratch), the statistics are reset first and copied to `stats_out`
+// (pinned host memory) last. A HIP graph may capture the sequence: it holds
+// kernels and copies only.
+rhd_status enqueue_develop(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
+                           const rhd_sensor_desc* d, const rhd_detail_desc* dd, rhd_plane* plane,
+                           rhd_source* src, DeviceStats* stats_out) {
+    NormalizeArgs a{};
+    rhd_status st = level_table(ctx, d, a.table);
+    if (st != RHD_OK) return st;
+    a.mosaic = mosaic->d;
+    a.row_stride = mosaic->row_stride;
+    a.crop_x = (int)d->crop_x;
+    a.crop_y = (int)d->crop_y;
+    a.width = (int)d->crop_width;
+    a.height = (int)d->crop_height;
+    a.crop_pattern = plane->pattern;
+    a.mode = d->highlight == RHD_HIGHLIGHT_CLIP ? 1 : (d->highlight == RHD_HIGHLIGHT_OPPOSED ? 2 : 0);
+    const float* lv = d->highlight == RHD_HIGHLIGHT_CLIP ? d->clip_levels : d->detection_levels;
+    for (int c = 0; c < 3; ++c) a.levels[c] = lv[c];
+    if ((st = reset_stats(ctx, L)) != RHD_OK) return st;
+    hipError_t e = launch_normalize(a, plane->d, L.d_stats, L.stream, PROF(L));
+    if (e != hipSuccess) return hip_fail(ctx, e, "normalize");
+    if (d->highlight == RHD_HIGHLIGHT_OPPOSED && (st = run_opposed(ctx, L, plane, d)) != RHD_OK)
+        return st;
+    if ((st = run_demosaic(ctx, L, plane, d, src)) != RHD_OK) return st;
+    if (capture_active(dd) && (st = run_capture(ctx, L, src, dd)) != RHD_OK) return st;
+    RHD_HIP(ctx, hipMemcpyAsync(stats_out, L.d_stats, sizeof(DeviceStats), hipMemcpyDeviceToHost,
+                                L.stream));
     return RHD_OK;
 }
 

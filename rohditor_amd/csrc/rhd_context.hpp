@@ -100,6 +100,16 @@ rhd_status new_source(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, rhd_sou
 rhd_status reset_stats(rhd_context* ctx, Lane& L);
 rhd_status finish_stats(rhd_context* ctx, Lane& L, rhd_metrics* m, int highlight);
 rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
+rhd_status stats_to_metrics(rhd_context* ctx, const DeviceStats& s, rhd_metrics* m, int highlight);
+rhd_status validate_desc(rhd_context* ctx, const rhd_sensor_desc* d);
+int crop_pattern(const rhd_sensor_desc* d);
+rhd_status new_plane(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, int32_t pattern,
+                     rhd_plane** out);
+rhd_status stage_scratch(rhd_context* ctx, Lane& L, const rhd_sensor_desc* d,
+                         const rhd_detail_desc* dd);
+rhd_status enqueue_develop(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
+                           const rhd_sensor_desc* d, const rhd_detail_desc* dd, rhd_plane* plane,
+                           rhd_source* src, DeviceStats* stats_out);
 rhd_status mosaic_upload(rhd_context* ctx, Lane& L, const uint16_t* host, uint64_t row_stride,
                          uint32_t width, uint32_t height, rhd_mosaic** out);
 // The whole stage on one lane; see api.cpp. With `base`, the demosaiced

@@ -555,3 +555,83 @@ class GpuServer:
 
     def reset_stats(self) -> None:
         self.proc._check(self.proc.lib.rhd_server_reset_stats(self.ptr))
+
+
+class BorrowedSource:
+    """A developed source owned by a batch (valid until the batch is closed)."""
+
+    def __init__(self, proc: GpuSensorProcessor, ptr):
+        self.proc = proc
+        self.ptr = ptr
+
+    def dims(self):
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        self.proc.lib.rhd_source_dims(self.ptr, ctypes.byref(w), ctypes.byref(h))
+        return w.value, h.value
+
+    def read_rgb(self) -> np.ndarray:
+        w, h = self.dims()
+        out = np.empty((h, w, 3), dtype=np.float32)
+        self.proc._check(self.proc.lib.rhd_source_read_rgb(
+            self.proc.ctx, self.ptr, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+
+class GpuBatch:
+    """Batched develop of resident mosaics of uneven sizes and recipes in one
+    HIP graph launch (include/rhd.h rhd_batch_*). `lanes` = 0 autotunes the
+    number of parallel branches."""
+
+    def __init__(self, proc: GpuSensorProcessor, mosaics: Sequence[ResidentMosaic],
+                 descs: Sequence[SensorDevelopmentDescription], details, lanes: int = 0):
+        n = len(mosaics)
+        self.proc = proc
+        self.count = n
+        self._mosaics = list(mosaics)  # must outlive the batch
+        cdescs = (_lib.SensorDesc * n)()
+        keep = []
+        for i, d in enumerate(descs):
+            c = d.to_c()
+            keep.append(d._keep)
+            cdescs[i] = c
+        cdetails = (_lib.DetailDesc * n)(*details)
+        ptrs = (ctypes.c_void_p * n)(*[m.ptr.value for m in mosaics])
+        out = ctypes.c_void_p()
+        proc._check(proc.lib.rhd_batch_create(proc.ctx, n, ptrs, cdescs, cdetails, lanes,
+                                              ctypes.byref(out)))
+        self.ptr = out
+
+    @property
+    def lanes(self) -> int:
+        return int(self.proc.lib.rhd_batch_lanes(self.ptr))
+
+    def run(self):
+        """Launches the graph once; returns (per-frame statuses, device ms)."""
+        statuses = (ctypes.c_int32 * self.count)()
+        ms = ctypes.c_float()
+        self.proc.lib.rhd_batch_run(self.ptr, statuses, None, ctypes.byref(ms))
+        return list(statuses), ms.value
+
+    def source(self, i: int) -> BorrowedSource:
+        return BorrowedSource(self.proc, self.proc.lib.rhd_batch_source(self.ptr, i))
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.proc.lib.rhd_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def no_capture_detail() -> "_lib.DetailDesc":
+    """A detail contract with capture sharpening off (in-range settings)."""
+    dd = _lib.DetailDesc()
+    dd.capture_enabled = 0
+    dd.amount, dd.sigma, dd.noise_protection = 0.0, 0.5, 0.0
+    dd.ceilings[:] = [1.0, 1.0, 1.0]
+    dd.iterations = 0
+    return dd

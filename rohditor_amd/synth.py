@@ -116,6 +116,16 @@ def frame(spec: SceneSpec) -> RawFrame:
     return RawFrame(info=info, mosaic=mosaic(spec))
 
 
+def frame_from_mosaic(samples: np.ndarray, cfa: str, black: float = 512.0,
+                      white: float = 16383.0) -> RawFrame:
+    """A RawFrame around existing u16 samples, labelled with `cfa`."""
+    h, w = samples.shape
+    info = RawFileInfo(width=w, height=h, cfa=cfa, black_levels=LevelPattern([black]),
+                       white_levels=(white,), as_shot_white_balance=AS_SHOT,
+                       active_area=ImageRect(0, 0, w, h))
+    return RawFrame(info=info, mosaic=samples)
+
+
 # Workloads (seeds and sizes of the benchmark configurations).
 def cpu_case() -> SceneSpec:
     """1536x1024 RGGB, 14-bit, 2 % clipped; normalize + Clip + AMaZE."""
