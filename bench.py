@@ -41,6 +41,11 @@ def parse():
                    help="run the batched workload instead: 32 mosaics of 12-61 MP with mixed CFA "
                         "phases, demosaic algorithms and RL iterations in one graph launch")
     p.add_argument("--batch-frames", type=int, default=32)
+    p.add_argument("--serve", action="store_true",
+                   help="run the serving workload instead: Poisson arrivals over 64 resident "
+                        "24-45 MP frames, against the plain serial path on the same schedule")
+    p.add_argument("--serve-requests", type=int, default=400)
+    p.add_argument("--serve-rate", type=float, default=40.0, help="mean arrivals per second")
     p.add_argument("--batch-lanes", type=int, default=0, help="0: autotune")
     p.add_argument("--served-lanes", type=int, default=3)
     p.add_argument("--served-requests", type=int, default=60)
@@ -193,6 +198,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.serve:
+        serve_bench(args)
         return
     from rohditor_amd import _lib, sensor
 
@@ -393,6 +401,138 @@ def batch_bench(args):
     for r in residents:
         r.release()
     proc.close()
+    print(json.dumps(line))
+
+
+SERVE_SIZES = [(6048, 4024), (7008, 4672), (8256, 5504)]
+
+
+def serve_bench(args):
+    """The serving workload: 64 resident frames of 24, 33 and 45 MP; Poisson
+    arrivals (`--serve-rate` per second, seeded); 70 % re-develops of a frame
+    already developed that change its highlight method or demosaic
+    algorithm, 30 % first develops of a frame; RL (sigma 0.7, 10 iterations)
+    throughout. The server runs with newest-wins, admission by device bytes
+    and the demosaiced-base cache. The same schedule is then replayed through
+    the plain path, one develop at a time in arrival order."""
+    import threading
+    from rohditor_amd import sensor, synth
+    rng = np.random.default_rng(0x52480444)
+    frames_n, n = 64, args.serve_requests
+    scenes = [synth.mosaic(synth.SceneSpec(width=w, height=h, seed=0x52480400 + k,
+                                           blob_fraction=0.05))
+              for k, (w, h) in enumerate(SERVE_SIZES)]
+    cfas = ["RGGB", "BGGR", "GRBG"]
+    frames = [synth.frame_from_mosaic(scenes[i % 3], cfas[(i // 3) % 3]) for i in range(frames_n)]
+    methods = [sensor.HighlightMethod.OPPOSED, sensor.HighlightMethod.CLIP]
+    algos = [sensor.DemosaicAlgorithm.AMAZE, sensor.DemosaicAlgorithm.RCD,
+             sensor.DemosaicAlgorithm.MHC]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    developed, current, reqs = [], {}, []
+    for i in range(n):
+        if developed and rng.random() < 0.7:
+            f = int(developed[rng.integers(0, len(developed))])
+            m, a = current[f]
+            if rng.random() < 0.5:
+                m = 1 - m
+            else:
+                a = (a + 1 + int(rng.integers(0, 2))) % 3
+        else:
+            f = int(rng.integers(0, frames_n))
+            m, a = 0, 0
+            if f not in current:
+                developed.append(f)
+        current[f] = (m, a)
+        recipe = sensor.EditRecipe(
+            highlights=sensor.HighlightAdjustments(method=methods[m]),
+            capture=sensor.CaptureSharpening(enabled=True, amount=0.5, radius=0.7,
+                                             noise_protection=0.5),
+            capture_iterations=10)
+        desc = sensor.SensorDevelopmentDescription.from_frame(
+            frames[f], recipe, sensor.RenderOptions(demosaic=algos[a]))
+        reqs.append((float(arrivals[i]), f, desc, sensor.detail_desc(recipe, desc.demosaic_balance)))
+    mp_of = lambda d: d.crop.width * d.crop.height / 1e6
+
+    # Served.
+    proc = sensor.GpuSensorProcessor(0, budget_bytes=200 << 30)
+    server = sensor.GpuServer(proc, lanes=args.served_lanes, admission_bytes=48 << 30,
+                              base_cache_bytes=16 << 30)
+    for f in range(frames_n):
+        server.put_frame(f, frames[f])
+    for _, f, desc, detail in reqs[:4]:  # warm-up: lane scratch and the device pool
+        server.wait(server.submit(f, desc, detail), keep=False)
+    server.reset_stats()
+    ids = [None] * n
+    t_start = time.perf_counter()
+
+    def submitter():
+        for i, (at, f, desc, detail) in enumerate(reqs):
+            delay = t_start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids[i] = server.submit(f, desc, detail)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    done_mp, statuses = 0.0, {}
+    for i in range(n):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        r = server.wait(ids[i], keep=False)
+        statuses[r.status] = statuses.get(r.status, 0) + 1
+        if r.ok:
+            done_mp += r.megapixels
+    th.join()
+    st = server.stats()
+    served_line = {"value": round(st.megapixels / (st.span_ms * 1e-3), 2),
+                   "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+                   "mean_ms": round(st.mean_ms, 3), "completed": int(st.completed),
+                   "coalesced": int(st.coalesced), "cancelled": int(st.cancelled),
+                   "base_hits": int(st.base_hits), "failed": int(st.failed),
+                   "span_s": round(st.span_ms * 1e-3, 3)}
+    server.close()
+
+    # Plain: the same schedule, one develop at a time in arrival order.
+    residents = {}
+    lat, busy = [], 0.0
+    for f in sorted(set(r[1] for r in reqs)):
+        residents[f] = proc.upload(frames[f])
+    for _, f, desc, detail in reqs[:4]:  # warm-up
+        proc.develop_resident(residents[f], desc, detail).release()
+    t_start = time.perf_counter()
+    total_mp = 0.0
+    for at, f, desc, detail in reqs:
+        now = time.perf_counter() - t_start
+        if now < at:
+            time.sleep(at - now)
+        t0 = time.perf_counter()
+        proc.develop_resident(residents[f], desc, detail).release()
+        t1 = time.perf_counter()
+        busy += t1 - t0
+        lat.append((t1 - t_start - at) * 1e3)
+        total_mp += mp_of(desc)
+    span = time.perf_counter() - t_start
+    for r in residents.values():
+        r.release()
+    proc.close()
+    lat = np.array(lat)
+    plain_line = {"value": round(total_mp / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                  "busy_mp_per_s": round(total_mp / busy, 2), "completed": n,
+                  "span_s": round(span, 3)}
+    line = {
+        "metric": "served sensor MP/s and p50/p99 develop latency vs plain serial GPU path",
+        "value": served_line["value"], "unit": "MP/s", "n_gpus": 1, "higher_is_better": True,
+        "requests": n, "rate_per_s": args.serve_rate, "lanes": args.served_lanes,
+        "served": served_line, "plain_serial": plain_line,
+        "latency_p99_vs_plain": round(plain_line["p99_ms"] / max(served_line["p99_ms"], 1e-9), 3),
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson arrivals ({args.serve_rate}/s, seed 0x52480444) over 64 "
+                               "resident frames of 6048x4024, 7008x4672 and 8256x5504; 70 % "
+                               "re-develops changing highlight (Opposed/Clip) or demosaic "
+                               "(AMaZE/RCD/MHC), 30 % first develops; RL sigma 0.7, 10 "
+                               "iterations; newest-wins, 48 GiB admission, 16 GiB base cache"},
+    }
     print(json.dumps(line))
 
 
