@@ -204,6 +204,11 @@ void rhd_plane_release(rhd_plane* plane);
 /* Interleaved W x H x 3 readback matching LinearRgbImage. */
 rhd_status rhd_source_read_rgb(rhd_context* ctx, const rhd_source* source, float* dst,
                                uint64_t count);
+/* A resident camera source from interleaved W x H x 3 host RGB, the
+ * counterpart of GpuSpatialProcessor::upload_captured_source
+ * (crates/gpu/src/spatial/mod.rs:110) for a CPU-developed source. */
+rhd_status rhd_source_upload(rhd_context* ctx, const float* rgb, uint32_t width, uint32_t height,
+                             rhd_source** out);
 /* Device pointers of the three planes (R, G, B; W*H floats each). */
 rhd_status rhd_source_planes(const rhd_source* source, float** red, float** green, float** blue);
 void rhd_source_dims(const rhd_source* source, uint32_t* width, uint32_t* height);
@@ -223,6 +228,71 @@ rhd_status rhd_profile_reset(rhd_context* ctx);
 uint32_t rhd_profile_count(rhd_context* ctx);
 rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,
                              uint64_t* launches, uint64_t* bytes_per_launch);
+
+/* ---------------------------------------------------------------------------
+ * Spatial completion: lens correction and exact area reduction of a resident
+ * camera source, the step after the stage (GpuSpatialProcessor,
+ * crates/gpu/src/spatial/mod.rs:33-344 and reduction.rs:15-316, over the
+ * CPU contract crates/optics/src/correction.rs and crates/core/src/
+ * resample.rs).
+ * ------------------------------------------------------------------------- */
+typedef enum rhd_distortion_model {
+    RHD_DISTORTION_NONE = 0,
+    RHD_DISTORTION_POLY3 = 1,  /* k1 */
+    RHD_DISTORTION_POLY5 = 2,  /* k1, k2 */
+    RHD_DISTORTION_PTLENS = 3  /* a, b, c */
+} rhd_distortion_model;
+
+typedef enum rhd_tca_model {
+    RHD_TCA_NONE = 0,
+    RHD_TCA_LINEAR = 1,  /* red[0] = kr, blue[0] = kb (already inverted, plan.rs:391-424) */
+    RHD_TCA_POLY3 = 2    /* [v, c, b] per channel */
+} rhd_tca_model;
+
+typedef enum rhd_vignetting_model { RHD_VIGNETTING_NONE = 0, RHD_VIGNETTING_PA = 1 } rhd_vignetting_model;
+
+/* A resolved OpticsExecution (crates/optics/src/plan.rs:36-48): dimensions of
+ * the corrected image (the source's), the f64 normalisation and lens centre,
+ * the automatic scale, and the rescaled model coefficients. Lens matching and
+ * calibration interpolation happen before, on the host. */
+typedef struct rhd_optics_desc {
+    uint32_t width, height;
+    double norm_scale, norm_unscale;
+    double center_x, center_y;
+    float scale;
+    int32_t distortion;    /* rhd_distortion_model */
+    float distortion_k[3];
+    int32_t tca;           /* rhd_tca_model */
+    float tca_red[3];
+    float tca_blue[3];
+    int32_t vignetting;    /* rhd_vignetting_model */
+    float vignetting_k[3]; /* PA k1, k2, k3 */
+} rhd_optics_desc;
+
+/* Preview dimensions for a long-edge limit (crates/core/src/cpu/stages.rs:
+ * 112-147): (d * max_long + long / 2) / long clamped to [2, d]. */
+rhd_status rhd_preview_dimensions(uint32_t width, uint32_t height, uint32_t max_long_edge,
+                                  uint32_t* target_width, uint32_t* target_height);
+/* Automatic scale of a plan (find_scale, crates/optics/src/plan.rs:521-559):
+ * the first of 1, 1.025, 1.025^2 ... (160 tries) at which 256 perimeter
+ * probes map inside the cubic footprint; the desc's own `scale` is ignored.
+ * RHD_INVALID_INPUT when none does. Host only. */
+rhd_status rhd_optics_find_scale(const rhd_optics_desc* optics, float* scale);
+/* GpuSpatialProcessor::reduce_preview (crates/gpu/src/spatial/reduction.rs:
+ * 15): optics (NULL = off) then exact area reduction to target_width x
+ * target_height, into a new source; `source` is not modified. Equal
+ * dimensions skip the reduction (resample.rs:253-255). An invalid vignetting
+ * gain, a non-finite value or a cubic footprint outside the image gives
+ * RHD_INVALID_INPUT and no output. `gpu_ms` (nullable): device time. */
+rhd_status rhd_reduce_preview(rhd_context* ctx, const rhd_source* source,
+                              const rhd_optics_desc* optics, uint32_t target_width,
+                              uint32_t target_height, const volatile uint8_t* cancel,
+                              rhd_source** out, float* gpu_ms);
+/* GpuSpatialProcessor::full_resolution_source (crates/gpu/src/spatial/
+ * mod.rs:200): the corrected image at the source's size. */
+rhd_status rhd_optics_full(rhd_context* ctx, const rhd_source* source,
+                           const rhd_optics_desc* optics, const volatile uint8_t* cancel,
+                           rhd_source** out, float* gpu_ms);
 
 /* ---------------------------------------------------------------------------
  * Serving loop: the counterpart of the engine worker's develop loop

@@ -169,5 +169,84 @@ def label_components(mask: np.ndarray) -> tuple[np.ndarray, int]:
     return lab, int(count.value)
 
 
+# Optics and area reduction (oracle/optics.hpp). `desc` is any ctypes
+# structure with the rhd_optics_desc layout (include/rhd.h).
+
+def optics_apply(rgb: np.ndarray, desc) -> tuple[int, np.ndarray]:
+    """crates/optics/src/correction.rs:69-262 (authority path) on (H, W, 3) RGB."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    out = np.zeros_like(v)
+    st = lib().rhdo_optics_apply(_p(v), ctypes.byref(desc), _p(out))
+    return st, out
+
+
+def optics_map(desc, scale: float, x: int, y: int) -> tuple[int, np.ndarray]:
+    """Source (x, y) per channel of output pixel (x, y) (plan.rs:465-496)."""
+    out = np.zeros(6, dtype=np.float32)
+    st = lib().rhdo_optics_map(ctypes.byref(desc), ctypes.c_float(scale), _sz(x), _sz(y),
+                               _p(out))
+    return st, out
+
+
+def optics_find_scale(desc) -> tuple[int, float]:
+    out = ctypes.c_float()
+    st = lib().rhdo_optics_find_scale(ctypes.byref(desc), ctypes.byref(out))
+    return st, float(out.value)
+
+
+def inverse_model(model: int, k, x: float, y: float) -> np.ndarray:
+    """1 Poly3, 2 Poly5, 3 PTLens inverse distortion; 4 reverse TCA Poly3 [v, c, b]."""
+    kk = np.zeros(3, dtype=np.float32)
+    kk[:len(k)] = k
+    out = np.zeros(2, dtype=np.float32)
+    lib().rhdo_inverse_model(ctypes.c_int(model), _p(kk), ctypes.c_float(x), ctypes.c_float(y),
+                             _p(out))
+    return out
+
+
+def cubic_sample(rgb: np.ndarray, channel: int, x: float, y: float) -> tuple[int, float]:
+    """crates/optics/src/resample.rs:3-42."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = ctypes.c_float()
+    st = lib().rhdo_cubic_sample(_p(v), _sz(v.shape[1]), _sz(v.shape[0]), ctypes.c_int(channel),
+                                 ctypes.c_float(x), ctypes.c_float(y), ctypes.byref(out))
+    return st, float(out.value)
+
+
+def cubic_weight(distance: float) -> float:
+    lib().rhdo_cubic_weight.restype = ctypes.c_float
+    return float(lib().rhdo_cubic_weight(ctypes.c_float(distance)))
+
+
+def area_axis(source: int, target: int):
+    """crates/core/src/resample.rs:46-112: (first, offset, count, weights)."""
+    first = np.zeros(target, np.uint64)
+    offset = np.zeros(target, np.uint64)
+    count = np.zeros(target, np.uint64)
+    weights = np.zeros(source + 2 * target, np.float32)
+    lib().rhdo_area_axis.restype = ctypes.c_long
+    n = lib().rhdo_area_axis(_sz(source), _sz(target), _p(first), _p(offset), _p(count),
+                             _p(weights), _sz(weights.size))
+    if n < 0:
+        raise ValueError("invalid area axis")
+    return first, offset, count, weights[:n].copy()
+
+
+def resize_area(rgb: np.ndarray, target_width: int, target_height: int) -> tuple[int, np.ndarray]:
+    """crates/core/src/resample.rs:235-316 on (H, W, 3) RGB."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros((target_height, target_width, 3), np.float32)
+    st = lib().rhdo_resize_area(_p(v), _sz(v.shape[1]), _sz(v.shape[0]), _p(out),
+                                _sz(target_width), _sz(target_height))
+    return st, out
+
+
+def preview_dimensions(width: int, height: int, max_long_edge: int) -> tuple[int, int, int]:
+    tw, th = ctypes.c_size_t(), ctypes.c_size_t()
+    st = lib().rhdo_preview_dimensions(_sz(width), _sz(height), _sz(max_long_edge),
+                                       ctypes.byref(tw), ctypes.byref(th))
+    return st, int(tw.value), int(th.value)
+
+
 def cpu_threads() -> int:
     return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)

@@ -9,6 +9,7 @@
 #include "capture.hpp"
 #include "demosaic.hpp"
 #include "opposed.hpp"
+#include "optics.hpp"
 #include "rcd.hpp"
 #include "sensor.hpp"
 
@@ -137,6 +138,96 @@ int rhdo_rcd_probe(const float* mosaic, size_t width, size_t height, int pattern
 int rhdo_label_components(uint32_t* labels, size_t width, size_t height, uint32_t* count) {
     *count = opposed::label_components(labels, width, height);
     return 0;
+}
+
+// Same field order and types as rhd_optics_desc in include/rhd.h, so one
+// ctypes structure serves both libraries.
+struct OpticsDesc {
+    uint32_t width, height;
+    double norm_scale, norm_unscale, center_x, center_y;
+    float scale;
+    int32_t distortion;
+    float distortion_k[3];
+    int32_t tca;
+    float tca_red[3], tca_blue[3];
+    int32_t vignetting;
+    float vignetting_k[3];
+};
+
+static optics::Execution execution_of(const OpticsDesc* d) {
+    optics::Execution e{};
+    e.width = d->width;
+    e.height = d->height;
+    e.norm_scale = d->norm_scale;
+    e.norm_unscale = d->norm_unscale;
+    e.center_x = d->center_x;
+    e.center_y = d->center_y;
+    e.scale = d->scale;
+    e.distortion = d->distortion;
+    e.tca = d->tca;
+    e.vignetting = d->vignetting;
+    for (int i = 0; i < 3; ++i) {
+        e.dist[i] = d->distortion_k[i];
+        e.red[i] = d->tca_red[i];
+        e.blue[i] = d->tca_blue[i];
+        e.vig[i] = d->vignetting_k[i];
+    }
+    return e;
+}
+
+// rgb: tight interleaved RGB, vignetted in place; out: the corrected image.
+int rhdo_optics_apply(float* rgb, const OpticsDesc* d, float* out) {
+    return optics::apply(rgb, execution_of(d), out);
+}
+
+int rhdo_optics_map(const OpticsDesc* d, float scale, size_t x, size_t y, float out[6]) {
+    return optics::map_coordinates(execution_of(d), scale, x, y, out);
+}
+
+int rhdo_optics_find_scale(const OpticsDesc* d, float* out) {
+    return optics::find_scale(execution_of(d), out);
+}
+
+// model: 1 Poly3, 2 Poly5, 3 PTLens; 4 = TCA Poly3 reverse with k = [v, c, b].
+int rhdo_inverse_model(int model, const float k[3], float x, float y, float out[2]) {
+    switch (model) {
+        case 1: optics::undist_poly3(x, y, k[0], &out[0], &out[1]); return 0;
+        case 2: optics::undist_poly5(x, y, k[0], k[1], &out[0], &out[1]); return 0;
+        case 3: optics::undist_ptlens(x, y, k[0], k[1], k[2], &out[0], &out[1]); return 0;
+        case 4: optics::tca_poly3_channel(x, y, k, &out[0], &out[1]); return 0;
+        default: return -1;
+    }
+}
+
+int rhdo_cubic_sample(const float* rgb, size_t width, size_t height, int channel, float x,
+                      float y, float* out) {
+    return optics::cubic_sample(rgb, width * 3, width, height, channel, x, y, out);
+}
+
+float rhdo_cubic_weight(float distance) { return optics::cubic_weight(distance); }
+
+// Writes first/offset/count per target (capacity `target`) and the weights
+// (capacity `weight_capacity`); returns the weight count, or -1.
+long rhdo_area_axis(size_t source, size_t target, uint64_t* first, uint64_t* offset,
+                    uint64_t* count, float* weights, size_t weight_capacity) {
+    optics::Axis a;
+    if (!optics::make_axis(source, target, &a)) return -1;
+    if (a.weights.size() > weight_capacity) return -1;
+    for (size_t t = 0; t < target; ++t) {
+        first[t] = a.first[t];
+        offset[t] = a.offset[t];
+        count[t] = a.count[t];
+    }
+    for (size_t i = 0; i < a.weights.size(); ++i) weights[i] = a.weights[i];
+    return (long)a.weights.size();
+}
+
+int rhdo_resize_area(const float* rgb, size_t w, size_t h, float* out, size_t tw, size_t th) {
+    return optics::resize_area(rgb, w, h, out, tw, th);
+}
+
+int rhdo_preview_dimensions(size_t w, size_t h, size_t max_long, size_t* tw, size_t* th) {
+    return optics::preview_dimensions(w, h, max_long, tw, th);
 }
 
 }  // extern "C"

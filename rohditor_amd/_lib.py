@@ -82,6 +82,16 @@ class ServerStats(ctypes.Structure):
                 ("running_bytes", c_uint64)]
 
 
+class OpticsDesc(ctypes.Structure):
+    """rhd_optics_desc: a resolved OpticsExecution (crates/optics/src/plan.rs:36-48)."""
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("norm_scale", ctypes.c_double),
+                ("norm_unscale", ctypes.c_double), ("center_x", ctypes.c_double),
+                ("center_y", ctypes.c_double), ("scale", c_float), ("distortion", c_int32),
+                ("distortion_k", c_float * 3), ("tca", c_int32), ("tca_red", c_float * 3),
+                ("tca_blue", c_float * 3), ("vignetting", c_int32),
+                ("vignetting_k", c_float * 3)]
+
+
 # Every symbol include/rhd.h declares, with (restype, argtypes).
 _P = c_void_p
 SIGNATURES = {
@@ -110,6 +120,7 @@ SIGNATURES = {
     "rhd_plane_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
     "rhd_plane_release": (None, [_P]),
     "rhd_source_read_rgb": (c_int, [_P, _P, _P, c_uint64]),
+    "rhd_source_upload": (c_int, [_P, _P, c_uint32, c_uint32, POINTER(_P)]),
     "rhd_source_planes": (c_int, [_P, POINTER(_P), POINTER(_P), POINTER(_P)]),
     "rhd_source_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32)]),
     "rhd_source_release": (None, [_P]),
@@ -119,6 +130,12 @@ SIGNATURES = {
     "rhd_profile_count": (c_uint32, [_P]),
     "rhd_profile_entry": (c_int, [_P, c_uint32, POINTER(c_char_p), POINTER(ctypes.c_double),
                                   POINTER(c_uint64), POINTER(c_uint64)]),
+    "rhd_preview_dimensions": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_uint32),
+                                       POINTER(c_uint32)]),
+    "rhd_optics_find_scale": (c_int, [POINTER(OpticsDesc), POINTER(c_float)]),
+    "rhd_reduce_preview": (c_int, [_P, _P, POINTER(OpticsDesc), c_uint32, c_uint32, _P,
+                                   POINTER(_P), POINTER(c_float)]),
+    "rhd_optics_full": (c_int, [_P, _P, POINTER(OpticsDesc), _P, POINTER(_P), POINTER(c_float)]),
     "rhd_server_create": (c_int, [_P, POINTER(ServerConfig), POINTER(_P)]),
     "rhd_server_destroy": (None, [_P]),
     "rhd_server_put_mosaic": (c_int, [_P, c_uint64, _P, c_uint64, c_uint32, c_uint32]),

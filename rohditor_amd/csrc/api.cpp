@@ -39,7 +39,6 @@ void context_unref(rhd_context* ctx) {
 }
 }  // namespace rhd
 
-#define PROF(L) ((L).profiling ? &(L).profiler : nullptr)
 
 namespace rhd {
 
@@ -117,11 +116,6 @@ rhd_status hip_fail(rhd_context* ctx, hipError_t e, const char* what) {
     return fail(ctx, RHD_SYNCHRONIZATION, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define RHD_HIP(ctx, call)                                          \
-    do {                                                            \
-        hipError_t _e = (call);                                     \
-        if (_e != hipSuccess) return hip_fail((ctx), _e, #call);    \
-    } while (0)
 
 bool cancelled(const volatile uint8_t* flag) {
     return flag && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != 0;
@@ -194,6 +188,7 @@ void drop_scratch(rhd_context* ctx, Lane& L) {
     free_buffer(ctx, L.rcd);
     free_buffer(ctx, L.capture);
     free_buffer(ctx, L.opposed);
+    free_buffer(ctx, L.optics);
 }
 
 // Reserve and allocate: a pooled block of the same size if there is one,
@@ -1147,6 +1142,35 @@ rhd_status rhd_source_read_rgb(rhd_context* ctx, const rhd_source* s, float* dst
         dst[3 * i + 1] = planar[n + i];
         dst[3 * i + 2] = planar[2 * n + i];
     }
+    return RHD_OK;
+}
+
+rhd_status rhd_source_upload(rhd_context* ctx, const float* rgb, uint32_t width, uint32_t height,
+                             rhd_source** out) {
+    if (!ctx || !out || !rgb) return RHD_INVALID_INPUT;
+    *out = nullptr;
+    if (width == 0 || height == 0)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "camera source must be non-empty");
+    (void)hipSetDevice(ctx->device);
+    Lane& L = ctx->lanes[0];
+    rhd_source* src = nullptr;
+    rhd_status st = new_source(ctx, &L, width, height, &src);
+    if (st != RHD_OK) return st;
+    const size_t n = (size_t)width * height;
+    std::vector<float> planar(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        planar[i] = rgb[3 * i];
+        planar[n + i] = rgb[3 * i + 1];
+        planar[2 * n + i] = rgb[3 * i + 2];
+    }
+    hipError_t e = hipMemcpyAsync(src->d, planar.data(), 3 * n * sizeof(float),
+                                  hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+    if (e != hipSuccess) {
+        rhd_source_release(src);
+        return hip_fail(ctx, e, "camera source upload");
+    }
+    *out = src;
     return RHD_OK;
 }
 

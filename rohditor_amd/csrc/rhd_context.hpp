@@ -33,7 +33,7 @@ struct Lane {
     hipStream_t stream = nullptr;
     DeviceStats* d_stats = nullptr;
     DeviceStats* h_stats = nullptr;
-    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed;
+    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed, optics;
     hipEvent_t ev[5] = {};
     bool profiling = false;
     Profiler profiler;
@@ -83,9 +83,23 @@ struct rhd_source {
     uint32_t width, height;
     uint64_t bytes;
 };
+#define PROF(L) ((L).profiling ? &(L).profiler : nullptr)
+
+#define RHD_HIP(ctx, call)                                          \
+    do {                                                            \
+        hipError_t _e = (call);                                     \
+        if (_e != hipSuccess) return hip_fail((ctx), _e, #call);    \
+    } while (0)
+
 namespace rhd {
 
 rhd_status fail(rhd_context* ctx, rhd_status status, const std::string& msg);
+rhd_status hip_fail(rhd_context* ctx, hipError_t e, const char* what);
+rhd_status check_handle_ctx(rhd_context* ctx, const void* handle_ctx);
+// Grows a lane scratch buffer to at least `bytes` (waits for the lane first).
+rhd_status ensure_scratch(rhd_context* ctx, Lane& L, DeviceBuffer& b, uint64_t bytes);
+// Milliseconds between two recorded events (0 on error).
+float elapsed(hipEvent_t a, hipEvent_t b);
 // Drops one reference; the last one frees the lanes and the pool.
 void context_unref(rhd_context* ctx);
 // Waits for the lane, then returns its scratch to the pool.
@@ -118,5 +132,12 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
                       const rhd_sensor_desc* d, const rhd_detail_desc* dd,
                       const volatile uint8_t* cancel, rhd_source** out, rhd_metrics* m,
                       rhd_source** base);
+// Optics correction and area reduction of `src` on one lane (spatial.cpp):
+// `optics` may be null; target equal to the source size with `full` set
+// gives the full-resolution corrected source.
+rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
+                      const rhd_optics_desc* optics, uint32_t target_width,
+                      uint32_t target_height, bool full, const volatile uint8_t* cancel,
+                      rhd_source** out, float* gpu_ms);
 
 }  // namespace rhd

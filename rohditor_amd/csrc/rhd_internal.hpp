@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/rhd.h"
+#include "optics_math.hpp"
 #include "rhd_device.hpp"
 
 namespace rhd {
@@ -139,5 +140,33 @@ struct OpposedArgs {
 };
 size_t opposed_scratch_bytes(int width, int height);
 hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);
+
+// One optics + area-reduction launch sequence (optics.hip). Planes are
+// planar f32 R, G, B of width x height; tables are device copies of what the
+// host built for the plan.
+struct OpticsLaunch {
+    const float* src;
+    float* vignetted;            // 3 planes; needed when vignetting precedes a remap
+    int width, height;
+    bool vignetting, remap, direct;
+    const float* vig_steps;      // width floats
+    float* vig_rows;             // optics_checkpoints(width) * height floats
+    float vig_start_x, vig_start_y, vig_ns;
+    float vig_k[3];
+    optics::Model model;
+    const float* in_x;           // width floats
+    const float* in_y;           // height floats
+    const uint32_t *h_first, *h_offset, *h_count;
+    const float* h_weights;
+    const uint32_t *v_first, *v_offset, *v_count;
+    const float* v_weights;
+    int target_width, target_height;
+    int group;                   // output columns per horizontal workgroup (1..256)
+    float* mid;                  // 3 planes of target_width x height
+    float* out;                  // 3 planes of the output
+    DeviceStats* stats;
+};
+size_t optics_checkpoints(int width);
+hipError_t launch_optics(const OpticsLaunch& a, hipStream_t s, Profiler* prof);
 
 }  // namespace rhd

@@ -438,6 +438,41 @@ class GpuSensorProcessor:
         self._check(self.lib.rhd_capture(self.ctx, source.ptr, ctypes.byref(detail),
                                          cancellation.ptr if cancellation else None, None))
 
+    def upload_source(self, rgb: np.ndarray) -> GpuDevelopedSource:
+        """A resident camera source from interleaved (H, W, 3) RGB
+        (GpuSpatialProcessor::upload_captured_source, crates/gpu/src/spatial/mod.rs:110)."""
+        v = np.ascontiguousarray(rgb, dtype=np.float32)
+        ptr = ctypes.c_void_p()
+        self._check(self.lib.rhd_source_upload(self.ctx, v.ctypes.data_as(ctypes.c_void_p),
+                                               v.shape[1], v.shape[0], ctypes.byref(ptr)))
+        return GpuDevelopedSource(self, ptr)
+
+    def reduce_preview(self, source: GpuDevelopedSource, optics, target: tuple,
+                       cancellation: Optional[CancellationToken] = None):
+        """Optics (an OpticsExecution or None) then exact area reduction to
+        `target` = (width, height) (GpuSpatialProcessor::reduce_preview,
+        crates/gpu/src/spatial/reduction.rs:15). Returns (source, gpu_ms)."""
+        ptr = ctypes.c_void_p()
+        ms = ctypes.c_float()
+        d = optics.to_c() if optics is not None else None
+        self._check(self.lib.rhd_reduce_preview(
+            self.ctx, source.ptr, ctypes.byref(d) if d is not None else None, target[0],
+            target[1], cancellation.ptr if cancellation else None, ctypes.byref(ptr),
+            ctypes.byref(ms)))
+        return GpuDevelopedSource(self, ptr), float(ms.value)
+
+    def full_resolution_source(self, source: GpuDevelopedSource, optics,
+                               cancellation: Optional[CancellationToken] = None):
+        """The corrected image at the source's size
+        (GpuSpatialProcessor::full_resolution_source, crates/gpu/src/spatial/mod.rs:200)."""
+        ptr = ctypes.c_void_p()
+        ms = ctypes.c_float()
+        d = optics.to_c() if optics is not None else None
+        self._check(self.lib.rhd_optics_full(
+            self.ctx, source.ptr, ctypes.byref(d) if d is not None else None,
+            cancellation.ptr if cancellation else None, ctypes.byref(ptr), ctypes.byref(ms)))
+        return GpuDevelopedSource(self, ptr), float(ms.value)
+
     def develop_resident(self, mosaic: ResidentMosaic, desc: SensorDevelopmentDescription,
                          detail=None, cancellation: Optional[CancellationToken] = None,
                          metrics: Optional["_lib.Metrics"] = None) -> GpuDevelopedSource:
