@@ -189,6 +189,7 @@ void drop_scratch(rhd_context* ctx, Lane& L) {
     free_buffer(ctx, L.capture);
     free_buffer(ctx, L.opposed);
     free_buffer(ctx, L.optics);
+    L.optics_rows = Lane::RowsKey{};
 }
 
 // Reserve and allocate: a pooled block of the same size if there is one,

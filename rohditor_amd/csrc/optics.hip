@@ -62,34 +62,77 @@ __device__ __forceinline__ float vig_multiplier(const Vig& v, float r2, bool& ba
     return 1.0f / gain;
 }
 
+// One thread per row; the row-independent increments pass through LDS in
+// chunks so that every lane reads the same word (a broadcast).
+constexpr int ROW_CHUNK = 1024;
 __global__ __launch_bounds__(64) void opt_vig_rows(const float* __restrict__ steps, float start_x,
                                                    float start_y, float ns, int width, int height,
                                                    float* __restrict__ rows) {
+    __shared__ __align__(16) float chunk[ROW_CHUNK];
     const int y = blockIdx.x * 64 + threadIdx.x;
-    if (y >= height) return;
-    const float yy = start_y + ns * (float)y;
+    const bool live = y < height;
+    const float yy = start_y + ns * (float)(live ? y : 0);
     float r2 = start_x * start_x + yy * yy;
-    for (int k0 = 0; k0 < width; k0 += CK) {
-        rows[(size_t)(k0 / CK) * height + y] = r2;
-        const int end = min(k0 + CK, width);
-        for (int k = k0; k < end; ++k) r2 += steps[k];
+    for (int c0 = 0; c0 < width; c0 += ROW_CHUNK) {
+        const int n = min(ROW_CHUNK, width - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += 64) chunk[i] = steps[c0 + i];
+        __syncthreads();
+        int k0 = 0;
+        for (; k0 + CK <= n; k0 += CK) {
+            if (live) rows[(size_t)((c0 + k0) / CK) * height + y] = r2;
+            // 16 increments read ahead of the dependent additions.
+            const float4* q = reinterpret_cast<const float4*>(chunk + k0);
+            const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            r2 += a.x; r2 += a.y; r2 += a.z; r2 += a.w;
+            r2 += b.x; r2 += b.y; r2 += b.z; r2 += b.w;
+            r2 += c.x; r2 += c.y; r2 += c.z; r2 += c.w;
+            r2 += d.x; r2 += d.y; r2 += d.z; r2 += d.w;
+        }
+        if (k0 < n) {
+            if (live) rows[(size_t)((c0 + k0) / CK) * height + y] = r2;
+            for (int k = k0; k < n; ++k) r2 += chunk[k];
+        }
     }
 }
 
+// Vignetted copy of the three planes. A workgroup takes 4096 columns of one
+// row: each thread first runs the radius recurrence over one 16-column
+// segment from its checkpoint (no walk) into LDS, then the workgroup streams
+// the pixels through with coalesced loads and stores.
+constexpr int VIG_COLS = 4096;
+constexpr int VIG_PITCH = CK + 1;  // pads the per-segment rows against bank conflicts
 __global__ __launch_bounds__(256) void opt_vignette(const float* __restrict__ src,
                                                     float* __restrict__ dst, Vig v, int width,
                                                     int height, DeviceStats* stats) {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= width) return;
-    const size_t plane = (size_t)width * height;
-    const size_t i = (size_t)y * width + x;
+    __shared__ float mult[(VIG_COLS / CK) * VIG_PITCH];
+    const uint3 b = xcd_block();
+    const int y = b.y;
+    const int x0 = b.x * VIG_COLS;
+    const int t = threadIdx.x;
     bool bad = false;
-    const float m = vig_multiplier(v, vig_radius(v, x, y, height), bad);
-    for (int c = 0; c < 3; ++c) {
-        const float value = src[c * plane + i] * m;
-        bad = bad || !optics::finite32(value);
-        dst[c * plane + i] = value;
+    {
+        const int seg_x = x0 + t * CK;
+        if (seg_x < width) {
+            float r2 = v.rows[(size_t)(seg_x / CK) * height + y];
+            const int n = min(CK, width - seg_x);
+            for (int j = 0; j < n; ++j) {
+                mult[t * VIG_PITCH + j] = vig_multiplier(v, r2, bad);
+                r2 += v.steps[seg_x + j];
+            }
+        }
+    }
+    __syncthreads();
+    const size_t plane = (size_t)width * height;
+    const int n = min(VIG_COLS, width - x0);
+    for (int i = t; i < n; i += 256) {
+        const float m = mult[(i / CK) * VIG_PITCH + i % CK];
+        const size_t o = (size_t)y * width + x0 + i;
+        for (int c = 0; c < 3; ++c) {
+            const float value = src[c * plane + o] * m;
+            bad = bad || !optics::finite32(value);
+            dst[c * plane + o] = value;
+        }
     }
     if (bad) flag(stats, BAD_VALUE);
 }
@@ -115,6 +158,23 @@ struct ReduceArgs {
     DeviceStats* stats;
 };
 
+// Catmull-Rom weights of the four taps x0 .. x0+3 around x = x0 + 1 + f,
+// 0 <= f < 1. Each tap's distance lies in a fixed branch of cubic_weight
+// (|d| in [1,2), [0,1), (0,1], (1,2]); at the shared end points both
+// polynomials give +0, so the values equal cubic_weight's exactly.
+__device__ __forceinline__ float weight_inner(float a) {
+    return 1.5f * a * a * a - 2.5f * a * a + 1.0f;
+}
+__device__ __forceinline__ float weight_outer(float a) {
+    return -0.5f * a * a * a + 2.5f * a * a - 4.0f * a + 2.0f;
+}
+__device__ __forceinline__ void tap_weights(float x, int x0, float w[4]) {
+    w[0] = weight_outer(x - (float)x0);
+    w[1] = weight_inner(x - (float)(x0 + 1));
+    w[2] = weight_inner(fabsf(x - (float)(x0 + 2)));
+    w[3] = weight_outer(fabsf(x - (float)(x0 + 3)));
+}
+
 // Cubic sample of one plane at (x, y) (crates/optics/src/resample.rs:3-42).
 __device__ __forceinline__ float cubic(const float* __restrict__ p, int width, int height, float x,
                                        float y, unsigned& bad) {
@@ -124,16 +184,15 @@ __device__ __forceinline__ float cubic(const float* __restrict__ p, int width, i
         return 0.0f;
     }
     const int x0 = (int)fx - 1, y0 = (int)fy - 1;
-    float wx[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) wx[k] = optics::cubic_weight(x - (float)(x0 + k));
+    float wx[4], wy[4];
+    tap_weights(x, x0, wx);
+    tap_weights(y, y0, wy);
     float value = 0.0f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const float wy = optics::cubic_weight(y - (float)(y0 + r));
         const float* row = p + (size_t)(y0 + r) * width + x0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) value += row[k] * (wy * wx[k]);
+        for (int k = 0; k < 4; ++k) value += row[k] * (wy[r] * wx[k]);
     }
     if (!optics::finite32(value)) {
         bad |= BAD_VALUE;
@@ -272,15 +331,17 @@ hipError_t launch_optics(const OpticsLaunch& L, hipStream_t s, Profiler* prof) {
     const size_t plane = (size_t)W * H;
     Vig vig{L.vig_steps, L.vig_rows, L.vig_k[0], L.vig_k[1], L.vig_k[2]};
     const int mode = L.remap ? 2 : (L.vignetting ? 1 : 0);
-    if (L.vignetting) {
+    if (L.vignetting && !L.rows_ready) {
         RHD_TIMED(prof, s, "opt_vig_rows", (uint64_t)optics_checkpoints(W) * H * 4,
                   opt_vig_rows<<<(H + 63) / 64, 64, 0, s>>>(L.vig_steps, L.vig_start_x,
                                                             L.vig_start_y, L.vig_ns, W, H,
                                                             L.vig_rows));
+    }
+    if (L.vignetting) {
         if (L.remap) {
             RHD_TIMED(prof, s, "opt_vignette", plane * 24,
-                      opt_vignette<<<dim3((W + 255) / 256, H), 256, 0, s>>>(L.src, L.vignetted,
-                                                                             vig, W, H, L.stats));
+                      opt_vignette<<<dim3((W + VIG_COLS - 1) / VIG_COLS, H), 256, 0, s>>>(
+                          L.src, L.vignetted, vig, W, H, L.stats));
         }
     }
     ReduceArgs a{};

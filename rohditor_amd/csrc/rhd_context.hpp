@@ -37,6 +37,13 @@ struct Lane {
     hipEvent_t ev[5] = {};
     bool profiling = false;
     Profiler profiler;
+    // Plan geometry whose vignetting checkpoints sit at the start of `optics`
+    // (they depend on nothing else); cleared whenever that buffer changes.
+    struct RowsKey {
+        uint32_t width = 0, height = 0;
+        float ns = 0.0f, start_x = 0.0f, start_y = 0.0f;
+        void* buffer = nullptr;
+    } optics_rows;
 };
 
 struct PooledBlock {

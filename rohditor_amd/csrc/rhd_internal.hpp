@@ -149,6 +149,7 @@ struct OpticsLaunch {
     float* vignetted;            // 3 planes; needed when vignetting precedes a remap
     int width, height;
     bool vignetting, remap, direct;
+    bool rows_ready;             // vig_rows already holds this plan's checkpoints
     const float* vig_steps;      // width floats
     float* vig_rows;             // optics_checkpoints(width) * height floats
     float vig_start_x, vig_start_y, vig_ns;

@@ -121,7 +121,7 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // buffer. Tables come first (one host-to-device copy), scratch after.
 struct Layout {
     size_t steps, in_x, in_y, h_first, h_offset, h_count, h_weights, v_first, v_offset,
-        v_count, v_weights, tables_end, rows, mid, vignetted, total;
+        v_count, v_weights, tables_end, rows, tables_begin, mid, vignetted, total;
 };
 
 }  // namespace
@@ -199,6 +199,10 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
         at = align256(at + bytes);
         return here;
     };
+    // The checkpoints first, at a fixed offset, so a later call with the same
+    // geometry finds them in place.
+    lay.rows = place(vignetting ? optics_checkpoints((int)W) * H * 4 : 0);
+    lay.tables_begin = at;
     lay.steps = place(steps.size() * 4);
     lay.in_x = place(in_x.size() * 4);
     lay.in_y = place(in_y.size() * 4);
@@ -211,14 +215,13 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
     lay.v_count = place(va.count.size() * 4);
     lay.v_weights = place(va.weights.size() * 4);
     lay.tables_end = at;
-    lay.rows = place(vignetting ? optics_checkpoints((int)W) * H * 4 : 0);
     lay.mid = place(reduce ? (size_t)target_width * H * 12 : 0);
     lay.vignetted = place(vignetting && remap ? (size_t)W * H * 12 : 0);
     lay.total = at;
 
-    std::vector<uint8_t> host(lay.tables_end);
+    std::vector<uint8_t> host(lay.tables_end - lay.tables_begin);
     auto put = [&](size_t off, const void* p, size_t bytes) {
-        if (bytes) std::memcpy(host.data() + off, p, bytes);
+        if (bytes) std::memcpy(host.data() + (off - lay.tables_begin), p, bytes);
     };
     put(lay.steps, steps.data(), steps.size() * 4);
     put(lay.in_x, in_x.data(), in_x.size() * 4);
@@ -233,8 +236,15 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
     put(lay.v_weights, va.weights.data(), va.weights.size() * 4);
 
     (void)hipSetDevice(ctx->device);
+    void* previous = L.optics.ptr;
     if ((st = ensure_scratch(ctx, L, L.optics, std::max<size_t>(lay.total, 256))) != RHD_OK)
         return st;
+    // Without vignetting the tables start at offset 0, over the checkpoints.
+    if (L.optics.ptr != previous || !vignetting) L.optics_rows = Lane::RowsKey{};
+    const bool rows_ready = vignetting && L.optics_rows.buffer == L.optics.ptr &&
+                            L.optics_rows.width == W && L.optics_rows.height == H &&
+                            L.optics_rows.ns == ns && L.optics_rows.start_x == start_x &&
+                            L.optics_rows.start_y == start_y;
     rhd_source* result = nullptr;
     if ((st = new_source(ctx, &L, target_width, target_height, &result)) != RHD_OK) return st;
     uint8_t* base = (uint8_t*)L.optics.ptr;
@@ -248,8 +258,9 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
     };
     if ((st = reset_stats(ctx, L)) != RHD_OK) return abandon(st);
     hipError_t e = hipEventRecord(L.ev[3], L.stream);
-    if (e == hipSuccess && lay.tables_end)
-        e = hipMemcpyAsync(base, host.data(), lay.tables_end, hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess && !host.empty())
+        e = hipMemcpyAsync(base + lay.tables_begin, host.data(), host.size(),
+                           hipMemcpyHostToDevice, L.stream);
     if (e != hipSuccess) return abandon(hip_fail(ctx, e, "optics tables"));
 
     if (!reduce && !vignetting && !remap) {
@@ -264,6 +275,7 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
         a.vignetting = vignetting;
         a.remap = remap;
         a.direct = !reduce;
+        a.rows_ready = rows_ready;
         a.vig_steps = fptr(lay.steps);
         a.vig_rows = fptr(lay.rows);
         a.vig_start_x = start_x;
@@ -293,7 +305,11 @@ rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
         e = launch_optics(a, L.stream, PROF(L));
     }
     if (e == hipSuccess) e = hipEventRecord(L.ev[4], L.stream);
-    if (e != hipSuccess) return abandon(hip_fail(ctx, e, "optics launch"));
+    if (e != hipSuccess) {
+        L.optics_rows = Lane::RowsKey{};
+        return abandon(hip_fail(ctx, e, "optics launch"));
+    }
+    if (vignetting) L.optics_rows = Lane::RowsKey{W, H, ns, start_x, start_y, L.optics.ptr};
     e = hipMemcpyAsync(L.h_stats, L.d_stats, sizeof(DeviceStats), hipMemcpyDeviceToHost,
                        L.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
