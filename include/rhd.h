@@ -347,6 +347,15 @@ rhd_status rhd_server_drop_mosaic(rhd_server* server, uint64_t frame_id);
 /* Queues a develop of a resident frame; the descriptions are copied. */
 rhd_status rhd_server_submit(rhd_server* server, uint64_t frame_id, const rhd_sensor_desc* desc,
                              const rhd_detail_desc* detail, uint64_t* request_id);
+/* The same, followed on the request's lane by the spatial completion
+ * (rhd_reduce_preview: optics, NULL = off, then area reduction to
+ * target_width x target_height), as the worker develops and then reduces a
+ * preview (crates/engine/src/preview_cache/spatial.rs:475-611); the result
+ * is the reduced source. */
+rhd_status rhd_server_submit_spatial(rhd_server* server, uint64_t frame_id,
+                                     const rhd_sensor_desc* desc, const rhd_detail_desc* detail,
+                                     const rhd_optics_desc* optics, uint32_t target_width,
+                                     uint32_t target_height, uint64_t* request_id);
 /* Waits up to `timeout_ms` (< 0: forever) for a request. Returns
  * Synchronization if it is still pending (it stays collectable); otherwise
  * its status, and on success hands over the developed source (or releases

@@ -142,6 +142,9 @@ SIGNATURES = {
     "rhd_server_drop_mosaic": (c_int, [_P, c_uint64]),
     "rhd_server_submit": (c_int, [_P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc),
                                   POINTER(c_uint64)]),
+    "rhd_server_submit_spatial": (c_int, [_P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc),
+                                          POINTER(OpticsDesc), c_uint32, c_uint32,
+                                          POINTER(c_uint64)]),
     "rhd_server_wait": (c_int, [_P, c_uint64, c_int32, POINTER(_P), POINTER(RequestInfo)]),
     "rhd_server_read_stats": (c_int, [_P, POINTER(ServerStats)]),
     "rhd_server_reset_stats": (c_int, [_P]),

@@ -66,6 +66,10 @@ struct Request {
     rhd_source* out = nullptr;
     int lane = -1;
     bool base_hit = false;
+    // Spatial completion after the stage (optics + area reduction).
+    bool spatial = false, has_optics = false;
+    rhd_optics_desc optics{};
+    uint32_t target_width = 0, target_height = 0;
     rhd_metrics metrics{};
     Clock::time_point submitted, started, finished;
 };
@@ -243,6 +247,15 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
         }
         if (fresh) rhd_source_release(fresh);
     }
+    if (r->status == RHD_OK && r->spatial) {
+        rhd_source* small = nullptr;
+        float ms = 0.0f;
+        r->status = spatial_on(ctx, L, r->out, r->has_optics ? &r->optics : nullptr,
+                               r->target_width, r->target_height, false, &r->cancel, &small, &ms);
+        rhd_source_release(r->out);
+        r->out = small;
+        r->metrics.total_ms += ms;
+    }
     std::lock_guard<std::mutex> g(s->mu);
     if (base) base->users -= 1;
     if (mosaic) {
@@ -290,6 +303,87 @@ void worker(rhd_server* s, int lane_index) {
         }
         run_request(s, *L, r);
     }
+}
+
+}  // namespace
+
+namespace {
+
+rhd_status submit(rhd_server* s, uint64_t frame_id, const rhd_sensor_desc* d,
+                  const rhd_detail_desc* dd, bool spatial, const rhd_optics_desc* optics,
+                  uint32_t target_width, uint32_t target_height, uint64_t* request_id) {
+    if (!s || !d || !request_id) return RHD_INVALID_INPUT;
+    if (spatial) {
+        if (target_width == 0 || target_height == 0 || target_width > d->crop_width ||
+            target_height > d->crop_height)
+            return fail(s->ctx, RHD_INVALID_DIMENSIONS,
+                        "area reduction target must be non-zero and no larger than the crop");
+        if (optics && (optics->width != d->crop_width || optics->height != d->crop_height))
+            return fail(s->ctx, RHD_BASE_MISMATCH, "optics plan dimensions differ from the crop");
+    }
+    rhd_status st = validate_detail(s->ctx, dd);
+    if (st != RHD_OK) return st;
+    if (!rhd_supports(d, dd))
+        return fail(s->ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
+    if (!d->black_levels || !d->white_levels || d->black_repeat_width == 0 ||
+        d->black_repeat_height == 0 || d->white_count == 0)
+        return fail(s->ctx, RHD_INVALID_INPUT, "missing level patterns");
+    Request* r = new Request();
+    r->frame = frame_id;
+    r->desc = *d;
+    r->black.assign(d->black_levels, d->black_levels + d->black_repeat_width * d->black_repeat_height);
+    r->white.assign(d->white_levels, d->white_levels + d->white_count);
+    r->desc.black_levels = r->black.data();
+    r->desc.white_levels = r->white.data();
+    if (dd) {
+        r->detail = *dd;
+        r->has_detail = true;
+    }
+    r->base_key = recipe_key(frame_id, r->desc, r->black, r->white);
+    r->footprint = footprint_of(r->desc, dd);
+    if (spatial) {
+        r->spatial = true;
+        r->has_optics = optics != nullptr;
+        if (optics) r->optics = *optics;
+        r->target_width = target_width;
+        r->target_height = target_height;
+        // Vignetted copy, radius checkpoints, horizontal results, output.
+        const uint64_t px = (uint64_t)d->crop_width * d->crop_height;
+        r->footprint += px * 12 + px / 4 +
+                        (uint64_t)target_width * (d->crop_height + target_height) * 12;
+    }
+    r->megapixels = (double)d->crop_width * d->crop_height / 1e6;
+    r->submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        r->id = s->next_id++;
+        s->submitted += 1;
+        if (!s->have_span) {
+            s->have_span = true;
+            s->span_start = r->submitted;
+            s->span_end = r->submitted;
+        }
+        // Newest wins (scheduler.rs:43-87): the frame's queued requests are
+        // coalesced into this one and its running request is cancelled.
+        for (auto it = s->queue.begin(); it != s->queue.end();) {
+            if ((*it)->frame == frame_id) {
+                (*it)->status = RHD_CANCELLED;
+                s->coalesced += 1;
+                finish_locked(s, *it);
+                it = s->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        for (auto& kv : s->requests)
+            if (kv.second->frame == frame_id && kv.second->state == State::Running)
+                kv.second->cancel = 1;
+        s->requests[r->id] = r;
+        s->queue.push_back(r);
+        *request_id = r->id;
+    }
+    s->work_cv.notify_all();
+    return RHD_OK;
 }
 
 }  // namespace
@@ -400,59 +494,14 @@ rhd_status rhd_server_drop_mosaic(rhd_server* s, uint64_t frame_id) {
 
 rhd_status rhd_server_submit(rhd_server* s, uint64_t frame_id, const rhd_sensor_desc* d,
                              const rhd_detail_desc* dd, uint64_t* request_id) {
-    if (!s || !d || !request_id) return RHD_INVALID_INPUT;
-    rhd_status st = validate_detail(s->ctx, dd);
-    if (st != RHD_OK) return st;
-    if (!rhd_supports(d, dd))
-        return fail(s->ctx, RHD_UNSUPPORTED_EDITS, "GPU sensor development does not support these edits");
-    if (!d->black_levels || !d->white_levels || d->black_repeat_width == 0 ||
-        d->black_repeat_height == 0 || d->white_count == 0)
-        return fail(s->ctx, RHD_INVALID_INPUT, "missing level patterns");
-    Request* r = new Request();
-    r->frame = frame_id;
-    r->desc = *d;
-    r->black.assign(d->black_levels, d->black_levels + d->black_repeat_width * d->black_repeat_height);
-    r->white.assign(d->white_levels, d->white_levels + d->white_count);
-    r->desc.black_levels = r->black.data();
-    r->desc.white_levels = r->white.data();
-    if (dd) {
-        r->detail = *dd;
-        r->has_detail = true;
-    }
-    r->base_key = recipe_key(frame_id, r->desc, r->black, r->white);
-    r->footprint = footprint_of(r->desc, dd);
-    r->megapixels = (double)d->crop_width * d->crop_height / 1e6;
-    r->submitted = Clock::now();
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        r->id = s->next_id++;
-        s->submitted += 1;
-        if (!s->have_span) {
-            s->have_span = true;
-            s->span_start = r->submitted;
-            s->span_end = r->submitted;
-        }
-        // Newest wins (scheduler.rs:43-87): the frame's queued requests are
-        // coalesced into this one and its running request is cancelled.
-        for (auto it = s->queue.begin(); it != s->queue.end();) {
-            if ((*it)->frame == frame_id) {
-                (*it)->status = RHD_CANCELLED;
-                s->coalesced += 1;
-                finish_locked(s, *it);
-                it = s->queue.erase(it);
-            } else {
-                ++it;
-            }
-        }
-        for (auto& kv : s->requests)
-            if (kv.second->frame == frame_id && kv.second->state == State::Running)
-                kv.second->cancel = 1;
-        s->requests[r->id] = r;
-        s->queue.push_back(r);
-        *request_id = r->id;
-    }
-    s->work_cv.notify_all();
-    return RHD_OK;
+    return submit(s, frame_id, d, dd, false, nullptr, 0, 0, request_id);
+}
+
+rhd_status rhd_server_submit_spatial(rhd_server* s, uint64_t frame_id, const rhd_sensor_desc* d,
+                                     const rhd_detail_desc* dd, const rhd_optics_desc* optics,
+                                     uint32_t target_width, uint32_t target_height,
+                                     uint64_t* request_id) {
+    return submit(s, frame_id, d, dd, true, optics, target_width, target_height, request_id);
 }
 
 rhd_status rhd_server_wait(rhd_server* s, uint64_t request_id, int32_t timeout_ms, rhd_source** out,

@@ -562,12 +562,22 @@ class GpuServer:
     def drop_frame(self, frame_id: int) -> None:
         self.proc._check(self.proc.lib.rhd_server_drop_mosaic(self.ptr, frame_id))
 
-    def submit(self, frame_id: int, desc: SensorDevelopmentDescription, detail=None) -> int:
+    def submit(self, frame_id: int, desc: SensorDevelopmentDescription, detail=None,
+               target: Optional[tuple] = None, optics=None) -> int:
+        """Queues a develop; with `target` = (width, height) the request also
+        runs the spatial completion (`optics`, an OpticsExecution or None,
+        then area reduction) and returns the reduced preview."""
         c = desc.to_c()
         rid = ctypes.c_uint64()
-        self.proc._check(self.proc.lib.rhd_server_submit(
-            self.ptr, frame_id, ctypes.byref(c), ctypes.byref(detail) if detail else None,
-            ctypes.byref(rid)))
+        dp = ctypes.byref(detail) if detail else None
+        if target is None:
+            self.proc._check(self.proc.lib.rhd_server_submit(
+                self.ptr, frame_id, ctypes.byref(c), dp, ctypes.byref(rid)))
+        else:
+            o = optics.to_c() if optics is not None else None
+            self.proc._check(self.proc.lib.rhd_server_submit_spatial(
+                self.ptr, frame_id, ctypes.byref(c), dp, ctypes.byref(o) if o is not None else None,
+                target[0], target[1], ctypes.byref(rid)))
         return rid.value
 
     def wait(self, request_id: int, timeout_ms: int = -1, keep: bool = True) -> ServedResult:

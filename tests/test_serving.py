@@ -204,3 +204,37 @@ def test_admission_limit_runs_one_at_a_time(cases, plain):
         assert s1 >= e0 - 0.5  # a request starts only after the previous one ended
     server.close()
     proc.close()
+
+
+def test_served_spatial_completion_equals_plain(cases, plain):
+    """Develop + optics + area reduction served on lanes equals the plain
+    develop followed by rhd_reduce_preview, alone and in a concurrent mix."""
+    from rohditor_amd import optics as opt
+    proc, server = _server(lanes=3, base_cache_bytes=1 << 30)
+    plans, expected = [], []
+    for i, (frame, desc, detail) in enumerate(cases):
+        server.put_frame(i, frame)
+        w, h = desc.crop.width, desc.crop.height
+        plan = opt.resolve_execution(w, h, camera_crop=1.0, focal=24.0,
+                                     distortion=opt.Poly5(0.02 * (i - 1.5), 0.01),
+                                     tca=opt.TcaLinear(1.0004, 0.9996),
+                                     vignetting=opt.Pa(-0.2, 0.05, 0.0))
+        target = opt.preview_dimensions(w, h, 500)
+        resident = proc.upload(frame)
+        src = proc.develop_resident(resident, desc, detail)
+        small, _ = proc.reduce_preview(src, plan, target)
+        expected.append(small.read_rgb())
+        for hnd in (small, src, resident):
+            hnd.release()
+        plans.append((plan, target))
+    for _ in range(2):  # the second round hits the base cache
+        rids = [server.submit(i, desc, detail, target=plans[i][1], optics=plans[i][0])
+                for i, (_, desc, detail) in enumerate(cases)]
+        for i, rid in enumerate(rids):
+            r, rgb = _collect(server, rid)
+            assert r.ok, r.status
+            np.testing.assert_array_equal(rgb, expected[i])
+            assert rgb.shape[:2] == (plans[i][1][1], plans[i][1][0])
+    assert server.stats().base_hits == len(cases)
+    server.close()
+    proc.close()
