@@ -294,6 +294,18 @@ rhd_status rhd_optics_full(rhd_context* ctx, const rhd_source* source,
                            const rhd_optics_desc* optics, const volatile uint8_t* cancel,
                            rhd_source** out, float* gpu_ms);
 
+/* White balance then the camera matrix to linear Rec.2020, in place on a
+ * resident camera source: apply_white_balance_cancellable and
+ * apply_camera_color_transform_cancellable (crates/core/src/cpu/stages.rs:
+ * 250-308), run after optics (crates/core/src/pipeline/orchestration.rs:
+ * 1464-1470). `gains` are the resolved WhiteBalanceGains (finite, positive);
+ * `camera_to_rec2020` is CameraColorTransform::camera_to_linear_rec2020,
+ * row-major. A DCP profile look (ProfileLook) is not applied: a caller whose
+ * transform has one routes the step to the CPU. */
+rhd_status rhd_color_transform(rhd_context* ctx, rhd_source* source, const float gains[3],
+                               const float camera_to_rec2020[9], const volatile uint8_t* cancel,
+                               float* gpu_ms);
+
 /* ---------------------------------------------------------------------------
  * Serving loop: the counterpart of the engine worker's develop loop
  * (crates/engine/src/coordinator/worker.rs:723-800) with its newest-wins

@@ -248,5 +248,14 @@ def preview_dimensions(width: int, height: int, max_long_edge: int) -> tuple[int
     return st, int(tw.value), int(th.value)
 
 
+def color(rgb: np.ndarray, gains, matrix) -> tuple[int, np.ndarray]:
+    """White balance then the camera matrix (crates/core/src/cpu/stages.rs:250-308)."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    g = np.asarray(gains, dtype=np.float32)
+    m = np.ascontiguousarray(matrix, dtype=np.float32).reshape(9)
+    st = lib().rhdo_color(_p(v), _sz(v.size // 3), _p(g), _p(m))
+    return st, v
+
+
 def cpu_threads() -> int:
     return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)

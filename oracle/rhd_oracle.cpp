@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "capture.hpp"
+#include "color.hpp"
 #include "demosaic.hpp"
 #include "opposed.hpp"
 #include "optics.hpp"
@@ -224,6 +225,11 @@ long rhdo_area_axis(size_t source, size_t target, uint64_t* first, uint64_t* off
 
 int rhdo_resize_area(const float* rgb, size_t w, size_t h, float* out, size_t tw, size_t th) {
     return optics::resize_area(rgb, w, h, out, tw, th);
+}
+
+// rgb: n interleaved pixels, in place; m: row-major camera -> Rec.2020.
+int rhdo_color(float* rgb, size_t n, const float gains[3], const float m[9]) {
+    return color::apply(rgb, n, gains, m);
 }
 
 int rhdo_preview_dimensions(size_t w, size_t h, size_t max_long, size_t* tw, size_t* th) {

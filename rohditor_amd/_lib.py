@@ -136,6 +136,8 @@ SIGNATURES = {
     "rhd_reduce_preview": (c_int, [_P, _P, POINTER(OpticsDesc), c_uint32, c_uint32, _P,
                                    POINTER(_P), POINTER(c_float)]),
     "rhd_optics_full": (c_int, [_P, _P, POINTER(OpticsDesc), _P, POINTER(_P), POINTER(c_float)]),
+    "rhd_color_transform": (c_int, [_P, _P, POINTER(c_float), POINTER(c_float), _P,
+                                    POINTER(c_float)]),
     "rhd_server_create": (c_int, [_P, POINTER(ServerConfig), POINTER(_P)]),
     "rhd_server_destroy": (None, [_P]),
     "rhd_server_put_mosaic": (c_int, [_P, c_uint64, _P, c_uint64, c_uint32, c_uint32]),

@@ -168,6 +168,9 @@ struct OpticsLaunch {
     DeviceStats* stats;
 };
 size_t optics_checkpoints(int width);
+// White balance then the camera matrix, in place on 3 planes of n pixels.
+hipError_t launch_color(float* rgb, size_t n, const float gains[3], const float m[9],
+                        hipStream_t s, Profiler* prof);
 hipError_t launch_optics(const OpticsLaunch& a, hipStream_t s, Profiler* prof);
 
 }  // namespace rhd

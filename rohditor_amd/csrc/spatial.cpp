@@ -378,6 +378,31 @@ rhd_status rhd_reduce_preview(rhd_context* ctx, const rhd_source* source,
                       cancel, out, gpu_ms);
 }
 
+rhd_status rhd_color_transform(rhd_context* ctx, rhd_source* source, const float gains[3],
+                               const float camera_to_rec2020[9], const volatile uint8_t* cancel,
+                               float* gpu_ms) {
+    if (!ctx || !source || !gains || !camera_to_rec2020) return RHD_INVALID_INPUT;
+    if (gpu_ms) *gpu_ms = 0.0f;
+    rhd_status st = check_handle_ctx(ctx, source->ctx);
+    if (st != RHD_OK) return st;
+    for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(gains[c]) || !(gains[c] > 0.0f))
+            return fail(ctx, RHD_INVALID_INPUT, "white-balance gains must be finite and positive");
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(camera_to_rec2020[k]))
+            return fail(ctx, RHD_INVALID_INPUT, "camera matrix is not finite");
+    if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
+    Lane& L = ctx->lanes[0];
+    (void)hipSetDevice(ctx->device);
+    RHD_HIP(ctx, hipEventRecord(L.ev[3], L.stream));
+    RHD_HIP(ctx, launch_color(source->d, (size_t)source->width * source->height, gains,
+                              camera_to_rec2020, L.stream, PROF(L)));
+    RHD_HIP(ctx, hipEventRecord(L.ev[4], L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
+    if (gpu_ms) *gpu_ms = elapsed(L.ev[3], L.ev[4]);
+    return RHD_OK;
+}
+
 rhd_status rhd_optics_full(rhd_context* ctx, const rhd_source* source,
                            const rhd_optics_desc* optics, const volatile uint8_t* cancel,
                            rhd_source** out, float* gpu_ms) {

@@ -473,6 +473,18 @@ class GpuSensorProcessor:
             cancellation.ptr if cancellation else None, ctypes.byref(ptr), ctypes.byref(ms)))
         return GpuDevelopedSource(self, ptr), float(ms.value)
 
+    def color_transform(self, source: GpuDevelopedSource, gains, camera_to_rec2020,
+                        cancellation: Optional[CancellationToken] = None) -> float:
+        """White balance then the camera matrix to linear Rec.2020, in place
+        (crates/core/src/cpu/stages.rs:250-308). Returns the GPU milliseconds."""
+        g = (ctypes.c_float * 3)(*[float(v) for v in gains])
+        m = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(camera_to_rec2020).reshape(9)])
+        ms = ctypes.c_float()
+        self._check(self.lib.rhd_color_transform(self.ctx, source.ptr, g, m,
+                                                 cancellation.ptr if cancellation else None,
+                                                 ctypes.byref(ms)))
+        return float(ms.value)
+
     def develop_resident(self, mosaic: ResidentMosaic, desc: SensorDevelopmentDescription,
                          detail=None, cancellation: Optional[CancellationToken] = None,
                          metrics: Optional["_lib.Metrics"] = None) -> GpuDevelopedSource:
