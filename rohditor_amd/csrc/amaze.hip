@@ -306,96 +306,82 @@ __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
 
 // green.rs:184-226, horizontal half: hcd is bounded in place along each row,
 // and cddiffsq is formed at red/blue sites from the final vcd and hcd. One
-// thread per tile row; 32-column chunks are staged through LDS so global
-// reads and writes stay coalesced, with each plane's chunk loaded in one
-// batch of independent loads.
-constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_W = BH_CHUNK + 4, BH_PITCH = BH_W + 1;
-constexpr int BH_PER_LANE = BH_ROWS * BH_W / 64;  // 36
-
-__device__ __forceinline__ void bh_stage(float* lds, const float* __restrict__ plane, int r0,
-                                         int c0, int rows, bool zero_outside) {
-    float v[BH_PER_LANE];
-#pragma unroll
-    for (int j = 0; j < BH_PER_LANE; ++j) {
-        const int e = threadIdx.x + 64 * j;
-        const int r = e / BH_W, c = e % BH_W;
-        const int col = c0 - 2 + c;
-        const bool inside = zero_outside ? (col >= 4 && col < TS - 4) : (col < TS);
-        v[j] = (r < rows && inside) ? plane[(r0 + r) * TS + col] : 0.0f;
-    }
-#pragma unroll
-    for (int j = 0; j < BH_PER_LANE; ++j) {
-        const int e = threadIdx.x + 64 * j;
-        const int r = e / BH_W, c = e % BH_W;
-        lds[r * BH_PITCH + c] = v[j];
-    }
-}
+// thread per tile row walks it in 32-column chunks: each chunk's hcd, hcdalt
+// and CFA columns (with the 4-column halo on the left, 8 on the right) and
+// its vcd arrive as aligned float4 loads straight into registers, the serial
+// recurrence runs fully unrolled over them, and hcd and cddiffsq leave as
+// float4 stores. cddiffsq is written as 0 at green sites, which nothing
+// reads (the Nyquist stencil touches red/blue sites only).
+constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_IN = BH_CHUNK + 8;
 
 __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
-    const int r0 = 4 + blk.x * BH_ROWS;
-    const int lane = threadIdx.x;
-    const int rr = r0 + lane;
-    __shared__ float s_hcd[BH_ROWS * BH_PITCH];
-    __shared__ float s_alt[BH_ROWS * BH_PITCH];
-    __shared__ float s_cfa[BH_ROWS * BH_PITCH];
-    __shared__ float s_out[BH_ROWS * BH_PITCH];
-    float* hcd = t.full(HCD);
-    const float* alt = t.full(HALT);
-    const float* cfa = t.full(CFA);
-    const float* vcd = t.full(VCD);
-    float* cds = t.full(CDS);
-    const int rows = min(BH_ROWS, TS - 4 - r0);
+    const int rr = 4 + blk.x * BH_ROWS + (int)threadIdx.x;
+    if (rr >= TS - 4) return;
+    float* __restrict__ hcd_row = t.full(HCD) + rr * TS;
+    const float* __restrict__ alt_row = t.full(HALT) + rr * TS;
+    const float* __restrict__ cfa_row = t.full(CFA) + rr * TS;
+    const float* __restrict__ vcd_row = t.full(VCD) + rr * TS;
+    float* __restrict__ cds_row = t.full(CDS) + rr * TS;
+    const bool green4 = (fc(a.pattern, rr, 4) & 1) == 1;  // colour at column 4 (and c0)
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded hcd two and one columns left (cols 2, 3 are zero)
-    bool green = rr < TS - 4 ? (fc(a.pattern, rr, 4) & 1) == 1 : false;
     for (int c0 = 4; c0 < TS - 4; c0 += BH_CHUNK) {
-        // Columns c0 - 2 .. c0 + 33 of hcd, hcdalt and the CFA for `rows` rows.
-        bh_stage(s_hcd, hcd, r0, c0, rows, true);
-        bh_stage(s_alt, alt, r0, c0, rows, true);
-        bh_stage(s_cfa, cfa, r0, c0, rows, false);
-        __syncthreads();
-        if (lane < rows) {
-            const int base = lane * BH_PITCH + 2;  // LDS column of c0
-            for (int k = 0; k < BH_CHUNK; ++k) {
-                const int cc = c0 + k;
-                if (cc >= TS - 4) break;
-                const int j = base + k;
-                float hcdvar = spread(b_m2, s_hcd[j], s_hcd[j + 2]);
-                float hcdaltvar = spread(s_alt[j - 2], s_alt[j], s_alt[j + 2]);
-                float h = hcdaltvar < hcdvar ? s_alt[j] : s_hcd[j];
-                float cur = s_cfa[j];
-                float bounded = green ? bound_at_green(h, cur, s_cfa[j - 1], s_cfa[j + 1], clip_pt)
-                                      : bound_at_red_blue(h, cur, s_cfa[j - 1], s_cfa[j + 1], clip_pt);
-                s_out[lane * BH_PITCH + k] = bounded;
-                b_m2 = b_m1;
-                b_m1 = bounded;
-                green = !green;
+        // Register k holds column c0 - 4 + k. hcd and hcdalt read as zero
+        // outside [4, TS - 4), the CFA as zero beyond the tile.
+        float h[BH_IN], al[BH_IN], cf[BH_IN], vc[BH_CHUNK];
+#pragma unroll
+        for (int q = 0; q < BH_IN / 4; ++q) {
+            const int col = c0 - 4 + 4 * q;
+            float4 hv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), av = hv, cv = hv;
+            if (col < TS) {
+                hv = *reinterpret_cast<const float4*>(hcd_row + col);
+                av = *reinterpret_cast<const float4*>(alt_row + col);
+                cv = *reinterpret_cast<const float4*>(cfa_row + col);
+            }
+            const float hs[4] = {hv.x, hv.y, hv.z, hv.w}, as[4] = {av.x, av.y, av.z, av.w};
+            const float cs[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool inside = col + e >= 4 && col + e < TS - 4;
+                h[4 * q + e] = inside ? hs[e] : 0.0f;
+                al[4 * q + e] = inside ? as[e] : 0.0f;
+                cf[4 * q + e] = col + e < TS ? cs[e] : 0.0f;
             }
         }
-        __syncthreads();
-        // Write the chunk back coalesced; cddiffsq at red/blue sites.
-        constexpr int OUT_PER_LANE = BH_ROWS * BH_CHUNK / 64;
-        float vv[OUT_PER_LANE];
 #pragma unroll
-        for (int j = 0; j < OUT_PER_LANE; ++j) {
-            const int e = lane + 64 * j;
-            const int r = e / BH_CHUNK, c = e % BH_CHUNK;
-            const int col = c0 + c, row = r0 + r;
-            const bool ok = r < rows && col < TS - 4;
-            vv[j] = ok && (fc(a.pattern, row, col) & 1) == 0 ? vcd[row * TS + col] : 0.0f;
+        for (int q = 0; q < BH_CHUNK / 4; ++q) {
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (c0 + 4 * q < TS - 4) v = *reinterpret_cast<const float4*>(vcd_row + c0 + 4 * q);
+            vc[4 * q] = v.x;
+            vc[4 * q + 1] = v.y;
+            vc[4 * q + 2] = v.z;
+            vc[4 * q + 3] = v.w;
         }
+        float out[BH_CHUNK], cd[BH_CHUNK];
 #pragma unroll
-        for (int j = 0; j < OUT_PER_LANE; ++j) {
-            const int e = lane + 64 * j;
-            const int r = e / BH_CHUNK, c = e % BH_CHUNK;
-            const int col = c0 + c, row = r0 + r;
-            if (r >= rows || col >= TS - 4) continue;
-            const float h = s_out[r * BH_PITCH + c];
-            hcd[row * TS + col] = h;
-            if ((fc(a.pattern, row, col) & 1) == 0) cds[row * TS + col] = sqr(vv[j] - h);
+        for (int k = 0; k < BH_CHUNK; ++k) {
+            const bool green = green4 != ((k & 1) != 0);
+            const float hcdvar = spread(b_m2, h[k + 4], h[k + 6]);
+            const float hcdaltvar = spread(al[k + 2], al[k + 4], al[k + 6]);
+            const float hv = hcdaltvar < hcdvar ? al[k + 4] : h[k + 4];
+            const float bounded = green ? bound_at_green(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt)
+                                        : bound_at_red_blue(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt);
+            out[k] = bounded;
+            cd[k] = green ? 0.0f : sqr(vc[k] - bounded);
+            b_m2 = b_m1;
+            b_m1 = bounded;
         }
-        __syncthreads();
+        // The last chunk holds 24 columns (TS - 8 = 248 = 7 * 32 + 24).
+#pragma unroll
+        for (int q = 0; q < BH_CHUNK / 4; ++q) {
+            if (c0 + 4 * q < TS - 4) {
+                *reinterpret_cast<float4*>(hcd_row + c0 + 4 * q) =
+                    make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+                *reinterpret_cast<float4*>(cds_row + c0 + 4 * q) =
+                    make_float4(cd[4 * q], cd[4 * q + 1], cd[4 * q + 2], cd[4 * q + 3]);
+            }
+        }
     }
 }
 
@@ -908,7 +894,7 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
                                  a.clip_point));
     RHD_TIMED(prof, s, "amz_bound_h", 4 * f + f + f / 2,
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
-                                 dim3(64), 0, s, a, a.clip_point));
+                                 dim3(BH_ROWS), 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_weights", 8 * f / 2 + 5 * h + 4 * h,
               hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
