@@ -31,7 +31,11 @@ constexpr int FULL = TS * TS, HALF = TS * TSH;
 enum FullPlane { CFA, DW0, DW1, DHV, VCD, HCD, VALT, HALT, DGV, DGH, CDS, NUM_FULL };
 enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, DELP, DELM, SQ1M, SQ1P, PMWT, RBM, RBP, RBINT,
                  NUM_HALF };
-constexpr size_t TILE_FLOATS = (size_t)NUM_FULL * FULL + (size_t)NUM_HALF * HALF + 2 * HALF / 4;
+// Per-row partial Nyquist boxes: one {min row, max row, min col, max col}
+// slot per (row, half-row) of the tile.
+constexpr int BOX_SLOTS = TS * 2;
+constexpr size_t TILE_FLOATS =
+    (size_t)NUM_FULL * FULL + (size_t)NUM_HALF * HALF + 2 * HALF / 4 + 4 * BOX_SLOTS;
 
 constexpr float EPS = 1.0e-5f, EPSSQ = 1.0e-10f, ARTHRESH = 0.75f;
 // math.rs:28-53; GAUSSGRAD carries the exact factor NYQTHRESH = 0.5.
@@ -54,6 +58,9 @@ struct Tile {
         return reinterpret_cast<uint8_t*>(base + (size_t)NUM_FULL * FULL + (size_t)NUM_HALF * HALF);
     }
     __device__ __forceinline__ uint8_t* nyq2() const { return nyq() + HALF; }
+    __device__ __forceinline__ int* boxes() const {
+        return reinterpret_cast<int*>(nyq() + 2 * HALF);
+    }
 };
 
 __device__ __forceinline__ Tile tile_of(const AmazeArgs& a, int z) {
@@ -98,13 +105,6 @@ __global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
         t.full(HCD)[i] = 0.0f;
         t.full(VALT)[i] = 0.0f;
         t.full(HALT)[i] = 0.0f;
-    }
-    if (rr == 0 && cc == 0) {
-        int* area = a.areas + 4 * (size_t)blk.z;
-        area[0] = INT_MAX;
-        area[1] = -1;
-        area[2] = INT_MAX;
-        area[3] = -1;
     }
 }
 
@@ -419,54 +419,72 @@ __device__ __forceinline__ float bound_towards(float estimate, float centre, flo
     return value > clip ? median3(value, a, b) : value;
 }
 
-__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
+// The three parts read disjoint planes, so they run as three kernels over
+// the red/blue sites: HV weights, the Nyquist test with the tile's bounding
+// box, and the diagonal estimates.
+struct Site {
+    int rr, cc, i;
+};
+__device__ __forceinline__ Site site_of(const AmazeArgs& a, const uint3& blk) {
     const int p = blk.x * 64 + (threadIdx.x & 63);
     const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int cc = 2 * p + gae(a.pattern, rr);
-    const int i = rr * TS + cc;
-    bool flagged = false;
-    if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
-        const float* vcd = t.full(VCD);
-        const float* hcd = t.full(HCD);
-        const float* dgv = t.full(DGV);
-        const float* dgh = t.full(DGH);
-        const float* w0 = t.full(DW0);
-        const float* w1 = t.full(DW1);
-        float v[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
-        float uave = v[3] + v[2] + v[1] + v[0];
-        float dave = v[3] + v[4] + v[5] + v[6];
-        float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
-        float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
-        float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
-        float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
-        float varl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) +
-                     sqr(hcd[i - 3] - lave);
-        float varr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) +
-                     sqr(hcd[i + 3] - rave);
-        float hwt = w1[i - 1] / (w1[i - 1] + w1[i + 1]);
-        float vwt = w0[i - TS] / (w0[i + TS] + w0[i - TS]);
-        float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
-        float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
-        float fluct_u = dgv[i] + dgv[i - TS] + dgv[i - 2 * TS];
-        float fluct_d = dgv[i] + dgv[i + TS] + dgv[i + 2 * TS];
-        float fluct_l = dgh[i] + dgh[i - 1] + dgh[i - 2];
-        float fluct_r = dgh[i] + dgh[i + 1] + dgh[i + 2];
-        float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
-        float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
-        float varwt = hcdvar / (vcdvar + hcdvar);
-        float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
-        t.half(HVWT)[i >> 1] =
-            ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
-                ? varwt
-                : diffwt;
+    return Site{rr, cc, rr * TS + cc};
+}
 
-        // Nyquist test statistic (nyquist.rs:28-57) and bounding box.
-        const float* c = t.full(CDS) + i;
-        const float* d = t.full(DHV) + i;
+__global__ __launch_bounds__(256) void amz_hvwt(AmazeArgs a) {
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const Site s = site_of(a, blk);
+    if (!(s.rr >= 6 && s.rr < TS - 6 && s.cc >= 6 && s.cc < TS - 6)) return;
+    const int i = s.i;
+    const float* __restrict__ vcd = t.full(VCD);
+    const float* __restrict__ hcd = t.full(HCD);
+    const float* __restrict__ dgv = t.full(DGV);
+    const float* __restrict__ dgh = t.full(DGH);
+    const float* __restrict__ w0 = t.full(DW0);
+    const float* __restrict__ w1 = t.full(DW1);
+    float v[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
+    float uave = v[3] + v[2] + v[1] + v[0];
+    float dave = v[3] + v[4] + v[5] + v[6];
+    float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
+    float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
+    float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
+    float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
+    float varl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) +
+                 sqr(hcd[i - 3] - lave);
+    float varr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) +
+                 sqr(hcd[i + 3] - rave);
+    float hwt = w1[i - 1] / (w1[i - 1] + w1[i + 1]);
+    float vwt = w0[i - TS] / (w0[i + TS] + w0[i - TS]);
+    float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
+    float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
+    float fluct_u = dgv[i] + dgv[i - TS] + dgv[i - 2 * TS];
+    float fluct_d = dgv[i] + dgv[i + TS] + dgv[i + 2 * TS];
+    float fluct_l = dgh[i] + dgh[i - 1] + dgh[i - 2];
+    float fluct_r = dgh[i] + dgh[i + 1] + dgh[i + 2];
+    float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
+    float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
+    float varwt = hcdvar / (vcdvar + hcdvar);
+    float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
+    t.half(HVWT)[i >> 1] =
+        ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
+            ? varwt
+            : diffwt;
+}
+
+// Nyquist test statistic (nyquist.rs:28-57) and the tile's bounding box of
+// flagged sites (:59-89).
+__global__ __launch_bounds__(256) void amz_nyquist(AmazeArgs a) {
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const Site s = site_of(a, blk);
+    bool flagged = false;
+    if (s.rr >= 6 && s.rr < TS - 6 && s.cc >= 6 && s.cc < TS - 6) {
+        const float* __restrict__ c = t.full(CDS) + s.i;
+        const float* __restrict__ d = t.full(DHV) + s.i;
         float test =
             (GAUSSODD[0] * c[0] +
              GAUSSODD[1] * (c[-TS - 1] + c[-TS + 1] + c[TS - 1] + c[TS + 1]) +
@@ -479,56 +497,90 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
                              d[TS - 2] + d[TS + 2] + d[2 * TS - 1] + d[2 * TS + 1]) +
              GAUSSGRAD[5] * (d[-2 * TS - 2] + d[-2 * TS + 2] + d[2 * TS - 2] + d[2 * TS + 2]));
         if (test > 0.0f) {
-            t.nyq()[i >> 1] = 1;
+            t.nyq()[s.i >> 1] = 1;
             flagged = true;
         }
     }
-    // A wavefront is one row of the tile, with cc increasing by lane, so its
-    // flagged sites span [cc of the first flagged lane, cc of the last]; one
-    // lane updates the tile's box. The atomics return nothing, so the wave
-    // never waits on them (a load here would also wait for its stores).
+    // A wavefront is one half row of the tile, with cc increasing by lane, so
+    // its flagged sites span [cc of the first flagged lane, cc of the last].
+    // Every wavefront writes its slot (empty: INT_MAX, -1, INT_MAX, -1);
+    // amz_box reduces the slots, so no two waves touch the same address.
     const unsigned long long m = __ballot(flagged);
+    int4 box = make_int4(INT_MAX, -1, INT_MAX, -1);
     if (m) {
         const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
-        const int c_lo = __shfl(cc, first, 64), c_hi = __shfl(cc, last, 64);
-        if ((threadIdx.x & 63) == 0) {
-            int* area = a.areas + 4 * (size_t)blk.z;
-            atomicMin(area + 0, rr);
-            atomicMax(area + 1, rr);
-            atomicMin(area + 2, c_lo);
-            atomicMax(area + 3, c_hi);
-        }
+        box = make_int4(s.rr, s.rr, __shfl(s.cc, first, 64), __shfl(s.cc, last, 64));
     }
-    if (rr >= 8 && rr < TS - 8 && cc >= 8 && cc < TS - 8) {
-        const float* cfa = t.full(CFA);
-        const int i1 = cc >> 1;
-        float centre = cfa[i];
-        float se1 = cfa[i + TS + 1], se2 = cfa[i + 2 * TS + 2];
-        float nw1 = cfa[i - TS - 1], nw2 = cfa[i - 2 * TS - 2];
-        float ne1 = cfa[i - TS + 1], ne2 = cfa[i - 2 * TS + 2];
-        float sw1 = cfa[i + TS - 1], sw2 = cfa[i + 2 * TS - 2];
-        float rbse = diagonal_estimate(centre, se1, se2);
-        float rbnw = diagonal_estimate(centre, nw1, nw2);
-        float rbne = diagonal_estimate(centre, ne1, ne2);
-        float rbsw = diagonal_estimate(centre, sw1, sw2);
-        const float* delm = t.half(DELM);
-        const float* delp = t.half(DELP);
-        const int h0 = rr * TSH;
-        float wtse = EPS + delm[h0 + i1] + delm[h0 + TSH + ((cc + 1) >> 1)] +
-                     delm[h0 + 2 * TSH + ((cc + 2) >> 1)];
-        float wtnw = EPS + delm[h0 + i1] + delm[h0 - TSH + ((cc - 1) >> 1)] +
-                     delm[h0 - 2 * TSH + ((cc - 2) >> 1)];
-        float wtne = EPS + delp[h0 + i1] + delp[h0 - TSH + ((cc + 1) >> 1)] +
-                     delp[h0 - 2 * TSH + ((cc + 2) >> 1)];
-        float wtsw = EPS + delp[h0 + i1] + delp[h0 + TSH + ((cc - 1) >> 1)] +
-                     delp[h0 + 2 * TSH + ((cc - 2) >> 1)];
-        float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
-        float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
-        float rbvarm = EPSSQ + even_kernel(t.half(SQ1M), rr, cc);
-        t.half(PMWT)[h0 + i1] = rbvarm / ((EPSSQ + even_kernel(t.half(SQ1P), rr, cc)) + rbvarm);
-        t.half(RBP)[h0 + i1] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
-        t.half(RBM)[h0 + i1] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
+    if ((threadIdx.x & 63) == 0)
+        reinterpret_cast<int4*>(t.boxes())[s.rr * 2 + blk.x] = box;
+}
+
+// Reduces a tile's slots into its Nyquist box (min/max over flagged sites).
+__global__ __launch_bounds__(256) void amz_box(AmazeArgs a) {
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.x);
+    const int4* slots = reinterpret_cast<const int4*>(t.boxes());
+    int4 b = make_int4(INT_MAX, -1, INT_MAX, -1);
+    for (int k = threadIdx.x; k < BOX_SLOTS; k += 256) {
+        const int4 v = slots[k];
+        b = make_int4(min(b.x, v.x), max(b.y, v.y), min(b.z, v.z), max(b.w, v.w));
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        b.x = min(b.x, __shfl_xor(b.x, off, 64));
+        b.y = max(b.y, __shfl_xor(b.y, off, 64));
+        b.z = min(b.z, __shfl_xor(b.z, off, 64));
+        b.w = max(b.w, __shfl_xor(b.w, off, 64));
+    }
+    __shared__ int4 part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int4 r = part[0];
+        for (int w = 1; w < 4; ++w)
+            r = make_int4(min(r.x, part[w].x), max(r.y, part[w].y), min(r.z, part[w].z),
+                          max(r.w, part[w].w));
+        int* area = a.areas + 4 * (size_t)blk.x;
+        area[0] = r.x;
+        area[1] = r.y;
+        area[2] = r.z;
+        area[3] = r.w;
+    }
+}
+
+// Diagonal estimates (diagonal.rs:98-144).
+__global__ __launch_bounds__(256) void amz_diagonal(AmazeArgs a, float clip_pt) {
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.z);
+    const Site s = site_of(a, blk);
+    const int rr = s.rr, cc = s.cc, i = s.i;
+    if (!(rr >= 8 && rr < TS - 8 && cc >= 8 && cc < TS - 8)) return;
+    const float* __restrict__ cfa = t.full(CFA);
+    const int h0 = rr * TSH, i1 = cc >> 1;
+    float centre = cfa[i];
+    float se1 = cfa[i + TS + 1], se2 = cfa[i + 2 * TS + 2];
+    float nw1 = cfa[i - TS - 1], nw2 = cfa[i - 2 * TS - 2];
+    float ne1 = cfa[i - TS + 1], ne2 = cfa[i - 2 * TS + 2];
+    float sw1 = cfa[i + TS - 1], sw2 = cfa[i + 2 * TS - 2];
+    float rbse = diagonal_estimate(centre, se1, se2);
+    float rbnw = diagonal_estimate(centre, nw1, nw2);
+    float rbne = diagonal_estimate(centre, ne1, ne2);
+    float rbsw = diagonal_estimate(centre, sw1, sw2);
+    const float* __restrict__ delm = t.half(DELM);
+    const float* __restrict__ delp = t.half(DELP);
+    float wtse = EPS + delm[h0 + i1] + delm[h0 + TSH + ((cc + 1) >> 1)] +
+                 delm[h0 + 2 * TSH + ((cc + 2) >> 1)];
+    float wtnw = EPS + delm[h0 + i1] + delm[h0 - TSH + ((cc - 1) >> 1)] +
+                 delm[h0 - 2 * TSH + ((cc - 2) >> 1)];
+    float wtne = EPS + delp[h0 + i1] + delp[h0 - TSH + ((cc + 1) >> 1)] +
+                 delp[h0 - 2 * TSH + ((cc + 2) >> 1)];
+    float wtsw = EPS + delp[h0 + i1] + delp[h0 + TSH + ((cc - 1) >> 1)] +
+                 delp[h0 + 2 * TSH + ((cc - 2) >> 1)];
+    float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
+    float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
+    float rbvarm = EPSSQ + even_kernel(t.half(SQ1M), rr, cc);
+    t.half(PMWT)[h0 + i1] = rbvarm / ((EPSSQ + even_kernel(t.half(SQ1P), rr, cc)) + rbvarm);
+    t.half(RBP)[h0 + i1] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
+    t.half(RBM)[h0 + i1] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
 }
 
 // Nyquist area of a tile (nyquist.rs:59-89); false when there is none.
@@ -895,8 +947,14 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
     RHD_TIMED(prof, s, "amz_bound_h", 4 * f + f + f / 2,
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
                                  dim3(BH_ROWS), 0, s, a, a.clip_point));
-    RHD_TIMED(prof, s, "amz_weights", 8 * f / 2 + 5 * h + 4 * h,
-              hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point));
+    RHD_TIMED(prof, s, "amz_hvwt", 6 * f / 2 + h / 2,
+              hipLaunchKernelGGL(amz_hvwt, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_nyquist", 2 * f / 2 + h / 8,
+              hipLaunchKernelGGL(amz_nyquist, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_box", 16ull * BOX_SLOTS * tiles,
+              hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_diagonal", f / 2 + 4 * h + 3 * h,
+              hipLaunchKernelGGL(amz_diagonal, pair_grid, blk, 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_chains", 3 * f / 2 + 6 * h + 6 * h,
