@@ -624,42 +624,55 @@ __global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
     t.nyq2()[i >> 1] = votes >= 5 ? 1 : (votes == 4 ? n[i >> 1] : 0);
 }
 
-// nyquist.rs:120-152: area interpolation of hvwt.
+// nyquist.rs:120-152: area interpolation of hvwt at the sites the vote
+// kept (about a tenth of them). A workgroup first lists its kept sites in
+// LDS, then its lanes take them densely, so the 7x7 gathers run in full
+// wavefronts; each site's result is independent of the order.
 __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
     const uint3 blk = xcd_block();
     Area ar;
     if (!tile_area(a, blk.z, ar)) return;
     const Tile t = tile_of(a, blk.z);
-    const int p = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int cc = 2 * p + gae(a.pattern, rr);
-    if (!in_area(ar, a.pattern, rr, cc)) return;
-    const int i = rr * TS + cc;
-    const uint8_t* n2 = t.nyq2();
-    if (n2[i >> 1] == 0) return;
-    const float* cfa = t.full(CFA);
-    float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
-    for (int di = -6; di < 7; di += 2) {
-        int i1 = i + di * TS - 6;
-        for (int dj = -6; dj < 7; dj += 2) {
-            if (n2[i1 >> 1] != 0) {
-                float c = cfa[i1];
-                sumcfa += c;
-                sumh += cfa[i1 - 1] + cfa[i1 + 1];
-                sumv += cfa[i1 - TS] + cfa[i1 + TS];
-                sumsqh += sqr(c - cfa[i1 - 1]) + sqr(c - cfa[i1 + 1]);
-                sumsqv += sqr(c - cfa[i1 - TS]) + sqr(c - cfa[i1 + TS]);
-                areawt += 1.0f;
-            }
-            i1 += 2;
-        }
+    __shared__ int s_sites[256];
+    __shared__ int s_count;
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    {
+        const int p = blk.x * 64 + (threadIdx.x & 63);
+        const int rr = blk.y * 4 + (threadIdx.x >> 6);
+        const int cc = 2 * p + gae(a.pattern, rr);
+        if (in_area(ar, a.pattern, rr, cc) && t.nyq2()[(rr * TS + cc) >> 1] != 0)
+            s_sites[atomicAdd(&s_count, 1)] = rr * TS + cc;
     }
-    sumh = sumcfa - 0.5f * sumh;
-    sumv = sumcfa - 0.5f * sumv;
-    areawt *= 0.5f;
-    float hcdvar = EPSSQ + fabsf(areawt * sumsqh - sumh * sumh);
-    float vcdvar = EPSSQ + fabsf(areawt * sumsqv - sumv * sumv);
-    t.half(HVWT)[i >> 1] = hcdvar / (vcdvar + hcdvar);
+    __syncthreads();
+    const int n = s_count;
+    const uint8_t* n2 = t.nyq2();
+    const float* cfa = t.full(CFA);
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const int i = s_sites[k];
+        float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
+        for (int di = -6; di < 7; di += 2) {
+            int i1 = i + di * TS - 6;
+            for (int dj = -6; dj < 7; dj += 2) {
+                if (n2[i1 >> 1] != 0) {
+                    float c = cfa[i1];
+                    sumcfa += c;
+                    sumh += cfa[i1 - 1] + cfa[i1 + 1];
+                    sumv += cfa[i1 - TS] + cfa[i1 + TS];
+                    sumsqh += sqr(c - cfa[i1 - 1]) + sqr(c - cfa[i1 + 1]);
+                    sumsqv += sqr(c - cfa[i1 - TS]) + sqr(c - cfa[i1 + TS]);
+                    areawt += 1.0f;
+                }
+                i1 += 2;
+            }
+        }
+        sumh = sumcfa - 0.5f * sumh;
+        sumv = sumcfa - 0.5f * sumv;
+        areawt *= 0.5f;
+        float hcdvar = EPSSQ + fabsf(areawt * sumsqh - sumh * sumh);
+        float vcdvar = EPSSQ + fabsf(areawt * sumsqv - sumv * sumv);
+        t.half(HVWT)[i >> 1] = hcdvar / (vcdvar + hcdvar);
+    }
 }
 
 // ---------------------------------------------------------------- row chains
