@@ -432,8 +432,8 @@ __device__ __forceinline__ Site site_of(const AmazeArgs& a, const uint3& blk) {
     return Site{rr, cc, rr * TS + cc};
 }
 
-__global__ __launch_bounds__(256) void amz_hvwt(AmazeArgs a) {
-    const uint3 blk = xcd_block();
+__device__ __forceinline__ void part_hvwt(const AmazeArgs& a, const uint3& blk) {
+    
     const Tile t = tile_of(a, blk.z);
     const Site s = site_of(a, blk);
     if (!(s.rr >= 6 && s.rr < TS - 6 && s.cc >= 6 && s.cc < TS - 6)) return;
@@ -477,8 +477,8 @@ __global__ __launch_bounds__(256) void amz_hvwt(AmazeArgs a) {
 
 // Nyquist test statistic (nyquist.rs:28-57) and the tile's bounding box of
 // flagged sites (:59-89).
-__global__ __launch_bounds__(256) void amz_nyquist(AmazeArgs a) {
-    const uint3 blk = xcd_block();
+__device__ __forceinline__ void part_nyquist(const AmazeArgs& a, const uint3& blk) {
+    
     const Tile t = tile_of(a, blk.z);
     const Site s = site_of(a, blk);
     bool flagged = false;
@@ -548,8 +548,8 @@ __global__ __launch_bounds__(256) void amz_box(AmazeArgs a) {
 }
 
 // Diagonal estimates (diagonal.rs:98-144).
-__global__ __launch_bounds__(256) void amz_diagonal(AmazeArgs a, float clip_pt) {
-    const uint3 blk = xcd_block();
+__device__ __forceinline__ void part_diagonal(const AmazeArgs& a, const uint3& blk, float clip_pt) {
+    
     const Tile t = tile_of(a, blk.z);
     const Site s = site_of(a, blk);
     const int rr = s.rr, cc = s.cc, i = s.i;
@@ -581,6 +581,13 @@ __global__ __launch_bounds__(256) void amz_diagonal(AmazeArgs a, float clip_pt) 
     t.half(PMWT)[h0 + i1] = rbvarm / ((EPSSQ + even_kernel(t.half(SQ1P), rr, cc)) + rbvarm);
     t.half(RBP)[h0 + i1] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
     t.half(RBM)[h0 + i1] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
+}
+
+__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
+    const uint3 blk = xcd_block();
+    part_hvwt(a, blk);
+    part_nyquist(a, blk);
+    part_diagonal(a, blk, clip_pt);
 }
 
 // Nyquist area of a tile (nyquist.rs:59-89); false when there is none.
@@ -960,14 +967,10 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
     RHD_TIMED(prof, s, "amz_bound_h", 4 * f + f + f / 2,
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
                                  dim3(BH_ROWS), 0, s, a, a.clip_point));
-    RHD_TIMED(prof, s, "amz_hvwt", 6 * f / 2 + h / 2,
-              hipLaunchKernelGGL(amz_hvwt, pair_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_nyquist", 2 * f / 2 + h / 8,
-              hipLaunchKernelGGL(amz_nyquist, pair_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_weights", 9 * f / 2 + 7 * h + h / 2 + h / 8,
+              hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_box", 16ull * BOX_SLOTS * tiles,
               hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_diagonal", f / 2 + 4 * h + 3 * h,
-              hipLaunchKernelGGL(amz_diagonal, pair_grid, blk, 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_chains", 3 * f / 2 + 6 * h + 6 * h,
