@@ -91,21 +91,24 @@ typedef struct rhd_sensor_desc {
 } rhd_sensor_desc;
 
 /*
- * Detail contract with capture sharpening only: crates/core/src/
- * sharpening.rs:19-69 (CaptureSharpeningContract) and crates/edit/src/
- * sharpening.rs:22-81 (CaptureSharpening). `iterations` is 8 in the
- * reference (sharpening.rs:13); other counts are an extension. False colour
- * and noise reduction are not built: a non-zero flag is UnsupportedEdits.
+ * Detail contract (crates/core/src/pipeline/detail_contract.rs:15-142):
+ * capture sharpening (crates/core/src/sharpening.rs:19-69,
+ * CaptureSharpeningContract; crates/edit/src/sharpening.rs:22-81) and false
+ * colour suppression (crates/edit/src/false_color.rs, 0..=5 median steps).
+ * `iterations` is 8 in the reference (sharpening.rs:13); other counts are an
+ * extension. Noise reduction is not built: a non-zero flag is
+ * UnsupportedEdits (routing).
  */
 typedef struct rhd_detail_desc {
     int32_t capture_enabled;
     float amount;           /* [0, 1] */
     float sigma;            /* "radius": Gaussian sigma in [0.3, 1.2] */
     float noise_protection; /* [0, 1] */
-    float ceilings[3];      /* crates/core/src/pipeline/detail.rs:323-337 */
+    float ceilings[3];      /* DetailContract::ceilings (detail.rs:323-337) */
     int32_t iterations;
-    int32_t false_colour;
+    int32_t false_colour;   /* FalseColorSuppression::steps, 0 = off */
     int32_t noise_reduction;
+    float balance[3];       /* DetailContract::balance: the as-shot demosaic balance */
 } rhd_detail_desc;
 
 /* crates/highlight/src/lib.rs:253-258 (ClipStats). */

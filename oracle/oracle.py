@@ -257,5 +257,15 @@ def color(rgb: np.ndarray, gains, matrix) -> tuple[int, np.ndarray]:
     return st, v
 
 
+def false_color(rgb: np.ndarray, steps: int, balance, ceilings) -> tuple[int, np.ndarray]:
+    """crates/detail/src/false_color.rs:45-135 on (H, W, 3) RGB; returns a copy."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    b = np.asarray(balance, dtype=np.float32)
+    c = np.asarray(ceilings, dtype=np.float32)
+    st = lib().rhdo_false_color(_p(v), _sz(v.shape[1]), _sz(v.shape[0]), ctypes.c_int(steps),
+                                _p(b), _p(c))
+    return st, v
+
+
 def cpu_threads() -> int:
     return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)

@@ -9,6 +9,7 @@
 #include "capture.hpp"
 #include "color.hpp"
 #include "demosaic.hpp"
+#include "detail.hpp"
 #include "opposed.hpp"
 #include "optics.hpp"
 #include "rcd.hpp"
@@ -230,6 +231,12 @@ int rhdo_resize_area(const float* rgb, size_t w, size_t h, float* out, size_t tw
 // rgb: n interleaved pixels, in place; m: row-major camera -> Rec.2020.
 int rhdo_color(float* rgb, size_t n, const float gains[3], const float m[9]) {
     return color::apply(rgb, n, gains, m);
+}
+
+// False-colour suppression in place on tight interleaved RGB.
+int rhdo_false_color(float* rgb, size_t width, size_t height, int steps, const float balance[3],
+                     const float ceilings[3]) {
+    return detail::false_color(rgb, width, height, steps, balance, ceilings);
 }
 
 int rhdo_preview_dimensions(size_t w, size_t h, size_t max_long, size_t* tw, size_t* th) {

@@ -36,7 +36,7 @@ class DetailDesc(ctypes.Structure):
     _fields_ = [
         ("capture_enabled", c_int32), ("amount", c_float), ("sigma", c_float),
         ("noise_protection", c_float), ("ceilings", c_float * 3), ("iterations", c_int32),
-        ("false_colour", c_int32), ("noise_reduction", c_int32),
+        ("false_colour", c_int32), ("noise_reduction", c_int32), ("balance", c_float * 3),
     ]
 
 

@@ -335,6 +335,10 @@ bool capture_active(const rhd_detail_desc* dd) {
     return dd && dd->capture_enabled && dd->amount > 0.0f;
 }
 
+bool detail_active(const rhd_detail_desc* dd) {
+    return capture_active(dd) || (dd && dd->false_colour > 0);
+}
+
 // crates/edit/src/sharpening.rs:50-81 and crates/core/src/sharpening.rs:26-42.
 rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd) {
     if (!dd) return RHD_OK;
@@ -342,6 +346,15 @@ rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd) {
     if (!in(dd->amount, 0.0f, 1.0f) || !in(dd->sigma, 0.3f, 1.2f) ||
         !in(dd->noise_protection, 0.0f, 1.0f))
         return fail(ctx, RHD_INVALID_INPUT, "capture sharpening settings are out of range");
+    if (dd->false_colour < 0 || dd->false_colour > 5)
+        return fail(ctx, RHD_INVALID_INPUT, "false-colour steps must be in 0..=5");
+    if (dd->false_colour > 0) {
+        if (!finite_positive3(dd->balance))
+            return fail(ctx, RHD_INVALID_INPUT, "channel balance must be finite and positive");
+        for (int c = 0; c < 3; ++c)
+            if (std::isnan(dd->ceilings[c]))
+                return fail(ctx, RHD_INVALID_INPUT, "detail ceilings must not be NaN");
+    }
     if (capture_active(dd)) {
         if (!finite_positive3(dd->ceilings))
             return fail(ctx, RHD_INVALID_INPUT, "capture ceilings must be finite and positive");
@@ -618,6 +631,23 @@ rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_det
     return RHD_OK;
 }
 
+// The detail stage's steps in the reference's order (crates/core/src/
+// pipeline/detail.rs:210-261): false colour, then capture sharpening. Noise
+// reduction is routed to the CPU before this is reached.
+rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd) {
+    if (!detail_active(dd)) return RHD_OK;
+    const int w = (int)src->width, h = (int)src->height;
+    rhd_status st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
+    if (st != RHD_OK) return st;
+    if (dd->false_colour > 0) {
+        hipError_t e = launch_false_colour(src->d, w, h, dd->false_colour, dd->balance,
+                                           dd->ceilings, (float*)L.capture.ptr, L.stream, PROF(L));
+        if (e != hipSuccess) return hip_fail(ctx, e, "false colour");
+    }
+    if (capture_active(dd)) return run_capture(ctx, L, src, dd);
+    return RHD_OK;
+}
+
 rhd_status check_handle_ctx(rhd_context* ctx, const void* handle_ctx) {
     if (!ctx) return RHD_INVALID_INPUT;
     if (handle_ctx != ctx) return fail(ctx, RHD_INVALID_INPUT, "handle belongs to another context");
@@ -663,7 +693,7 @@ rhd_status stage_scratch(rhd_context* ctx, Lane& L, const rhd_sensor_desc* d,
             (uint64_t)((w + 173) / 174) * ((h + 173) / 174), MAX_TILES_PER_LAUNCH);
         st = ensure_scratch(ctx, L, L.rcd, tiles * rcd_scratch_bytes_per_tile());
     }
-    if (st == RHD_OK && capture_active(dd))
+    if (st == RHD_OK && detail_active(dd))
         st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
     return st;
 }
@@ -695,7 +725,7 @@ rhd_status enqueue_develop(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
     if (d->highlight == RHD_HIGHLIGHT_OPPOSED && (st = run_opposed(ctx, L, plane, d)) != RHD_OK)
         return st;
     if ((st = run_demosaic(ctx, L, plane, d, src)) != RHD_OK) return st;
-    if (capture_active(dd) && (st = run_capture(ctx, L, src, dd)) != RHD_OK) return st;
+    if (detail_active(dd) && (st = run_detail(ctx, L, src, dd)) != RHD_OK) return st;
     RHD_HIP(ctx, hipMemcpyAsync(stats_out, L.d_stats, sizeof(DeviceStats), hipMemcpyDeviceToHost,
                                 L.stream));
     return RHD_OK;
@@ -765,9 +795,9 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
             if (e != hipSuccess) st = hip_fail(ctx, e, "base copy");
         }
     }
-    if (st == RHD_OK && capture_active(dd)) {
+    if (st == RHD_OK && detail_active(dd)) {
         if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_capture(ctx, L, src, dd);
+        if (st == RHD_OK) st = run_detail(ctx, L, src, dd);
     }
     step(hipEventRecord(L.ev[4], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK) st = finish_stats(ctx, L, m, d->highlight);
@@ -870,7 +900,7 @@ int rhd_supports(const rhd_sensor_desc* d, const rhd_detail_desc* dd) {
     bool highlight = d->highlight == RHD_HIGHLIGHT_OFF || d->highlight == RHD_HIGHLIGHT_CLIP ||
                      d->highlight == RHD_HIGHLIGHT_OPPOSED;
     bool demosaic = d->demosaic >= RHD_DEMOSAIC_BILINEAR && d->demosaic <= RHD_DEMOSAIC_AMAZE;
-    bool detail = !dd || (!dd->false_colour && !dd->noise_reduction);
+    bool detail = !dd || (!dd->noise_reduction && dd->false_colour >= 0 && dd->false_colour <= 5);
     bool capture = !capture_active(dd) || std::ceil(3.0f * dd->sigma) <= 4.0f;
     return highlight && demosaic && detail && capture ? 1 : 0;
 }
@@ -1002,9 +1032,9 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
     if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src);
     if (st == RHD_OK && hipEventRecord(L.ev[3], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK && capture_active(dd)) {
+    if (st == RHD_OK && detail_active(dd)) {
         if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_capture(ctx, L, src, dd);
+        if (st == RHD_OK) st = run_detail(ctx, L, src, dd);
     }
     if (st == RHD_OK && hipEventRecord(L.ev[4], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
@@ -1032,14 +1062,14 @@ rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
     rhd_status st = check_handle_ctx(ctx, src->ctx);
     if (st != RHD_OK) return st;
     if ((st = validate_detail(ctx, dd)) != RHD_OK) return st;
-    if (!capture_active(dd)) return RHD_OK;
+    if (!detail_active(dd)) return RHD_OK;
     if (std::ceil(3.0f * dd->sigma) > 4.0f)
         return fail(ctx, RHD_UNSUPPORTED_EDITS, "capture radius above 4 taps");
     if (cancelled(cancel)) return fail(ctx, RHD_CANCELLED, "cancelled");
     (void)hipSetDevice(ctx->device);
     if ((st = reset_stats(ctx, L)) != RHD_OK) return st;
     RHD_HIP(ctx, hipEventRecord(L.ev[3], L.stream));
-    if ((st = run_capture(ctx, L, src, dd)) != RHD_OK) {
+    if ((st = run_detail(ctx, L, src, dd)) != RHD_OK) {
         (void)hipStreamSynchronize(L.stream);
         return st;
     }

@@ -52,6 +52,7 @@ double cost_of(const rhd_sensor_desc& d, const rhd_detail_desc& dd) {
                                                       : 8.0;
     if (d.highlight == RHD_HIGHLIGHT_OPPOSED) per_px += 60.0;
     if (capture_active(&dd)) per_px += 20.0 + 6.0 * dd.iterations;
+    per_px += 4.0 * dd.false_colour;
     return px * per_px;
 }
 
@@ -60,7 +61,8 @@ std::string signature(const std::vector<Job>& jobs) {
     for (const Job& j : jobs) {
         s += std::to_string(j.desc.crop_width) + "x" + std::to_string(j.desc.crop_height) + ":" +
              std::to_string(j.desc.highlight) + std::to_string(j.desc.demosaic) + ":" +
-             std::to_string(capture_active(&j.detail) ? j.detail.iterations : -1) + ";";
+             std::to_string(capture_active(&j.detail) ? j.detail.iterations : -1) + ":" +
+             std::to_string(j.detail.false_colour) + ";";
     }
     return s;
 }

@@ -116,6 +116,10 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out);
 rhd_status device_alloc(rhd_context* ctx, Lane* L, uint64_t bytes, void** out);
 void device_free(rhd_context* ctx, void* p, uint64_t bytes);
 bool capture_active(const rhd_detail_desc* dd);
+// Any detail-stage step to run (false colour or capture sharpening).
+bool detail_active(const rhd_detail_desc* dd);
+// False colour then capture sharpening on `src` (in place).
+rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
 rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd);
 rhd_status new_source(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, rhd_source** out);
 rhd_status reset_stats(rhd_context* ctx, Lane& L);

@@ -126,6 +126,10 @@ struct CaptureArgs {
     DeviceStats* stats;
 };
 size_t capture_scratch_bytes(int width, int height);
+// False-colour suppression in place on planar RGB; `scratch` holds 4 planes.
+hipError_t launch_false_colour(float* rgb, int width, int height, int steps,
+                               const float balance[3], const float ceilings[3], float* scratch,
+                               hipStream_t s, Profiler* prof);
 hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof);
 
 struct OpposedArgs {

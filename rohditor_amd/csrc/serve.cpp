@@ -114,7 +114,7 @@ uint64_t footprint_of(const rhd_sensor_desc& d, const rhd_detail_desc* dd) {
     if (d.demosaic == RHD_DEMOSAIC_AMAZE) bytes += tiles_of(w, h, 224) * amaze_scratch_bytes_per_tile();
     if (d.demosaic == RHD_DEMOSAIC_RCD) bytes += tiles_of(w, h, 174) * rcd_scratch_bytes_per_tile();
     if (d.highlight == RHD_HIGHLIGHT_OPPOSED) bytes += opposed_scratch_bytes((int)w, (int)h);
-    if (capture_active(dd)) bytes += capture_scratch_bytes((int)w, (int)h);
+    if (detail_active(dd)) bytes += capture_scratch_bytes((int)w, (int)h);
     return bytes;
 }
 
@@ -219,9 +219,9 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
             r->status = e == hipSuccess ? RHD_OK : fail(ctx, RHD_SYNCHRONIZATION, "base copy");
         }
         if (r->status == RHD_OK) r->status = reset_stats(ctx, L);
-        if (r->status == RHD_OK && capture_active(dd)) {
+        if (r->status == RHD_OK && detail_active(dd)) {
             if (cancelled(&r->cancel)) r->status = fail(ctx, RHD_CANCELLED, "cancelled");
-            if (r->status == RHD_OK) r->status = run_capture(ctx, L, out, dd);
+            if (r->status == RHD_OK) r->status = run_detail(ctx, L, out, dd);
         }
         if (r->status == RHD_OK) r->status = finish_stats(ctx, L, nullptr, RHD_HIGHLIGHT_OFF);
         if (r->status != RHD_OK) {

@@ -143,6 +143,8 @@ class EditRecipe:
     white_balance_gains: Optional[Sequence[float]] = None
     # RL iterations; the reference fixes 8.
     capture_iterations: int = CAPTURE_SHARPENING_ITERATIONS
+    # FalseColorSuppression::steps (crates/edit/src/false_color.rs), 0..=5.
+    false_color_steps: int = 0
 
 
 @dataclass
@@ -259,13 +261,19 @@ def capture_ceilings(recipe: EditRecipe, gains: Sequence[float]) -> np.ndarray:
     return np.ones(3, dtype=np.float32)
 
 
-def detail_desc(recipe: EditRecipe, gains: Sequence[float]) -> "_lib.DetailDesc":
+def detail_desc(recipe: EditRecipe, gains: Sequence[float],
+                balance: Optional[Sequence[float]] = None) -> "_lib.DetailDesc":
+    """DetailContract::new (crates/core/src/pipeline/detail_contract.rs:23-40):
+    ceilings from the rendering `gains`, balance = the as-shot demosaic
+    balance (`gains` when not given)."""
     c = recipe.capture
     dd = _lib.DetailDesc()
     dd.capture_enabled = int(c.enabled)
     dd.amount, dd.sigma, dd.noise_protection = c.amount, c.radius, c.noise_protection
     dd.ceilings[:] = [float(v) for v in capture_ceilings(recipe, gains)]
     dd.iterations = recipe.capture_iterations
+    dd.false_colour = int(recipe.false_color_steps)
+    dd.balance[:] = [float(v) for v in (balance if balance is not None else gains)]
     return dd
 
 
@@ -504,7 +512,7 @@ class GpuSensorProcessor:
         desc = SensorDevelopmentDescription.from_frame(frame, recipe, options)
         gains = (recipe.white_balance_gains if recipe.white_balance_gains is not None
                  else desc.demosaic_balance)
-        detail = detail_desc(recipe, gains)
+        detail = detail_desc(recipe, gains, desc.demosaic_balance)
         m = _lib.Metrics()
         ptr = ctypes.c_void_p()
         c = desc.to_c()
