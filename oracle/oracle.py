@@ -269,3 +269,93 @@ def false_color(rgb: np.ndarray, steps: int, balance, ceilings) -> tuple[int, np
 
 def cpu_threads() -> int:
     return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def noise_profiles(rgb: np.ndarray, balance):
+    """crates/detail/src/denoise/mod.rs:60-110 on (H, W, 3) RGB. Returns
+    (status, [(a, b, bins)] for luma, C1, C2); status 1 = too small to profile."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32)
+    b = np.asarray(balance, dtype=np.float32)
+    out = np.zeros(9, np.float64)
+    st = lib().rhdo_noise_profiles(_p(v), _sz(v.shape[1]), _sz(v.shape[0]), _p(b), _p(out))
+    return st, [(float(out[3 * c]), float(out[3 * c + 1]), int(out[3 * c + 2])) for c in range(3)]
+
+
+def estimate_noise(plane: np.ndarray, brightness: np.ndarray):
+    """crates/detail/src/noise.rs:48-125 on one (H, W) plane."""
+    p = np.ascontiguousarray(plane, dtype=np.float32)
+    b = np.ascontiguousarray(brightness, dtype=np.float32)
+    out = np.zeros(3, np.float64)
+    st = lib().rhdo_estimate_noise(_p(p), _p(b), _sz(p.shape[1]), _sz(p.shape[0]), _p(out))
+    return st, (float(out[0]), float(out[1]), int(out[2]))
+
+
+def chroma_denoise(rgb: np.ndarray, strength: float, balance, ceilings, profiles):
+    """crates/detail/src/denoise/chroma.rs:88-148; profiles = [(a, b), ...] for
+    luma, C1, C2. Returns (status, copy)."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    b = np.asarray(balance, dtype=np.float32)
+    c = np.asarray(ceilings, dtype=np.float32)
+    p = np.asarray([x for pr in profiles for x in pr[:2]], dtype=np.float32)
+    st = lib().rhdo_chroma_denoise(_p(v), _sz(v.shape[1]), _sz(v.shape[0]),
+                                   ctypes.c_float(strength), _p(b), _p(c), _p(p))
+    return st, v
+
+
+def luma_denoise(rgb: np.ndarray, strength: float, detail: float, balance, ceilings, profile):
+    """crates/detail/src/denoise/luma.rs:128-176; profile = (a, b)."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    b = np.asarray(balance, dtype=np.float32)
+    c = np.asarray(ceilings, dtype=np.float32)
+    p = np.asarray(profile[:2], dtype=np.float32)
+    st = lib().rhdo_luma_denoise(_p(v), _sz(v.shape[1]), _sz(v.shape[0]),
+                                 ctypes.c_float(strength), ctypes.c_float(detail), _p(b), _p(c),
+                                 _p(p))
+    return st, v
+
+
+def b3_smooth(plane: np.ndarray, step: int) -> np.ndarray:
+    """crates/detail/src/atrous.rs:63-120."""
+    p = np.ascontiguousarray(plane, dtype=np.float32)
+    out = np.empty_like(p)
+    lib().rhdo_b3_smooth(_p(p), _sz(p.shape[1]), _sz(p.shape[0]), _sz(step), _p(out))
+    return out
+
+
+def local_mean_square(band: np.ndarray) -> np.ndarray:
+    """crates/detail/src/denoise/luma.rs:279-321."""
+    p = np.ascontiguousarray(band, dtype=np.float32)
+    out = np.empty_like(p)
+    lib().rhdo_local_mean_square(_p(p), _sz(p.shape[1]), _sz(p.shape[0]), _p(out))
+    return out
+
+
+def mad(values) -> float:
+    """crates/detail/src/noise.rs:127-139."""
+    v = np.ascontiguousarray(values, dtype=np.float32).copy()
+    lib().rhdo_mad.restype = ctypes.c_float
+    return float(lib().rhdo_mad(_p(v), _sz(v.size)))
+
+
+def detail_stage(rgb: np.ndarray, *, false_colour=0, chroma=0.0, luminance=0.0,
+                 luminance_detail=0.5, balance=(1.0, 1.0, 1.0), ceilings=(np.inf,) * 3,
+                 capture=None):
+    """crates/core/src/pipeline/detail.rs:210-307: profiles of the input, false
+    colour, chroma NR, luminance NR, capture sharpening. capture = (amount,
+    sigma, noise_protection, iterations) or None. Returns (status, copy,
+    profiles or None)."""
+    v = np.ascontiguousarray(rgb, dtype=np.float32).copy()
+    b = np.asarray(balance, dtype=np.float32)
+    c = np.asarray(ceilings, dtype=np.float32)
+    prof = np.zeros(9, np.float64)
+    have = ctypes.c_int(0)
+    cap = capture or (0.0, 0.7, 0.0, 0)
+    st = lib().rhdo_detail_stage(
+        _p(v), _sz(v.shape[1]), _sz(v.shape[0]), ctypes.c_int(false_colour),
+        ctypes.c_float(chroma), ctypes.c_float(luminance), ctypes.c_float(luminance_detail),
+        _p(b), _p(c), ctypes.c_int(1 if capture else 0), ctypes.c_float(cap[0]),
+        ctypes.c_float(cap[1]), ctypes.c_float(cap[2]), ctypes.c_int(cap[3]), _p(prof),
+        ctypes.byref(have))
+    profiles = ([(float(prof[3 * k]), float(prof[3 * k + 1]), int(prof[3 * k + 2]))
+                 for k in range(3)] if have.value else None)
+    return st, v, profiles

@@ -9,6 +9,7 @@
 #include "capture.hpp"
 #include "color.hpp"
 #include "demosaic.hpp"
+#include "denoise.hpp"
 #include "detail.hpp"
 #include "opposed.hpp"
 #include "optics.hpp"
@@ -241,6 +242,115 @@ int rhdo_false_color(float* rgb, size_t width, size_t height, int steps, const f
 
 int rhdo_preview_dimensions(size_t w, size_t h, size_t max_long, size_t* tw, size_t* th) {
     return optics::preview_dimensions(w, h, max_long, tw, th);
+}
+
+// Noise profiles {luma, c1, c2} of tight interleaved RGB as {a, b, bins}
+// triples. Returns 0, 1 when the image is too small to profile, 2 for an
+// invalid balance.
+int rhdo_noise_profiles(const float* rgb, size_t width, size_t height, const float balance[3],
+                        double out[9]) {
+    denoise::Profile p[3];
+    const int st = denoise::profiles(rgb, width, height, balance, p);
+    if (st == 0)
+        for (int c = 0; c < 3; ++c) {
+            out[3 * c] = p[c].a;
+            out[3 * c + 1] = p[c].b;
+            out[3 * c + 2] = (double)p[c].bins;
+        }
+    return st;
+}
+
+// One plane's profile with its own brightness plane (noise.rs:48-125).
+int rhdo_estimate_noise(const float* plane, const float* brightness, size_t width, size_t height,
+                        double out[3]) {
+    std::vector<float> p(plane, plane + width * height), b(brightness, brightness + width * height);
+    denoise::Profile r;
+    if (denoise::estimate(p, b, width, height, r)) return 1;
+    out[0] = r.a;
+    out[1] = r.b;
+    out[2] = (double)r.bins;
+    return 0;
+}
+
+static void unpack_profiles(const float prof[6], denoise::Profile p[3]) {
+    for (int c = 0; c < 3; ++c) {
+        p[c].a = prof[2 * c];
+        p[c].b = prof[2 * c + 1];
+    }
+}
+
+// prof = {a, b} of luma, c1, c2.
+int rhdo_chroma_denoise(float* rgb, size_t width, size_t height, float strength,
+                        const float balance[3], const float ceilings[3], const float prof[6]) {
+    denoise::Profile p[3];
+    unpack_profiles(prof, p);
+    return denoise::chroma(rgb, width, height, strength, balance, ceilings, p);
+}
+
+int rhdo_luma_denoise(float* rgb, size_t width, size_t height, float strength, float detail_,
+                      const float balance[3], const float ceilings[3], const float prof[2]) {
+    denoise::Profile p;
+    p.a = prof[0];
+    p.b = prof[1];
+    return denoise::luma(rgb, width, height, strength, detail_, balance, ceilings, p);
+}
+
+// Band-limited helpers for the pinning tests.
+int rhdo_b3_smooth(const float* src, size_t width, size_t height, size_t step, float* out) {
+    std::vector<float> tmp;
+    denoise::smooth(src, width, height, step, tmp, out);
+    return 0;
+}
+
+int rhdo_local_mean_square(const float* band, size_t width, size_t height, float* out) {
+    std::vector<float> b(band, band + width * height), tmp, o;
+    denoise::local_mean_square(b, width, height, tmp, o);
+    std::memcpy(out, o.data(), o.size() * sizeof(float));
+    return 0;
+}
+
+float rhdo_mad(float* values, size_t n) {
+    std::vector<float> v(values, values + n);
+    return denoise::mad(v);
+}
+
+// The detail stage's steps in order (crates/core/src/pipeline/detail.rs:
+// 210-307): noise profiles of the stage input, false colour, chroma NR,
+// luminance NR, capture sharpening. prof_out (may be null) receives the
+// profiles as {a, b, bins} x {luma, c1, c2}; *profiled is 1 when they
+// exist. Returns the first failing step's status.
+int rhdo_detail_stage(float* rgb, size_t width, size_t height, int false_colour, float chroma,
+                      float luminance, float luminance_detail, const float balance[3],
+                      const float ceilings[3], int capture_enabled, float amount, float sigma,
+                      float noise_protection, int iterations, double prof_out[9],
+                      int* profiled) {
+    denoise::Profile p[3];
+    int have = 0;
+    if (chroma > 0.0f || luminance > 0.0f) {
+        const int st = denoise::profiles(rgb, width, height, balance, p);
+        if (st == 2) return 2;
+        have = st == 0;
+    }
+    if (profiled) *profiled = have;
+    if (prof_out && have)
+        for (int c = 0; c < 3; ++c) {
+            prof_out[3 * c] = p[c].a;
+            prof_out[3 * c + 1] = p[c].b;
+            prof_out[3 * c + 2] = (double)p[c].bins;
+        }
+    int st = detail::false_color(rgb, width, height, false_colour, balance, ceilings);
+    if (st) return 10 + st;
+    if (have && chroma > 0.0f && (st = denoise::chroma(rgb, width, height, chroma, balance,
+                                                       ceilings, p)))
+        return 20 + st;
+    if (have && luminance > 0.0f && (st = denoise::luma(rgb, width, height, luminance,
+                                                        luminance_detail, balance, ceilings, p[0])))
+        return 30 + st;
+    if (capture_enabled && amount > 0.0f &&
+        (st = capture::apply<float>(rgb, width, height, amount, sigma, noise_protection, ceilings,
+                                    iterations, nullptr, nullptr, nullptr, 1)))
+        return 40 + st;
+    return 0;
 }
 
 }  // extern "C"
