@@ -93,11 +93,15 @@ typedef struct rhd_sensor_desc {
 /*
  * Detail contract (crates/core/src/pipeline/detail_contract.rs:15-142):
  * capture sharpening (crates/core/src/sharpening.rs:19-69,
- * CaptureSharpeningContract; crates/edit/src/sharpening.rs:22-81) and false
- * colour suppression (crates/edit/src/false_color.rs, 0..=5 median steps).
+ * CaptureSharpeningContract; crates/edit/src/sharpening.rs:22-81), false
+ * colour suppression (crates/edit/src/false_color.rs, 0..=5 median steps)
+ * and noise reduction (crates/edit/src/noise_reduction.rs:27-35: chroma and
+ * luminance strengths, luminance detail, each in [0, 1]). The steps run in
+ * the order of crates/core/src/pipeline/detail.rs:210-307: noise profiles of
+ * the stage input, false colour, chroma NR, luminance NR, capture. Images
+ * with fewer than one 16x16 profile tile skip noise reduction, as there.
  * `iterations` is 8 in the reference (sharpening.rs:13); other counts are an
- * extension. Noise reduction is not built: a non-zero flag is
- * UnsupportedEdits (routing).
+ * extension.
  */
 typedef struct rhd_detail_desc {
     int32_t capture_enabled;
@@ -107,8 +111,10 @@ typedef struct rhd_detail_desc {
     float ceilings[3];      /* DetailContract::ceilings (detail.rs:323-337) */
     int32_t iterations;
     int32_t false_colour;   /* FalseColorSuppression::steps, 0 = off */
-    int32_t noise_reduction;
     float balance[3];       /* DetailContract::balance: the as-shot demosaic balance */
+    float nr_chroma;        /* NoiseReduction::chroma, 0 = off */
+    float nr_luminance;     /* NoiseReduction::luminance, 0 = off */
+    float nr_luminance_detail; /* NoiseReduction::luminance_detail (0.5 neutral) */
 } rhd_detail_desc;
 
 /* crates/highlight/src/lib.rs:253-258 (ClipStats). */
@@ -183,6 +189,30 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
  * (crates/gpu/src/capture/mod.rs:54), in place on `source`. */
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* source, const rhd_detail_desc* detail,
                        const volatile uint8_t* cancel, rhd_metrics* metrics);
+
+/* NoiseProfile (crates/detail/src/noise.rs:24-34): variance a*max(x,0) + b;
+ * bins = 0 for the flat fallback. */
+typedef struct rhd_noise_profile {
+    float a, b;
+    uint32_t bins;
+} rhd_noise_profile;
+/* estimate_noise_profiles (crates/detail/src/denoise/mod.rs:60-110; the GPU
+ * executor's estimate_noise_from_resident, crates/gpu/src/capture/noise.rs:
+ * 35-129): profiles {luma, C1, C2} of a resident source for `balance`,
+ * computed on the device. *profiled = 0 (and `out` untouched) when the image
+ * holds no 16x16 tile in its sampled rows. */
+rhd_status rhd_noise_profiles(rhd_context* ctx, const rhd_source* source, const float balance[3],
+                              rhd_noise_profile out[3], int32_t* profiled, float* gpu_ms);
+/* denoise_chroma (crates/detail/src/denoise/chroma.rs:88-148) in place on a
+ * resident source with the given profiles {luma, C1, C2}. */
+rhd_status rhd_denoise_chroma(rhd_context* ctx, rhd_source* source, float strength,
+                              const float balance[3], const float ceilings[3],
+                              const rhd_noise_profile profiles[3], float* gpu_ms);
+/* denoise_luma (crates/detail/src/denoise/luma.rs:128-176) in place with
+ * the luma profile. */
+rhd_status rhd_denoise_luma(rhd_context* ctx, rhd_source* source, float strength, float detail,
+                            const float balance[3], const float ceilings[3],
+                            const rhd_noise_profile* luma, float* gpu_ms);
 
 /* GpuSensorProcessor::develop (crates/gpu/src/sensor/develop.rs:68-142):
  * normalize -> highlight -> demosaic (+ capture) from a resident mosaic. */

@@ -36,8 +36,13 @@ class DetailDesc(ctypes.Structure):
     _fields_ = [
         ("capture_enabled", c_int32), ("amount", c_float), ("sigma", c_float),
         ("noise_protection", c_float), ("ceilings", c_float * 3), ("iterations", c_int32),
-        ("false_colour", c_int32), ("noise_reduction", c_int32), ("balance", c_float * 3),
+        ("false_colour", c_int32), ("balance", c_float * 3), ("nr_chroma", c_float),
+        ("nr_luminance", c_float), ("nr_luminance_detail", c_float),
     ]
+
+
+class NoiseProfile(ctypes.Structure):
+    _fields_ = [("a", c_float), ("b", c_float), ("bins", c_uint32)]
 
 
 class ClipStats(ctypes.Structure):
@@ -138,6 +143,12 @@ SIGNATURES = {
     "rhd_optics_full": (c_int, [_P, _P, POINTER(OpticsDesc), _P, POINTER(_P), POINTER(c_float)]),
     "rhd_color_transform": (c_int, [_P, _P, POINTER(c_float), POINTER(c_float), _P,
                                     POINTER(c_float)]),
+    "rhd_noise_profiles": (c_int, [_P, _P, POINTER(c_float), POINTER(NoiseProfile),
+                                   POINTER(c_int32), POINTER(c_float)]),
+    "rhd_denoise_chroma": (c_int, [_P, _P, c_float, POINTER(c_float), POINTER(c_float),
+                                   POINTER(NoiseProfile), POINTER(c_float)]),
+    "rhd_denoise_luma": (c_int, [_P, _P, c_float, c_float, POINTER(c_float), POINTER(c_float),
+                                 POINTER(NoiseProfile), POINTER(c_float)]),
     "rhd_server_create": (c_int, [_P, POINTER(ServerConfig), POINTER(_P)]),
     "rhd_server_destroy": (None, [_P]),
     "rhd_server_put_mosaic": (c_int, [_P, c_uint64, _P, c_uint64, c_uint32, c_uint32]),

@@ -335,8 +335,28 @@ bool capture_active(const rhd_detail_desc* dd) {
     return dd && dd->capture_enabled && dd->amount > 0.0f;
 }
 
+bool noise_active(const rhd_detail_desc* dd) {
+    return dd && (dd->nr_chroma > 0.0f || dd->nr_luminance > 0.0f);
+}
+
 bool detail_active(const rhd_detail_desc* dd) {
-    return capture_active(dd) || (dd && dd->false_colour > 0);
+    return capture_active(dd) || (dd && dd->false_colour > 0) || noise_active(dd);
+}
+
+// Bytes of lane scratch the detail stage works in, then a small area at its
+// end that holds the noise profiles between the estimate and the filters.
+static uint64_t detail_work_bytes(int w, int h, const rhd_detail_desc* dd) {
+    uint64_t bytes = capture_scratch_bytes(w, h);
+    if (noise_active(dd)) {
+        bytes = std::max<uint64_t>(bytes, noise_filter_bytes(w, h));
+        const NoiseGeometry g = noise_geometry(w, h);
+        if (g.tiles > 0) bytes = std::max<uint64_t>(bytes, g.bytes);
+    }
+    return (bytes + 255) & ~(uint64_t)255;
+}
+
+uint64_t detail_scratch_bytes(int w, int h, const rhd_detail_desc* dd) {
+    return detail_work_bytes(w, h, dd) + 256;
 }
 
 // crates/edit/src/sharpening.rs:50-81 and crates/core/src/sharpening.rs:26-42.
@@ -348,7 +368,11 @@ rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd) {
         return fail(ctx, RHD_INVALID_INPUT, "capture sharpening settings are out of range");
     if (dd->false_colour < 0 || dd->false_colour > 5)
         return fail(ctx, RHD_INVALID_INPUT, "false-colour steps must be in 0..=5");
-    if (dd->false_colour > 0) {
+    // crates/edit/src/noise_reduction.rs:63-91.
+    if (!in(dd->nr_chroma, 0.0f, 1.0f) || !in(dd->nr_luminance, 0.0f, 1.0f) ||
+        !in(dd->nr_luminance_detail, 0.0f, 1.0f))
+        return fail(ctx, RHD_INVALID_INPUT, "noise reduction settings must be finite in 0..=1");
+    if (dd->false_colour > 0 || noise_active(dd)) {
         if (!finite_positive3(dd->balance))
             return fail(ctx, RHD_INVALID_INPUT, "channel balance must be finite and positive");
         for (int c = 0; c < 3; ++c)
@@ -632,17 +656,43 @@ rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_det
 }
 
 // The detail stage's steps in the reference's order (crates/core/src/
-// pipeline/detail.rs:210-261): false colour, then capture sharpening. Noise
-// reduction is routed to the CPU before this is reached.
+// pipeline/detail.rs:210-307): noise profiles of the stage input, false
+// colour, chroma NR, luminance NR, capture sharpening. Everything is queued
+// on the lane's stream; the profiles never leave the device.
 rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd) {
     if (!detail_active(dd)) return RHD_OK;
     const int w = (int)src->width, h = (int)src->height;
-    rhd_status st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
+    const uint64_t work = detail_work_bytes(w, h, dd);
+    rhd_status st = ensure_scratch(ctx, L, L.capture, work + 256);
     if (st != RHD_OK) return st;
+    float* scratch = (float*)L.capture.ptr;
+    float* prof = (float*)((char*)L.capture.ptr + work);
+    bool denoise = false;
+    if (noise_active(dd)) {
+        const NoiseGeometry g = noise_geometry(w, h);
+        if (g.tiles > MAX_NOISE_TILES || h > 65535)
+            return fail(ctx, RHD_INVALID_DIMENSIONS, "image too large for the noise profile");
+        if (g.tiles > 0) {
+            hipError_t e = launch_noise_profile(src->d, w, h, dd->balance, scratch, prof,
+                                                L.stream, PROF(L));
+            if (e != hipSuccess) return hip_fail(ctx, e, "noise profile");
+            denoise = true;
+        }
+    }
     if (dd->false_colour > 0) {
         hipError_t e = launch_false_colour(src->d, w, h, dd->false_colour, dd->balance,
-                                           dd->ceilings, (float*)L.capture.ptr, L.stream, PROF(L));
+                                           dd->ceilings, scratch, L.stream, PROF(L));
         if (e != hipSuccess) return hip_fail(ctx, e, "false colour");
+    }
+    if (denoise && dd->nr_chroma > 0.0f) {
+        hipError_t e = launch_chroma_nr(src->d, w, h, dd->nr_chroma, dd->balance, dd->ceilings,
+                                        prof, scratch, L.stream, PROF(L));
+        if (e != hipSuccess) return hip_fail(ctx, e, "chroma noise reduction");
+    }
+    if (denoise && dd->nr_luminance > 0.0f) {
+        hipError_t e = launch_luma_nr(src->d, w, h, dd->nr_luminance, dd->nr_luminance_detail,
+                                      dd->balance, dd->ceilings, prof, scratch, L.stream, PROF(L));
+        if (e != hipSuccess) return hip_fail(ctx, e, "luminance noise reduction");
     }
     if (capture_active(dd)) return run_capture(ctx, L, src, dd);
     return RHD_OK;
@@ -694,7 +744,7 @@ rhd_status stage_scratch(rhd_context* ctx, Lane& L, const rhd_sensor_desc* d,
         st = ensure_scratch(ctx, L, L.rcd, tiles * rcd_scratch_bytes_per_tile());
     }
     if (st == RHD_OK && detail_active(dd))
-        st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
+        st = ensure_scratch(ctx, L, L.capture, detail_scratch_bytes(w, h, dd));
     return st;
 }
 
@@ -827,6 +877,43 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
 
 }  // namespace rhd
 
+// Lane 0 scratch with `work` bytes for the filters and the profile area
+// after them.
+static rhd_status noise_scratch(rhd_context* ctx, Lane& L, uint64_t work, float** prof) {
+    work = (work + 255) & ~(uint64_t)255;
+    rhd_status st = ensure_scratch(ctx, L, L.capture, work + 256);
+    if (st == RHD_OK) *prof = (float*)((char*)L.capture.ptr + work);
+    return st;
+}
+
+// Copies host profiles next to the filters' scratch and runs `launch`.
+template <class Launch>
+static rhd_status run_filter(rhd_context* ctx, rhd_source* src, const rhd_noise_profile* p,
+                             int count, float* gpu_ms, const char* what, Launch launch) {
+    Lane& L = ctx->lanes[0];
+    (void)hipSetDevice(ctx->device);
+    const int w = (int)src->width, h = (int)src->height;
+    float* prof = nullptr;
+    rhd_status st = noise_scratch(ctx, L, noise_filter_bytes(w, h), &prof);
+    if (st != RHD_OK) return st;
+    float host[6] = {};
+    for (int c = 0; c < count; ++c) {
+        host[2 * c] = p[c].a;
+        host[2 * c + 1] = p[c].b;
+    }
+    RHD_HIP(ctx, hipMemcpyAsync(prof, host, sizeof(host), hipMemcpyHostToDevice, L.stream));
+    RHD_HIP(ctx, hipEventRecord(L.ev[3], L.stream));
+    hipError_t e = launch(L, prof, (float*)L.capture.ptr);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(L.stream);
+        return hip_fail(ctx, e, what);
+    }
+    RHD_HIP(ctx, hipEventRecord(L.ev[4], L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
+    if (gpu_ms) *gpu_ms = elapsed(L.ev[3], L.ev[4]);
+    return RHD_OK;
+}
+
 extern "C" {
 
 rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
@@ -900,7 +987,7 @@ int rhd_supports(const rhd_sensor_desc* d, const rhd_detail_desc* dd) {
     bool highlight = d->highlight == RHD_HIGHLIGHT_OFF || d->highlight == RHD_HIGHLIGHT_CLIP ||
                      d->highlight == RHD_HIGHLIGHT_OPPOSED;
     bool demosaic = d->demosaic >= RHD_DEMOSAIC_BILINEAR && d->demosaic <= RHD_DEMOSAIC_AMAZE;
-    bool detail = !dd || (!dd->noise_reduction && dd->false_colour >= 0 && dd->false_colour <= 5);
+    bool detail = !dd || (dd->false_colour >= 0 && dd->false_colour <= 5);
     bool capture = !capture_active(dd) || std::ceil(3.0f * dd->sigma) <= 4.0f;
     return highlight && demosaic && detail && capture ? 1 : 0;
 }
@@ -1077,6 +1164,78 @@ rhd_status rhd_capture(rhd_context* ctx, rhd_source* src, const rhd_detail_desc*
     if ((st = finish_stats(ctx, L, nullptr, RHD_HIGHLIGHT_OFF)) != RHD_OK) return st;
     if (m) m->capture_ms = elapsed(L.ev[3], L.ev[4]);
     return RHD_OK;
+}
+
+rhd_status rhd_noise_profiles(rhd_context* ctx, const rhd_source* src, const float balance[3],
+                              rhd_noise_profile out[3], int32_t* profiled, float* gpu_ms) {
+    if (!ctx || !src || !balance || !out || !profiled) return RHD_INVALID_INPUT;
+    *profiled = 0;
+    if (gpu_ms) *gpu_ms = 0.0f;
+    rhd_status st = check_handle_ctx(ctx, src->ctx);
+    if (st != RHD_OK) return st;
+    if (!finite_positive3(balance))
+        return fail(ctx, RHD_INVALID_INPUT, "channel balance must be finite and positive");
+    const int w = (int)src->width, h = (int)src->height;
+    const NoiseGeometry g = noise_geometry(w, h);
+    if (g.tiles == 0) return RHD_OK;
+    if (g.tiles > MAX_NOISE_TILES || h > 65535)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "image too large for the noise profile");
+    Lane& L = ctx->lanes[0];
+    (void)hipSetDevice(ctx->device);
+    float* prof = nullptr;
+    if ((st = noise_scratch(ctx, L, g.bytes, &prof)) != RHD_OK) return st;
+    float host[9];
+    RHD_HIP(ctx, hipEventRecord(L.ev[3], L.stream));
+    RHD_HIP(ctx, launch_noise_profile(src->d, w, h, balance, L.capture.ptr, prof, L.stream,
+                                      PROF(L)));
+    RHD_HIP(ctx, hipEventRecord(L.ev[4], L.stream));
+    RHD_HIP(ctx, hipMemcpyAsync(host, prof, sizeof(host), hipMemcpyDeviceToHost, L.stream));
+    RHD_HIP(ctx, hipStreamSynchronize(L.stream));
+    for (int c = 0; c < 3; ++c) out[c] = {host[2 * c], host[2 * c + 1], (uint32_t)host[6 + c]};
+    *profiled = 1;
+    if (gpu_ms) *gpu_ms = elapsed(L.ev[3], L.ev[4]);
+    return RHD_OK;
+}
+
+rhd_status rhd_denoise_chroma(rhd_context* ctx, rhd_source* src, float strength,
+                              const float balance[3], const float ceilings[3],
+                              const rhd_noise_profile profiles[3], float* gpu_ms) {
+    if (!ctx || !src || !balance || !ceilings || !profiles) return RHD_INVALID_INPUT;
+    if (gpu_ms) *gpu_ms = 0.0f;
+    rhd_status st = check_handle_ctx(ctx, src->ctx);
+    if (st != RHD_OK) return st;
+    if (!std::isfinite(strength) || !(strength >= 0.0f && strength <= 1.0f))
+        return fail(ctx, RHD_INVALID_INPUT, "chroma strength must be finite in 0..=1");
+    if (strength == 0.0f) return RHD_OK;
+    if (!finite_positive3(balance))
+        return fail(ctx, RHD_INVALID_INPUT, "channel balance must be finite and positive");
+    const int w = (int)src->width, h = (int)src->height;
+    return run_filter(ctx, src, profiles, 3, gpu_ms, "chroma noise reduction",
+                      [&](Lane& L, float* prof, float* scratch) {
+                          return launch_chroma_nr(src->d, w, h, strength, balance, ceilings,
+                                                  prof, scratch, L.stream, PROF(L));
+                      });
+}
+
+rhd_status rhd_denoise_luma(rhd_context* ctx, rhd_source* src, float strength, float detail,
+                            const float balance[3], const float ceilings[3],
+                            const rhd_noise_profile* luma, float* gpu_ms) {
+    if (!ctx || !src || !balance || !ceilings || !luma) return RHD_INVALID_INPUT;
+    if (gpu_ms) *gpu_ms = 0.0f;
+    rhd_status st = check_handle_ctx(ctx, src->ctx);
+    if (st != RHD_OK) return st;
+    if (!(strength >= 0.0f && strength <= 1.0f) || !(detail >= 0.0f && detail <= 1.0f))
+        return fail(ctx, RHD_INVALID_INPUT, "luminance strength and detail must be in 0..=1");
+    if (strength == 0.0f) return RHD_OK;
+    if (!finite_positive3(balance))
+        return fail(ctx, RHD_INVALID_INPUT, "channel balance must be finite and positive");
+    const int w = (int)src->width, h = (int)src->height;
+    if (h > 65535) return fail(ctx, RHD_INVALID_DIMENSIONS, "image taller than 65535 rows");
+    return run_filter(ctx, src, luma, 1, gpu_ms, "luminance noise reduction",
+                      [&](Lane& L, float* prof, float* scratch) {
+                          return launch_luma_nr(src->d, w, h, strength, detail, balance,
+                                                ceilings, prof, scratch, L.stream, PROF(L));
+                      });
 }
 
 rhd_status rhd_develop_resident(rhd_context* ctx, const rhd_mosaic* mosaic,

@@ -53,6 +53,8 @@ double cost_of(const rhd_sensor_desc& d, const rhd_detail_desc& dd) {
     if (d.highlight == RHD_HIGHLIGHT_OPPOSED) per_px += 60.0;
     if (capture_active(&dd)) per_px += 20.0 + 6.0 * dd.iterations;
     per_px += 4.0 * dd.false_colour;
+    if (dd.nr_chroma > 0.0f) per_px += 12.0;
+    if (dd.nr_luminance > 0.0f) per_px += 16.0;
     return px * per_px;
 }
 
@@ -62,7 +64,7 @@ std::string signature(const std::vector<Job>& jobs) {
         s += std::to_string(j.desc.crop_width) + "x" + std::to_string(j.desc.crop_height) + ":" +
              std::to_string(j.desc.highlight) + std::to_string(j.desc.demosaic) + ":" +
              std::to_string(capture_active(&j.detail) ? j.detail.iterations : -1) + ":" +
-             std::to_string(j.detail.false_colour) + ";";
+             std::to_string(j.detail.false_colour) + (noise_active(&j.detail) ? "n" : "") + ";";
     }
     return s;
 }

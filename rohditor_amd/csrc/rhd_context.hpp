@@ -118,6 +118,9 @@ void device_free(rhd_context* ctx, void* p, uint64_t bytes);
 bool capture_active(const rhd_detail_desc* dd);
 // Any detail-stage step to run (false colour or capture sharpening).
 bool detail_active(const rhd_detail_desc* dd);
+bool noise_active(const rhd_detail_desc* dd);
+// Lane scratch the detail stage needs for a w x h frame.
+uint64_t detail_scratch_bytes(int w, int h, const rhd_detail_desc* dd);
 // False colour then capture sharpening on `src` (in place).
 rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
 rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd);

@@ -132,6 +132,30 @@ hipError_t launch_false_colour(float* rgb, int width, int height, int steps,
                                hipStream_t s, Profiler* prof);
 hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof);
 
+// Noise profile of a width x height image: the sampled rows (every fourth
+// 64-row band from 1024 rows on), the 16x16 tiles over them, the brightness
+// bins, the padded sort size, and the byte offsets of the estimate's scratch.
+struct NoiseGeometry {
+    int rows, tiles_x, bins;
+    int64_t tiles, sort_n;
+    size_t band0, tmp, energy, bright, keys, keys2, points, bytes;
+};
+constexpr int64_t MAX_NOISE_TILES = 1 << 24;
+NoiseGeometry noise_geometry(int width, int height);
+// Scratch of the chroma filter (7 planes; the luma filter needs 4).
+size_t noise_filter_bytes(int width, int height);
+// Writes prof = {a, b} of luma, C1, C2, then their bin counts (9 floats),
+// computed entirely on the device. Needs noise_geometry(...).tiles in
+// 1..MAX_NOISE_TILES.
+hipError_t launch_noise_profile(const float* rgb, int width, int height, const float balance[3],
+                                void* scratch, float* prof, hipStream_t s, Profiler* pr);
+hipError_t launch_chroma_nr(float* rgb, int width, int height, float strength,
+                            const float balance[3], const float ceilings[3], const float* prof,
+                            float* scratch, hipStream_t s, Profiler* pr);
+hipError_t launch_luma_nr(float* rgb, int width, int height, float strength, float detail,
+                          const float balance[3], const float ceilings[3], const float* prof,
+                          float* scratch, hipStream_t s, Profiler* pr);
+
 struct OpposedArgs {
     float* mosaic;
     int width, height, pattern;

@@ -114,7 +114,7 @@ uint64_t footprint_of(const rhd_sensor_desc& d, const rhd_detail_desc* dd) {
     if (d.demosaic == RHD_DEMOSAIC_AMAZE) bytes += tiles_of(w, h, 224) * amaze_scratch_bytes_per_tile();
     if (d.demosaic == RHD_DEMOSAIC_RCD) bytes += tiles_of(w, h, 174) * rcd_scratch_bytes_per_tile();
     if (d.highlight == RHD_HIGHLIGHT_OPPOSED) bytes += opposed_scratch_bytes((int)w, (int)h);
-    if (detail_active(dd)) bytes += capture_scratch_bytes((int)w, (int)h);
+    if (detail_active(dd)) bytes += detail_scratch_bytes((int)w, (int)h, dd);
     return bytes;
 }
 

@@ -136,6 +136,18 @@ class CaptureSharpening:
 
 
 @dataclass
+class NoiseReduction:
+    """crates/edit/src/noise_reduction.rs:27-61: strengths in [0, 1] (0 = off)
+    and the luminance filter's texture preservation (0.5 neutral)."""
+    chroma: float = 0.0
+    luminance: float = 0.0
+    luminance_detail: float = 0.5
+
+    def is_active(self) -> bool:
+        return self.chroma > 0.0 or self.luminance > 0.0
+
+
+@dataclass
 class EditRecipe:
     highlights: HighlightAdjustments = field(default_factory=HighlightAdjustments)
     capture: CaptureSharpening = field(default_factory=CaptureSharpening)
@@ -145,6 +157,7 @@ class EditRecipe:
     capture_iterations: int = CAPTURE_SHARPENING_ITERATIONS
     # FalseColorSuppression::steps (crates/edit/src/false_color.rs), 0..=5.
     false_color_steps: int = 0
+    noise_reduction: "NoiseReduction" = field(default_factory=lambda: NoiseReduction())
 
 
 @dataclass
@@ -274,6 +287,9 @@ def detail_desc(recipe: EditRecipe, gains: Sequence[float],
     dd.iterations = recipe.capture_iterations
     dd.false_colour = int(recipe.false_color_steps)
     dd.balance[:] = [float(v) for v in (balance if balance is not None else gains)]
+    nr = recipe.noise_reduction
+    dd.nr_chroma, dd.nr_luminance = nr.chroma, nr.luminance
+    dd.nr_luminance_detail = nr.luminance_detail
     return dd
 
 
@@ -445,6 +461,51 @@ class GpuSensorProcessor:
                 cancellation: Optional[CancellationToken] = None) -> None:
         self._check(self.lib.rhd_capture(self.ctx, source.ptr, ctypes.byref(detail),
                                          cancellation.ptr if cancellation else None, None))
+
+    def noise_profiles(self, source: GpuDevelopedSource, balance):
+        """estimate_noise_profiles (crates/detail/src/denoise/mod.rs:60-110),
+        on the device: [(a, b, bins)] for luma, C1, C2, or None when the image
+        is too small to profile. Returns (profiles, gpu_ms)."""
+        b = (ctypes.c_float * 3)(*[float(v) for v in balance])
+        out = (_lib.NoiseProfile * 3)()
+        have = ctypes.c_int32()
+        ms = ctypes.c_float()
+        self._check(self.lib.rhd_noise_profiles(self.ctx, source.ptr, b, out, ctypes.byref(have),
+                                                ctypes.byref(ms)))
+        if not have.value:
+            return None, float(ms.value)
+        return [(float(p.a), float(p.b), int(p.bins)) for p in out], float(ms.value)
+
+    @staticmethod
+    def _profiles(profiles):
+        out = (_lib.NoiseProfile * len(profiles))()
+        for o, p in zip(out, profiles):
+            o.a, o.b = float(p[0]), float(p[1])
+            o.bins = int(p[2]) if len(p) > 2 else 0
+        return out
+
+    def denoise_chroma(self, source: GpuDevelopedSource, strength: float, balance, ceilings,
+                       profiles) -> float:
+        """denoise_chroma (crates/detail/src/denoise/chroma.rs:88-148) in place;
+        profiles = [(a, b), ...] for luma, C1, C2. Returns the GPU milliseconds."""
+        b = (ctypes.c_float * 3)(*[float(v) for v in balance])
+        c = (ctypes.c_float * 3)(*[float(v) for v in ceilings])
+        ms = ctypes.c_float()
+        self._check(self.lib.rhd_denoise_chroma(self.ctx, source.ptr, float(strength), b, c,
+                                                self._profiles(profiles), ctypes.byref(ms)))
+        return float(ms.value)
+
+    def denoise_luma(self, source: GpuDevelopedSource, strength: float, detail: float, balance,
+                     ceilings, profile) -> float:
+        """denoise_luma (crates/detail/src/denoise/luma.rs:128-176) in place
+        with the luma profile (a, b)."""
+        b = (ctypes.c_float * 3)(*[float(v) for v in balance])
+        c = (ctypes.c_float * 3)(*[float(v) for v in ceilings])
+        ms = ctypes.c_float()
+        self._check(self.lib.rhd_denoise_luma(self.ctx, source.ptr, float(strength),
+                                              float(detail), b, c, self._profiles([profile]),
+                                              ctypes.byref(ms)))
+        return float(ms.value)
 
     def upload_source(self, rgb: np.ndarray) -> GpuDevelopedSource:
         """A resident camera source from interleaved (H, W, 3) RGB

@@ -55,7 +55,10 @@ def test_supports_mirrors_the_reference_plus_opposed():
     recipe = sensor.EditRecipe()
     desc = sensor.SensorDevelopmentDescription.from_frame(frame, recipe, sensor.RenderOptions())
     dd = sensor.detail_desc(recipe, desc.demosaic_balance)
-    dd.noise_reduction = 1
+    dd.nr_chroma, dd.nr_luminance = 0.5, 0.5
+    assert sensor.GpuSensorProcessor.supports(desc, dd)
+    # capture radii above 4 taps are routed (ceil(3 sigma) > 4)
+    dd.capture_enabled, dd.amount, dd.sigma = 1, 0.5, 1.5
     assert not sensor.GpuSensorProcessor.supports(desc, dd)
 
 

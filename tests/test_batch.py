@@ -78,10 +78,10 @@ def test_batch_equals_plain_on_every_replay(batch_case, lanes):
     batch.close()
 
 
-def test_batch_rejects_unsupported_edits(batch_case):
+def test_batch_rejects_invalid_detail(batch_case):
     proc, cases, residents, plain = batch_case
     detail = sensor.detail_desc(sensor.EditRecipe(), cases[0][1].demosaic_balance)
-    detail.noise_reduction = 1
+    detail.nr_chroma = 1.5
     with pytest.raises(sensor.GpuPreviewError) as e:
         sensor.GpuBatch(proc, residents[:1], [cases[0][1]], [detail], lanes=1)
-    assert e.value.is_routing()
+    assert e.value.status == _lib.INVALID_INPUT
