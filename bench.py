@@ -51,6 +51,9 @@ def parse():
                         "area reduction to a 2560 px long edge")
     p.add_argument("--c5-frames", type=int, default=4,
                    help="with --c5: use the first N frames of the 1:3 mix (1 = the 102 MP frame)")
+    p.add_argument("--detail", action="store_true",
+                   help="the 24 MP develop with the whole detail stage: false colour (2 steps), "
+                        "chroma and luminance noise reduction (0.5) and RL")
     p.add_argument("--batch-lanes", type=int, default=0, help="0: autotune")
     p.add_argument("--served-lanes", type=int, default=3)
     p.add_argument("--served-requests", type=int, default=60)
@@ -77,7 +80,7 @@ COPY_BYTES = 1 << 30
 def short_kernel_name(name):
     """`rhd::(anonymous namespace)::cap_iterate<3>(...)` -> `cap_iterate<3>`."""
     import re
-    m = re.search(r"::(\w+)(<\d+>)?\(", name)
+    m = re.search(r"::(\w+)(<\w+>)?\(", name)
     return (m.group(1) + (m.group(2) or "")) if m else name.split("(")[0][:48]
 
 
@@ -209,6 +212,9 @@ def main():
         return
     if args.c5:
         c5_bench(args)
+        return
+    if args.detail:
+        detail_bench(args)
         return
     from rohditor_amd import _lib, sensor
 
@@ -664,6 +670,109 @@ def c5_bench(args):
         f[0].release()
     proc.close()
     print(json.dumps(line))
+
+
+NR_KERNELS = ("nr_est_split", "nr_b3_h", "nr_est_band", "nr_est_tiles", "nr_keys", "nr_sort",
+              "nr_gather", "nr_select", "nr_select_hist", "nr_fit", "nr_chroma_level",
+              "nr_chroma_last", "nr_luma_h", "nr_luma_v", "nr_luma_energy_h", "nr_luma_shrink",
+              "nr_luma_last")
+
+
+def detail_bench(args):
+    """The 24 MP frame of the headline workload developed with the whole
+    detail stage (crates/core/src/pipeline/detail.rs:210-307): noise profiles
+    of the demosaiced frame, false colour (2 median steps), chroma and
+    luminance noise reduction (strength 0.5, detail 0.5), RL(10). Timed next
+    to the headline develop without them; per-kernel times from a profiled
+    pass. The CPU baseline runs the oracle's detail stage on a 1536x1024 crop
+    of the same demosaiced frame."""
+    from rohditor_amd import sensor
+    spec, frame, base_recipe, options = workload()
+    recipe = sensor.EditRecipe(
+        highlights=base_recipe.highlights, capture=base_recipe.capture,
+        capture_iterations=base_recipe.capture_iterations, false_color_steps=2,
+        noise_reduction=sensor.NoiseReduction(chroma=0.5, luminance=0.5, luminance_detail=0.5))
+    proc = sensor.GpuSensorProcessor(0)
+    desc = sensor.SensorDevelopmentDescription.from_frame(frame, recipe, options)
+    detail = sensor.detail_desc(recipe, desc.demosaic_balance)
+    plain = sensor.detail_desc(base_recipe, desc.demosaic_balance)
+    resident = proc.upload(frame)
+    mp = desc.crop.width * desc.crop.height / 1e6
+
+    def timed(dd):
+        for _ in range(args.warmup):
+            proc.develop_resident(resident, desc, dd).release()
+        proc.lib.rhd_synchronize(proc.ctx)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s0 = time.perf_counter()
+            proc.develop_resident(resident, desc, dd).release()
+            lat.append(time.perf_counter() - s0)
+        proc.lib.rhd_synchronize(proc.ctx)
+        return (time.perf_counter() - t0) / args.steps, np.array(lat) * 1e3
+
+    full_s, lat = timed(detail)
+    plain_s, _ = timed(plain)
+    print(f"detail: full {full_s * 1e3:.3f} ms, plain {plain_s * 1e3:.3f} ms", file=sys.stderr,
+          flush=True)
+    proc.lib.rhd_profile_reset(proc.ctx)
+    proc.lib.rhd_profile_enable(proc.ctx, 1)
+    for _ in range(args.steps):
+        proc.develop_resident(resident, desc, detail).release()
+    proc.lib.rhd_profile_enable(proc.ctx, 0)
+    prof = profile_table(proc)
+    nr = {k: v for k, v in prof.items() if k in NR_KERNELS}
+    nr_ms = sum(v[0] for v in nr.values()) / args.steps
+    dom = max((k for k in nr if nr[k][2] > 0), key=lambda k: nr[k][0])
+    dom_ms, dom_launches, dom_bytes = nr[dom]
+    avg_ms = dom_ms / max(dom_launches, 1)
+    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        base = proc.develop_resident(resident, desc, plain)
+        rgb = base.read_rgb()[:1024, :1536].copy()
+        base.release()
+        cpu = detail_cpu_baseline(rgb, detail)
+    resident.release()
+    proc.close()
+    line = {
+        "metric": "sensor MP/s with the whole detail stage (false colour + chroma/luma NR + RL)",
+        "value": round(mp / full_s, 2), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(full_s * 1e3, 3), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "plain_ms_per_step": round(plain_s * 1e3, 3),
+        "noise_reduction_gpu_ms_per_step": round(nr_ms, 4),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3)},
+        "roofline": {"bound": "hbm", "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
+                     "launches_per_step": dom_launches / args.steps,
+                     "algorithmic_bytes_per_launch": int(dom_bytes),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4)},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(
+            prof.items(), key=lambda kv: -kv[1][0])},
+        "cpu_baseline": cpu,
+        "config": {"workload": "full-sensor 6048x4024 RGGB 14-bit, Opposed+AMaZE, false colour "
+                               "2 steps, chroma NR 0.5, luminance NR 0.5 (detail 0.5), "
+                               "RL(sigma 0.7, 10 iterations)", "parallelism": "single GPU"},
+    }
+    print(json.dumps(line))
+
+
+def detail_cpu_baseline(rgb, detail):
+    from oracle import oracle
+    h, w = rgb.shape[:2]
+    t0 = time.perf_counter()
+    st, _, _ = oracle.detail_stage(
+        rgb, false_colour=detail.false_colour, chroma=detail.nr_chroma,
+        luminance=detail.nr_luminance, luminance_detail=detail.nr_luminance_detail,
+        balance=detail.balance[:], ceilings=detail.ceilings[:],
+        capture=(detail.amount, detail.sigma, detail.noise_protection, detail.iterations))
+    dt = time.perf_counter() - t0
+    return {"value": round(w * h / 1e6 / dt, 3), "unit": "MP/s", "cores": 1, "kind": "port",
+            "sample": f"the detail stage alone on a {w}x{h} crop of the demosaiced frame, "
+                      f"{dt:.2f} s, status {st}"}
 
 
 def profile_table(proc):

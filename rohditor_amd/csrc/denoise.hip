@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void nr_b3_h(const float* __restrict__ src,
     const float* line = src + blockIdx.z * pitch + (size_t)y * w;
     float t[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) t[k] = line[mirror_half(x + (k - 2) * step, w)];
+    for (int k = 0; k < 5; ++k) t[k] = line[mirror_near(x + (k - 2) * step, w)];
     dst[blockIdx.z * pitch + (size_t)y * w + x] = b3(t[0], t[1], t[2], t[3], t[4]);
 }
 
@@ -114,9 +114,9 @@ __global__ __launch_bounds__(256) void nr_est_band(float* planes, const float* _
     const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
     if (x >= w) return;
     const float* t = tmp + z * pitch + x;
-    const float s = b3(t[(size_t)mirror_half(y - 2, rows) * w], t[(size_t)mirror_half(y - 1, rows) * w],
-                       t[(size_t)y * w], t[(size_t)mirror_half(y + 1, rows) * w],
-                       t[(size_t)mirror_half(y + 2, rows) * w]);
+    const float s = b3(t[(size_t)mirror_near(y - 2, rows) * w], t[(size_t)mirror_near(y - 1, rows) * w],
+                       t[(size_t)y * w], t[(size_t)mirror_near(y + 1, rows) * w],
+                       t[(size_t)mirror_near(y + 2, rows) * w]);
     const size_t o = (size_t)y * w + x;
     const float d = planes[z * pitch + o] - s;
     if (z == 0)
@@ -203,8 +203,7 @@ __global__ __launch_bounds__(1024) void nr_sort_chunk(uint64_t* keys, int N, int
     const int k_first = k_only ? k_only : 2, k_last = k_only ? k_only : L;
     for (int k = k_first; k <= k_last; k <<= 1)
         for (int j = (k_only ? L : k) >> 1; j > 0; j >>= 1) {
-            const int t = threadIdx.x;
-            if (t < L / 2) {
+            for (int t = threadIdx.x; t < L / 2; t += 1024) {
                 const int i = (t / j) * 2 * j + (t % j);
                 order_pair(s[i], s[i + j], ((base + i) & k) == 0);
             }
@@ -225,6 +224,27 @@ __global__ __launch_bounds__(256) void nr_sort_step(uint64_t* keys, int N, int k
     a[i + j] = up ? hi : lo;
 }
 
+// Sorts each brightness bin of each plane's energy keys in LDS (the bins
+// are contiguous and already hold their keys; L >= the longest bin).
+template <int L>
+__global__ __launch_bounds__(1024) void nr_sort_bins(uint64_t* keys, int N, int nt, int bins) {
+    __shared__ uint64_t s[L];
+    const int b = blockIdx.x;
+    const int64_t start = (int64_t)b * nt / bins, len = (int64_t)(b + 1) * nt / bins - start;
+    uint64_t* a = keys + (size_t)blockIdx.y * N + start;
+    for (int i = threadIdx.x; i < L; i += 1024) s[i] = i < len ? a[i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= L; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < L / 2; t += 1024) {
+                const int i = (t / j) * 2 * j + (t % j);
+                order_pair(s[i], s[i + j], (i & k) == 0);
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < len; i += 1024) a[i] = s[i];
+}
+
 void sort_keys(uint64_t* keys, int N, int arrays, hipStream_t s, Profiler* prof) {
     const int L = N < SORT_CHUNK ? N : SORT_CHUNK;
     const dim3 chunks(N / L, arrays);
@@ -242,86 +262,197 @@ void sort_keys(uint64_t* keys, int N, int arrays, hipStream_t s, Profiler* prof)
 constexpr float DETAIL_NOISE_GAIN = 0.8908f;
 constexpr float MAD_TO_SIGMA = 1.4826f;
 constexpr int MAX_BINS = 16;
-constexpr int SELECT_THREADS = 1024;
+constexpr int HIST_CHUNK = 4096;  // samples per histogram workgroup
 constexpr int BRIGHT_CHUNK = 4096;
 
-// noise.rs:96-109 for brightness bin blockIdx.x of plane blockIdx.y: the
-// flattest quarter's samples give sigma = 1.4826 MAD / 0.8908, the tiles'
-// mean brightness (in their energy order) gives x; point = (x, sigma^2).
-__global__ __launch_bounds__(SELECT_THREADS) void nr_select(
-    const float* __restrict__ band0, const float* __restrict__ planes, size_t pitch, int w,
-    int tiles_x, int nt, int bins, const uint64_t* __restrict__ by_bright,
-    const uint64_t* __restrict__ by_energy, int N, const float* __restrict__ bright,
-    float* __restrict__ points) {
-    __shared__ unsigned hist[256];
-    __shared__ float s_bright[BRIGHT_CHUNK];
-    __shared__ uint32_t s_prefix, s_k;
-    __shared__ float s_value[2];
-    const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
-    if (tid == 0) s_value[0] = 0.0f;
-    const int64_t start = (int64_t)b * nt / bins, end = (int64_t)(b + 1) * nt / bins;
-    const int64_t len = end - start;
-    const int64_t flat = len / 4 > 1 ? len / 4 : 1;
-    const uint32_t count = (uint32_t)(flat * 256);
-    const uint64_t* order = by_energy + (size_t)c * N + start;
-    const float* band = c == 0 ? band0 : planes + c * pitch;
-    auto tile_of = [&](int64_t i) -> uint32_t { return (uint32_t)by_bright[order[i] & POS_MASK]; };
-    auto sample = [&](uint32_t i) -> float {
-        const uint32_t t = tile_of(i >> 8);
-        const size_t y = (size_t)(t / tiles_x) * 16 + ((i >> 4) & 15);
-        const size_t x = (size_t)(t % tiles_x) * 16 + (i & 15);
-        return band[y * w + x];
-    };
-    // Two selections at index count/2 in total order: the median, then the
-    // median of |v - median| (noise.rs:127-139).
-    for (int round = 0; round < 2; ++round) {
-        if (tid == 0) {
-            s_prefix = 0;
-            s_k = count / 2;
-        }
-        uint32_t mask = 0;
-        for (int shift = 24; shift >= 0; shift -= 8) {
-            if (tid < 256) hist[tid] = 0;
-            __syncthreads();
-            const uint32_t prefix = s_prefix;
-            const float median = s_value[0];
-            for (uint32_t i = tid; i < count; i += SELECT_THREADS) {
-                float v = sample(i);
-                if (round) v = fabsf(v - median);
-                const uint32_t key = order_key(v);
-                if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                uint32_t k = s_k, below = 0;
-                for (uint32_t d = 0; d < 256; ++d) {
-                    const uint32_t hd = hist[d];
-                    if (k < below + hd) {
-                        s_prefix = prefix | (d << shift);
-                        s_k = k - below;
-                        break;
-                    }
-                    below += hd;
-                }
-            }
-            mask |= 0xffu << shift;
-            __syncthreads();
-        }
-        if (tid == 0) s_value[round] = key_value(s_prefix);
+// Brightness bin b of nt tiles in `bins` bins: its first sorted position and
+// the size of its flattest quarter (noise.rs:96-109, 119-121).
+__host__ __device__ __forceinline__ void segment(int nt, int bins, int b, int64_t& start,
+                                                 int64_t& flat) {
+    start = (int64_t)b * nt / bins;
+    const int64_t len = (int64_t)(b + 1) * nt / bins - start;
+    flat = len / 4 > 1 ? len / 4 : 1;
+}
+
+// The bin that holds item `i` of a per-plane list where bin b owns `per(b)`
+// consecutive items; returns the bin and the index of its first item.
+template <class Per>
+__device__ __forceinline__ int locate(int64_t i, int bins, Per per, int64_t& first) {
+    first = 0;
+    for (int b = 0; b < bins; ++b) {
+        const int64_t n = per(b);
+        if (i < first + n) return b;
+        first += n;
+    }
+    return -1;
+}
+
+struct Selection {
+    int nt, bins, N, tiles_x, w;
+    size_t pitch;
+    int64_t plane_samples;  // flat samples of one plane, all bins
+};
+
+// Copies the flattest tiles' band samples of every (plane, bin) into one
+// contiguous run per bin: one workgroup per flat tile, 16x16 threads.
+__global__ __launch_bounds__(256) void nr_gather(Selection q, const float* __restrict__ band0,
+                                                 const float* __restrict__ planes,
+                                                 const uint64_t* __restrict__ by_bright,
+                                                 const uint64_t* __restrict__ by_energy,
+                                                 float* __restrict__ samples) {
+    const int c = blockIdx.y;
+    int64_t first;
+    const int b = locate(blockIdx.x, q.bins, [&](int k) {
+        int64_t st, fl;
+        segment(q.nt, q.bins, k, st, fl);
+        return fl;
+    }, first);
+    if (b < 0) return;
+    int64_t start, flat;
+    segment(q.nt, q.bins, b, start, flat);
+    const int64_t i = blockIdx.x - first;
+    const uint32_t tile =
+        (uint32_t)by_bright[by_energy[(size_t)c * q.N + start + i] & POS_MASK];
+    const float* band = c == 0 ? band0 : planes + c * q.pitch;
+    const size_t y = (size_t)(tile / q.tiles_x) * 16 + (threadIdx.x >> 4);
+    const size_t x = (size_t)(tile % q.tiles_x) * 16 + (threadIdx.x & 15);
+    // Samples of bin b start after the flat samples of the bins before it.
+    samples[(size_t)c * q.plane_samples + (size_t)first * 256 + (size_t)i * 256 + threadIdx.x] =
+        band[y * q.w + x];
+}
+
+// Radix-select state of one (plane, bin): the key prefix found so far, the
+// rank still sought below it, and the two results.
+struct SelectState {
+    uint32_t prefix, rank;
+    float median, mad;
+};
+
+__global__ void nr_select_init(Selection q, SelectState* __restrict__ st,
+                               uint32_t* __restrict__ hist) {
+    const int seg = blockIdx.x, t = threadIdx.x;
+    hist[seg * 256 + t] = 0;
+    if (t == 0) {
+        int64_t start, flat;
+        segment(q.nt, q.bins, seg % q.bins, start, flat);
+        st[seg].prefix = 0;
+        st[seg].rank = (uint32_t)(flat * 256 / 2);
+        st[seg].median = 0.0f;
+        st[seg].mad = 0.0f;
+    }
+}
+
+// One 8-bit radix pass over HIST_CHUNK samples of one (plane, bin): the
+// histogram of the digit below the prefix, of the samples (round 0) or of
+// their distance to the median (round 1), in total order.
+__global__ __launch_bounds__(256) void nr_select_hist(Selection q, int round, int pass,
+                                                      const float* __restrict__ samples,
+                                                      const SelectState* __restrict__ st,
+                                                      uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const int c = blockIdx.y, t = threadIdx.x;
+    int64_t first;
+    const int b = locate(blockIdx.x, q.bins, [&](int k) {
+        int64_t s0, fl;
+        segment(q.nt, q.bins, k, s0, fl);
+        return (fl * 256 + HIST_CHUNK - 1) / HIST_CHUNK;
+    }, first);
+    if (b < 0) return;
+    int64_t sample0 = 0;
+    for (int k = 0; k < b; ++k) {
+        int64_t s0, fl;
+        segment(q.nt, q.bins, k, s0, fl);
+        sample0 += fl * 256;
+    }
+    int64_t start, flat;
+    segment(q.nt, q.bins, b, start, flat);
+    const int seg = c * q.bins + b;
+    const int64_t lo = (blockIdx.x - first) * (int64_t)HIST_CHUNK;
+    const int64_t hi = flat * 256 < lo + HIST_CHUNK ? flat * 256 : lo + HIST_CHUNK;
+    const float* v = samples + (size_t)c * q.plane_samples + sample0;
+    const uint32_t prefix = st[seg].prefix;
+    const float median = st[seg].median;
+    const int shift = 24 - 8 * pass;
+    const uint32_t mask = pass == 0 ? 0u : 0xffffffffu << (32 - 8 * pass);
+    h[t] = 0;
+    __syncthreads();
+    for (int64_t i = lo + t; i < hi; i += 256) {
+        const float x = round ? fabsf(v[i] - median) : v[i];
+        const uint32_t key = order_key(x);
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[t]) atomicAdd(&hist[seg * 256 + t], h[t]);
+}
+
+// Picks the digit that holds the sought rank, clears the histogram for the
+// next pass and, after the last pass, stores the selected value.
+__global__ __launch_bounds__(256) void nr_select_digit(Selection q, int round, int pass,
+                                                       SelectState* __restrict__ st,
+                                                       uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const int seg = blockIdx.x, t = threadIdx.x;
+    // Read before the barriers: one thread rewrites the state afterwards.
+    const uint32_t rank = st[seg].rank, old_prefix = st[seg].prefix;
+    const uint32_t mine = hist[seg * 256 + t];
+    hist[seg * 256 + t] = 0;
+    h[t] = mine;
+    __syncthreads();
+    // Inclusive prefix sums of the digit counts.
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint32_t add = t >= d ? h[t - d] : 0u;
+        __syncthreads();
+        h[t] += add;
         __syncthreads();
     }
-    // f32 Sum folds from -0.0, in the flat tiles' order.
+    const uint32_t above = h[t], below = above - mine;
+    // The one digit whose counts span the sought rank.
+    if (!(rank >= below && rank < above)) return;
+    const int shift = 24 - 8 * pass;
+    const uint32_t prefix = old_prefix | ((uint32_t)t << shift);
+    if (pass < 3) {
+        st[seg].prefix = prefix;
+        st[seg].rank = rank - below;
+        return;
+    }
+    if (round == 0) {
+        // The second selection seeks the same index, len / 2, among the
+        // distances to this median.
+        int64_t start, flat;
+        segment(q.nt, q.bins, seg % q.bins, start, flat);
+        st[seg].median = key_value(prefix);
+        st[seg].prefix = 0;
+        st[seg].rank = (uint32_t)(flat * 256 / 2);
+    } else {
+        st[seg].mad = key_value(prefix);
+    }
+}
+
+// noise.rs:104-108: sigma = 1.4826 MAD / 0.8908 and x = the mean brightness
+// of the flat tiles, summed in their energy order (f32 Sum folds from -0.0).
+__global__ __launch_bounds__(1024) void nr_points(Selection q,
+                                                  const uint64_t* __restrict__ by_bright,
+                                                  const uint64_t* __restrict__ by_energy,
+                                                  const float* __restrict__ bright,
+                                                  const SelectState* __restrict__ st,
+                                                  float* __restrict__ points) {
+    __shared__ float s_bright[BRIGHT_CHUNK];
+    const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    int64_t start, flat;
+    segment(q.nt, q.bins, b, start, flat);
+    const uint64_t* order = by_energy + (size_t)c * q.N + start;
     float sum = -0.0f;
     for (int64_t base = 0; base < flat; base += BRIGHT_CHUNK) {
         const int64_t m = flat - base < BRIGHT_CHUNK ? flat - base : BRIGHT_CHUNK;
-        for (int64_t j = tid; j < m; j += SELECT_THREADS) s_bright[j] = bright[tile_of(base + j)];
+        for (int64_t j = tid; j < m; j += 1024)
+            s_bright[j] = bright[(uint32_t)by_bright[order[base + j] & POS_MASK]];
         __syncthreads();
         if (tid == 0)
             for (int64_t j = 0; j < m; ++j) sum += s_bright[j];
         __syncthreads();
     }
     if (tid == 0) {
-        const float sigma = MAD_TO_SIGMA * s_value[1] / DETAIL_NOISE_GAIN;
+        const float sigma = MAD_TO_SIGMA * st[c * q.bins + b].mad / DETAIL_NOISE_GAIN;
         points[(c * MAX_BINS + b) * 2] = sum / (float)flat;
         points[(c * MAX_BINS + b) * 2 + 1] = sigma * sigma;
     }
@@ -393,8 +524,8 @@ __global__ __launch_bounds__(256) void nr_chroma_level(
     int rows[3], cols[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        rows[k] = mirror_half(y + (k - 1) * step, h);
-        cols[k] = mirror_half(x + (k - 1) * step, w);
+        rows[k] = mirror_near(y + (k - 1) * step, h);
+        cols[k] = mirror_near(x + (k - 1) * step, w);
     }
     float g[3][3], s1[3][3], s2[3][3];
 #pragma unroll
@@ -420,8 +551,12 @@ __global__ __launch_bounds__(256) void nr_chroma_level(
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
+            // The centre tap is the guide itself (finite, as zeroed), and inv
+            // is finite, so its weight is exactly 0.25 * 0.25 * 1.
             const float d = g[j][k] - centre;
-            const float wt = KERNEL[j] * KERNEL[k] * fast_exp_neg(d * d * inv);
+            const float wt = (j == 1 && k == 1)
+                                 ? KERNEL[1] * KERNEL[1]
+                                 : KERNEL[j] * KERNEL[k] * fast_exp_neg(d * d * inv);
             sum1 += wt * s1[j][k];
             sum2 += wt * s2[j][k];
             wsum += wt;
@@ -483,7 +618,7 @@ __global__ __launch_bounds__(256) void nr_luma_h(const float* __restrict__ src, 
     float t[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const size_t q = row + mirror_half(x + (k - 2) * step, w);
+        const size_t q = row + mirror_near(x + (k - 2) * step, w);
         t[k] = FROM_RGB ? luma_of(src[q], src[n + q], src[2 * n + q], bal) : src[q];
     }
     tmp[row + x] = b3(t[0], t[1], t[2], t[3], t[4]);
@@ -498,10 +633,10 @@ __global__ __launch_bounds__(256) void nr_luma_v(const float* src, size_t n, int
     const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     const float* t = tmp + x;
-    const float s = b3(t[(size_t)mirror_half(y - 2 * step, h) * w],
-                       t[(size_t)mirror_half(y - step, h) * w], t[(size_t)y * w],
-                       t[(size_t)mirror_half(y + step, h) * w],
-                       t[(size_t)mirror_half(y + 2 * step, h) * w]);
+    const float s = b3(t[(size_t)mirror_near(y - 2 * step, h) * w],
+                       t[(size_t)mirror_near(y - step, h) * w], t[(size_t)y * w],
+                       t[(size_t)mirror_near(y + step, h) * w],
+                       t[(size_t)mirror_near(y + 2 * step, h) * w]);
     const size_t i = (size_t)y * w + x;
     const float cur = FROM_RGB ? luma_of(src[i], src[n + i], src[2 * n + i], bal) : src[i];
     smooth[i] = s;
@@ -517,7 +652,7 @@ __global__ __launch_bounds__(256) void nr_luma_energy_h(const float* __restrict_
     float sum = 0.0f;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        const float v = line[mirror_half(x + k - 3, w)];
+        const float v = line[mirror_near(x + k - 3, w)];
         sum += v * v;
     }
     out[(size_t)y * w + x] = sum;
@@ -537,7 +672,7 @@ __global__ __launch_bounds__(256) void nr_luma_shrink(const float* __restrict__ 
     if (x >= w) return;
     float energy = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) energy += hsum[(size_t)mirror_half(y + k - 3, h) * w + x];
+    for (int k = 0; k < 7; ++k) energy += hsum[(size_t)mirror_near(y + k - 3, h) * w + x];
     energy *= 1.0f / 49.0f;
     const size_t i = (size_t)y * w + x;
     const float coarse = smooth[i];
@@ -599,7 +734,19 @@ NoiseGeometry noise_geometry(int width, int height) {
     g.keys = align_up(g.bright + (size_t)g.tiles * sizeof(float));
     g.keys2 = align_up(g.keys + (size_t)N * sizeof(uint64_t));
     g.points = align_up(g.keys2 + 3 * (size_t)N * sizeof(uint64_t));
-    g.bytes = align_up(g.points + 3 * MAX_BINS * 2 * sizeof(float));
+    const int bins = g.bins >= 8 ? g.bins : 1;
+    g.flat_tiles = 0;
+    g.hist_blocks = 0;
+    for (int b = 0; b < bins && g.tiles > 0; ++b) {
+        int64_t start, flat;
+        segment((int)g.tiles, bins, b, start, flat);
+        g.flat_tiles += flat;
+        g.hist_blocks += (flat * 256 + HIST_CHUNK - 1) / HIST_CHUNK;
+    }
+    g.state = align_up(g.points + 3 * MAX_BINS * 2 * sizeof(float));
+    g.hist = align_up(g.state + 3 * MAX_BINS * sizeof(SelectState));
+    g.samples = align_up(g.hist + 3 * MAX_BINS * 256 * sizeof(uint32_t));
+    g.bytes = align_up(g.samples + 3 * (size_t)g.flat_tiles * 256 * sizeof(float));
     return g;
 }
 
@@ -643,11 +790,41 @@ hipError_t launch_noise_profile(const float* rgb, int width, int height, const f
     RHD_TIMED(pr, s, "nr_keys", 48 * (uint64_t)N,
               nr_keys_energy<<<dim3((N + 255) / 256, 3), 256, 0, s>>>(keys, energy, nt, N, bins,
                                                                       keys2));
-    sort_keys(keys2, N, 3, s, pr);
-    RHD_TIMED(pr, s, "nr_select", 8 * 4 * px / 4,
-              nr_select<<<dim3(bins, 3), SELECT_THREADS, 0, s>>>(
-                  band0, planes, pitch, width, g.tiles_x, nt, bins, keys, keys2, N, bright,
-                  points));
+    // Inside each bin only: whole bins in LDS when they fit, else the
+    // full-array network (the bin field keeps the bins apart).
+    const int64_t longest = ((int64_t)nt + bins - 1) / bins;
+    const dim3 per_bin(bins, 3);
+    if (longest <= 2048)
+        RHD_TIMED(pr, s, "nr_sort", 16 * 3 * (uint64_t)nt,
+                  nr_sort_bins<2048><<<per_bin, 1024, 0, s>>>(keys2, N, nt, bins));
+    else if (longest <= 4096)
+        RHD_TIMED(pr, s, "nr_sort", 16 * 3 * (uint64_t)nt,
+                  nr_sort_bins<4096><<<per_bin, 1024, 0, s>>>(keys2, N, nt, bins));
+    else if (longest <= 8192)
+        RHD_TIMED(pr, s, "nr_sort", 16 * 3 * (uint64_t)nt,
+                  nr_sort_bins<8192><<<per_bin, 1024, 0, s>>>(keys2, N, nt, bins));
+    else
+        sort_keys(keys2, N, 3, s, pr);
+    Selection q{nt, bins, N, g.tiles_x, width, pitch, g.flat_tiles * 256};
+    SelectState* state = (SelectState*)(base + g.state);
+    uint32_t* hist = (uint32_t*)(base + g.hist);
+    float* samples = (float*)(base + g.samples);
+    const uint64_t nsamples = 3 * (uint64_t)g.flat_tiles * 256;
+    RHD_TIMED(pr, s, "nr_select", 0,
+              nr_select_init<<<3 * bins, 256, 0, s>>>(q, state, hist));
+    RHD_TIMED(pr, s, "nr_gather", 8 * nsamples,
+              nr_gather<<<dim3((unsigned)g.flat_tiles, 3), 256, 0, s>>>(q, band0, planes, keys,
+                                                                       keys2, samples));
+    for (int round = 0; round < 2; ++round)
+        for (int pass = 0; pass < 4; ++pass) {
+            RHD_TIMED(pr, s, "nr_select_hist", 4 * nsamples,
+                      nr_select_hist<<<dim3((unsigned)g.hist_blocks, 3), 256, 0, s>>>(
+                          q, round, pass, samples, state, hist));
+            RHD_TIMED(pr, s, "nr_select", 0,
+                      nr_select_digit<<<3 * bins, 256, 0, s>>>(q, round, pass, state, hist));
+        }
+    RHD_TIMED(pr, s, "nr_select", 0,
+              nr_points<<<dim3(bins, 3), 1024, 0, s>>>(q, keys, keys2, bright, state, points));
     RHD_TIMED(pr, s, "nr_fit", 0, nr_fit<<<1, 64, 0, s>>>(points, g.bins, prof));
     return hipGetLastError();
 }

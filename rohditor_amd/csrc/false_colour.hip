@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void fc_median(const float* __restrict__ a,
     int rows[3], cols[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        rows[k] = mirror_half(y + k - 1, height) * width;
-        cols[k] = mirror_half(x + k - 1, width);
+        rows[k] = mirror_near(y + k - 1, height) * width;
+        cols[k] = mirror_near(x + k - 1, width);
     }
     float ca[3][3], cb[3][3];
 #pragma unroll

@@ -35,6 +35,16 @@ __host__ __device__ __forceinline__ int mirror_half(int index, int length) {
     return folded < length ? folded : 2 * length - 1 - folded;
 }
 
+// mirror_half for stencil taps: inside the image a compare, one reflection
+// for taps up to one length outside, the modulo form only beyond that
+// (taps farther apart than the image is wide).
+__host__ __device__ __forceinline__ int mirror_near(int index, int length) {
+    if (index >= 0 && index < length) return index;
+    if (index < 0 && index >= -length) return -index - 1;
+    if (index >= length && index < 2 * length) return 2 * length - 1 - index;
+    return mirror_half(index, length);
+}
+
 // Compare-select min/max (crates/demosaic/src/amaze/math.rs:95-105).
 __device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }

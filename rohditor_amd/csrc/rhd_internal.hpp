@@ -137,8 +137,8 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof);
 // bins, the padded sort size, and the byte offsets of the estimate's scratch.
 struct NoiseGeometry {
     int rows, tiles_x, bins;
-    int64_t tiles, sort_n;
-    size_t band0, tmp, energy, bright, keys, keys2, points, bytes;
+    int64_t tiles, sort_n, flat_tiles, hist_blocks;
+    size_t band0, tmp, energy, bright, keys, keys2, points, state, hist, samples, bytes;
 };
 constexpr int64_t MAX_NOISE_TILES = 1 << 24;
 NoiseGeometry noise_geometry(int width, int height);
