@@ -44,7 +44,9 @@ typedef enum rhd_status {
     RHD_UNSUPPORTED_EDITS = 6,  /* routing: run on the CPU, keep the device */
     RHD_SYNCHRONIZATION = 7,    /* a HIP call or kernel failed */
     RHD_INVALID_INPUT = 8,      /* invalid metadata, non-finite samples, bad handles */
-    RHD_READBACK = 9
+    RHD_READBACK = 9,
+    RHD_RAW_UNSUPPORTED = 10,   /* RawError::Unsupported: not a raw this decoder reads */
+    RHD_RAW_CORRUPT = 11        /* RawError::Corrupt: the container or strip is damaged */
 } rhd_status;
 
 /* crates/image/src/lib.rs:52-58 (BayerPattern); codes as in
@@ -189,6 +191,38 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
  * (crates/gpu/src/capture/mod.rs:54), in place on `source`. */
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* source, const rhd_detail_desc* detail,
                        const volatile uint8_t* cancel, rhd_metrics* metrics);
+
+/*
+ * RAW ingest: the Sony ARW strip that crates/raw/src/rawler_adapter.rs:
+ * 131-170 turns into RawFrame::mosaic through rawler 0.7.2. rhd_arw_probe
+ * walks the TIFF container on the host (IFD0, SubIFDs, the IFD chain) to the
+ * raw IFD; rhd_mosaic_decode_arw uploads its strip and decodes it on the
+ * device into a resident mosaic (row stride = width) that rhd_develop_resident
+ * and the batch and serving paths take like an uploaded one. Compressed
+ * (TIFF compression 32767, 8 bits per pixel, tone curve from tag 0x7010) is
+ * decoded by a kernel; uncompressed 16-bit little-endian strips are uploaded
+ * as they are. Metadata (CFA, levels, white balance) still comes from the
+ * caller's RawFileInfo.
+ */
+typedef enum rhd_arw_kind {
+    RHD_ARW_NONE = 0,
+    RHD_ARW_COMPRESSED = 1,
+    RHD_ARW_UNCOMPRESSED16 = 2
+} rhd_arw_kind;
+typedef struct rhd_arw_layout {
+    uint32_t width, height, bits_per_sample, compression;
+    uint64_t offset, bytes;  /* the strip inside the file */
+    uint16_t curve[4];       /* SonyCurve knots (compressed only) */
+    int32_t kind;            /* rhd_arw_kind */
+} rhd_arw_layout;
+/* RHD_RAW_UNSUPPORTED (not a TIFF, or no raw IFD this decoder reads) or
+ * RHD_RAW_CORRUPT (an IFD, value or strip outside the file, an IFD loop). */
+rhd_status rhd_arw_probe(const uint8_t* file, uint64_t size, rhd_arw_layout* out);
+rhd_status rhd_mosaic_decode_arw(rhd_context* ctx, const uint8_t* file, uint64_t size,
+                                 const rhd_arw_layout* layout, rhd_mosaic** out, float* gpu_ms);
+/* Copies a resident mosaic back (row_stride * height samples), for tests. */
+rhd_status rhd_mosaic_read(rhd_context* ctx, const rhd_mosaic* mosaic, uint16_t* dst,
+                           uint64_t count);
 
 /* NoiseProfile (crates/detail/src/noise.rs:24-34): variance a*max(x,0) + b;
  * bins = 0 for the flat fallback. */

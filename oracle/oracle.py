@@ -359,3 +359,34 @@ def detail_stage(rgb: np.ndarray, *, false_colour=0, chroma=0.0, luminance=0.0,
     profiles = ([(float(prof[3 * k]), float(prof[3 * k + 1]), int(prof[3 * k + 2]))
                  for k in range(3)] if have.value else None)
     return st, v, profiles
+
+
+def decode_arw2(strip: bytes, width: int, height: int, curve) -> tuple[int, np.ndarray]:
+    """Sony compressed ARW (rawler 0.7.2's published algorithm): u16 mosaic."""
+    buf = np.frombuffer(bytes(strip), dtype=np.uint8).copy()
+    out = np.zeros((height, width), np.uint16)
+    c = np.asarray(curve, dtype=np.uint16)
+    st = lib().rhdo_decode_arw2(_p(buf), _sz(buf.size), _sz(width), _sz(height), _p(c), _p(out))
+    return st, out
+
+
+def arw_curve(curve):
+    """(table, base, delta) of the tone curve and its dithering lookup."""
+    c = np.asarray(curve, dtype=np.uint16)
+    t, b, d = (np.zeros(4096, np.uint16) for _ in range(3))
+    lib().rhdo_arw_curve(_p(c), _p(t), _p(b), _p(d))
+    return t, b, d
+
+
+def arw_probe(data: bytes):
+    """(status, layout dict or None); status 1 not TIFF, 2 corrupt, 3 no raw IFD."""
+    buf = np.frombuffer(bytes(data), dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros(12, np.uint64)
+    st = lib().rhdo_arw_probe(_p(buf), _sz(len(data)), _p(out))
+    if st:
+        return st, None
+    keys = ["width", "height", "bits_per_sample", "compression", "offset", "bytes"]
+    d = {k: int(out[i]) for i, k in enumerate(keys)}
+    d["curve"] = [int(v) for v in out[6:10]]
+    d["has_curve"], d["kind"] = int(out[10]), int(out[11])
+    return 0, d

@@ -6,6 +6,7 @@
 // algorithm in double precision (the FP64 twin).
 #include <cstring>
 
+#include "arw.hpp"
 #include "capture.hpp"
 #include "color.hpp"
 #include "demosaic.hpp"
@@ -351,6 +352,39 @@ int rhdo_detail_stage(float* rgb, size_t width, size_t height, int false_colour,
                                     iterations, nullptr, nullptr, nullptr, 1)))
         return 40 + st;
     return 0;
+}
+
+// Sony compressed ARW strip -> u16 mosaic (width x height).
+int rhdo_decode_arw2(const uint8_t* strip, size_t bytes, size_t width, size_t height,
+                     const uint16_t curve[4], uint16_t* out) {
+    return arw::decode_arw2(strip, bytes, width, height, curve, out);
+}
+
+// The tone curve table (4096 entries) and its dithering lookup.
+int rhdo_arw_curve(const uint16_t curve[4], uint16_t table[4096], uint16_t base[4096],
+                   uint16_t delta[4096]) {
+    const std::vector<uint16_t> t = arw::curve_table(curve);
+    const arw::Lookup l = arw::lookup(t);
+    for (size_t i = 0; i < 4096; ++i) {
+        table[i] = t[i];
+        base[i] = l.base[i];
+        delta[i] = l.delta[i];
+    }
+    return 0;
+}
+
+// out = {width, height, bits_per_sample, compression, offset, bytes, curve[4],
+// has_curve, kind}; returns 0, 1 (not TIFF), 2 (corrupt), 3 (no raw IFD).
+int rhdo_arw_probe(const uint8_t* file, size_t size, uint64_t out[12]) {
+    arw::Layout l;
+    const int st = arw::probe(file, size, l);
+    if (st == 0) {
+        const uint64_t v[12] = {l.width, l.height, l.bits_per_sample, l.compression, l.offset,
+                                l.bytes, l.curve[0], l.curve[1], l.curve[2], l.curve[3],
+                                (uint64_t)l.has_curve, (uint64_t)l.kind};
+        std::memcpy(out, v, sizeof(v));
+    }
+    return st;
 }
 
 }  // extern "C"

@@ -12,10 +12,12 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "librhd.so"
 
 OK, CANCELLED, UNSUPPORTED, OVER_BUDGET, INVALID_DIMENSIONS, BASE_MISMATCH, \
-    UNSUPPORTED_EDITS, SYNCHRONIZATION, INVALID_INPUT, READBACK = range(10)
+    UNSUPPORTED_EDITS, SYNCHRONIZATION, INVALID_INPUT, READBACK, RAW_UNSUPPORTED, \
+    RAW_CORRUPT = range(12)
 STATUS_NAMES = ["Ok", "Cancelled", "Unsupported", "OverBudget", "InvalidDimensions",
                 "BaseMismatch", "UnsupportedEdits", "Synchronization", "InvalidInput",
-                "Readback"]
+                "Readback", "RawUnsupported", "RawCorrupt"]
+ARW_NONE, ARW_COMPRESSED, ARW_UNCOMPRESSED16 = range(3)
 
 
 class SensorDesc(ctypes.Structure):
@@ -39,6 +41,12 @@ class DetailDesc(ctypes.Structure):
         ("false_colour", c_int32), ("balance", c_float * 3), ("nr_chroma", c_float),
         ("nr_luminance", c_float), ("nr_luminance_detail", c_float),
     ]
+
+
+class ArwLayout(ctypes.Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("bits_per_sample", c_uint32),
+                ("compression", c_uint32), ("offset", c_uint64), ("bytes", c_uint64),
+                ("curve", ctypes.c_uint16 * 4), ("kind", c_int32)]
 
 
 class NoiseProfile(ctypes.Structure):
@@ -143,6 +151,10 @@ SIGNATURES = {
     "rhd_optics_full": (c_int, [_P, _P, POINTER(OpticsDesc), _P, POINTER(_P), POINTER(c_float)]),
     "rhd_color_transform": (c_int, [_P, _P, POINTER(c_float), POINTER(c_float), _P,
                                     POINTER(c_float)]),
+    "rhd_arw_probe": (c_int, [_P, c_uint64, POINTER(ArwLayout)]),
+    "rhd_mosaic_decode_arw": (c_int, [_P, _P, c_uint64, POINTER(ArwLayout), POINTER(_P),
+                                      POINTER(c_float)]),
+    "rhd_mosaic_read": (c_int, [_P, _P, _P, c_uint64]),
     "rhd_noise_profiles": (c_int, [_P, _P, POINTER(c_float), POINTER(NoiseProfile),
                                    POINTER(c_int32), POINTER(c_float)]),
     "rhd_denoise_chroma": (c_int, [_P, _P, c_float, POINTER(c_float), POINTER(c_float),

@@ -189,6 +189,7 @@ void drop_scratch(rhd_context* ctx, Lane& L) {
     free_buffer(ctx, L.capture);
     free_buffer(ctx, L.opposed);
     free_buffer(ctx, L.optics);
+    free_buffer(ctx, L.raw);
     L.optics_rows = Lane::RowsKey{};
 }
 
@@ -974,6 +975,8 @@ const char* rhd_status_name(rhd_status s) {
         case RHD_SYNCHRONIZATION: return "Synchronization";
         case RHD_INVALID_INPUT: return "InvalidInput";
         case RHD_READBACK: return "Readback";
+        case RHD_RAW_UNSUPPORTED: return "RawUnsupported";
+        case RHD_RAW_CORRUPT: return "RawCorrupt";
     }
     return "Unknown";
 }

@@ -33,7 +33,7 @@ struct Lane {
     hipStream_t stream = nullptr;
     DeviceStats* d_stats = nullptr;
     DeviceStats* h_stats = nullptr;
-    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed, optics;
+    DeviceBuffer amaze, amaze_areas, rcd, capture, opposed, optics, raw;
     hipEvent_t ev[5] = {};
     bool profiling = false;
     Profiler profiler;
@@ -116,7 +116,7 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out);
 rhd_status device_alloc(rhd_context* ctx, Lane* L, uint64_t bytes, void** out);
 void device_free(rhd_context* ctx, void* p, uint64_t bytes);
 bool capture_active(const rhd_detail_desc* dd);
-// Any detail-stage step to run (false colour or capture sharpening).
+// Any detail-stage step to run (noise reduction, false colour or capture).
 bool detail_active(const rhd_detail_desc* dd);
 bool noise_active(const rhd_detail_desc* dd);
 // Lane scratch the detail stage needs for a w x h frame.

@@ -333,6 +333,26 @@ class ResidentMosaic(_Handle):
     """A decoded RawFrame mosaic kept on the GPU."""
     _release = "rhd_mosaic_release"
 
+    def read(self, width: int, height: int, row_stride: Optional[int] = None) -> np.ndarray:
+        stride = row_stride or width
+        out = np.empty((height, stride), dtype=np.uint16)
+        self.proc._check(self.proc.lib.rhd_mosaic_read(
+            self.proc.ctx, self.ptr, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out[:, :width]
+
+
+def arw_probe(data: bytes) -> "_lib.ArwLayout":
+    """Finds the raw strip of a Sony ARW (TIFF) file: its size, compression,
+    offset and tone-curve knots. Raises GpuPreviewError with RawUnsupported or
+    RawCorrupt (RawError::Unsupported / Corrupt of crates/raw/src/lib.rs)."""
+    lib = _lib.load()
+    layout = _lib.ArwLayout()
+    buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+    st = lib.rhd_arw_probe(buf.ctypes.data_as(ctypes.c_void_p), len(data), ctypes.byref(layout))
+    if st != _lib.OK:
+        raise GpuPreviewError(st, lib.rhd_status_name(st).decode())
+    return layout
+
 
 class GpuMosaic(_Handle):
     """GpuNormalizedMosaic / GpuHighlightedMosaic."""
@@ -416,6 +436,18 @@ class GpuSensorProcessor:
                                                frame.row_stride, frame.info.width, h,
                                                ctypes.byref(ptr)))
         return ResidentMosaic(self, ptr)
+
+    def decode_arw(self, data: bytes, layout: Optional["_lib.ArwLayout"] = None):
+        """Uploads an ARW file's raw strip and decodes it on the device into a
+        resident mosaic (row stride = width). Returns (mosaic, gpu_ms)."""
+        layout = layout if layout is not None else arw_probe(data)
+        buf = np.frombuffer(data, dtype=np.uint8)
+        ptr = ctypes.c_void_p()
+        ms = ctypes.c_float()
+        self._check(self.lib.rhd_mosaic_decode_arw(self.ctx, buf.ctypes.data_as(ctypes.c_void_p),
+                                                   len(data), ctypes.byref(layout),
+                                                   ctypes.byref(ptr), ctypes.byref(ms)))
+        return ResidentMosaic(self, ptr), float(ms.value)
 
     def normalize(self, mosaic: ResidentMosaic, desc: SensorDevelopmentDescription,
                   cancellation: Optional[CancellationToken] = None, metrics=None) -> GpuMosaic:

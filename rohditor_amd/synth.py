@@ -135,3 +135,112 @@ def cpu_case() -> SceneSpec:
 def full_sensor_24mp() -> SceneSpec:
     """6048x4024 RGGB, 14-bit, 2 % clipped rectangles plus 5 % saturated blobs."""
     return SceneSpec(6048, 4024, 0x52480002, blob_fraction=0.05)
+
+
+# Sony compressed ARW (TIFF compression 32767): tone-curve knots of the tag
+# 0x7010 (each value >> 2 is a 12-bit knot) whose table ends at 15352.
+SONY_CURVE = (4096, 8192, 12288, 16380)
+
+
+def sony_curve_table(curve=SONY_CURVE) -> np.ndarray:
+    """The 4096-entry tone curve the four knots define (segment i climbs 2^i)."""
+    knots = [0] + [(int(v) >> 2) & 0xFFF for v in curve] + [4095]
+    out = np.zeros(4096, np.int64)
+    for i in range(5):
+        for j in range(knots[i] + 1, knots[i + 1] + 1):
+            out[j] = out[j - 1] + (1 << i)
+    return out.astype(np.uint16)
+
+
+def sony_codes(samples: np.ndarray, curve=SONY_CURVE) -> np.ndarray:
+    """11-bit codes whose curve value (at index code * 2) is nearest below or
+    at each sample."""
+    table = sony_curve_table(curve)[0::2].astype(np.int64)
+    idx = np.searchsorted(table, samples.astype(np.int64), side="right") - 1
+    return np.clip(idx, 0, 2047).astype(np.uint16)
+
+
+def encode_arw2(codes: np.ndarray) -> bytes:
+    """Packs (H, W) 11-bit codes, W a multiple of 32, into Sony's compressed
+    rows: per 32 pixels two 16-byte blocks (even columns, then odd), each an
+    11-bit max, 11-bit min, 4-bit positions of both and fourteen 7-bit deltas
+    scaled by 2^max(0, bit_length(max - min) - 7), read LSB first."""
+    h, w = codes.shape
+    if w % 32:
+        raise ValueError("width must be a multiple of 32")
+    v = codes.astype(np.int64).reshape(h, w // 32, 16, 2).transpose(0, 1, 3, 2).reshape(-1, 16)
+    vmax, vmin = v.max(1), v.min(1)
+    imax = v.argmax(1)
+    imin = v.argmin(1)
+    same = imax == imin  # all 16 equal: any two distinct positions
+    imax = np.where(same, 0, imax)
+    imin = np.where(same, 1, imin)
+    delta = vmax - vmin
+    bl = np.zeros_like(delta)
+    d = delta.copy()
+    while np.any(d):
+        bl += d > 0
+        d >>= 1
+    shift = np.maximum(bl - 7, 0)
+    q = np.clip((v - vmin[:, None]) >> shift[:, None], 0, 127)
+    bits = np.zeros((v.shape[0], 128), np.uint8)
+
+    def put(start, width, value):
+        for k in range(width):
+            bits[:, start + k] = (value >> k) & 1
+
+    put(0, 11, vmax)
+    put(11, 11, vmin)
+    put(22, 4, imax)
+    put(26, 4, imin)
+    pos = np.full(v.shape[0], 30)
+    rows = np.arange(v.shape[0])
+    for i in range(16):
+        keep = (imax != i) & (imin != i)
+        for k in range(7):
+            bits[rows[keep], pos[keep] + k] = (q[keep, i] >> k) & 1
+        pos = pos + 7 * keep
+    return np.packbits(bits, axis=1, bitorder="little").tobytes()
+
+
+def arw_file(strip: bytes, width: int, height: int, curve=SONY_CURVE, compression: int = 32767,
+             bits_per_sample: int = 14) -> bytes:
+    """A minimal little-endian ARW-like TIFF: IFD0 (a tiny placeholder) whose
+    SubIFD holds the raw strip with its width, height, bits per sample,
+    compression, strip offset and byte count, and the Sony curve (0x7010)."""
+    import struct
+
+    def ifd(entries, at, next_ifd=0):
+        # entries: (tag, type, count, value bytes); values over 4 bytes go after.
+        out = struct.pack("<H", len(entries))
+        extra = b""
+        data_at = at + 2 + 12 * len(entries) + 4
+        for tag, typ, count, val in entries:
+            if len(val) <= 4:
+                out += struct.pack("<HHI", tag, typ, count) + val.ljust(4, b"\0")
+            else:
+                out += struct.pack("<HHII", tag, typ, count, data_at + len(extra))
+                extra += val + (b"\0" if len(val) % 2 else b"")
+        return out + struct.pack("<I", next_ifd) + extra
+
+    ifd0_at = 8
+    ifd0_size = 2 + 12 * 2 + 4
+    raw_at = ifd0_at + ifd0_size
+    raw_entries = [
+        (0x0100, 4, 1, struct.pack("<I", width)),
+        (0x0101, 4, 1, struct.pack("<I", height)),
+        (0x0102, 3, 1, struct.pack("<H", bits_per_sample)),
+        (0x0103, 3, 1, struct.pack("<H", compression)),
+        (0x0111, 4, 1, b"\0\0\0\0"),
+        (0x0117, 4, 1, struct.pack("<I", len(strip))),
+        (0x7010, 3, 4, struct.pack("<4H", *curve)),
+    ]
+    raw_size = len(ifd(raw_entries, raw_at))
+    strip_at = raw_at + raw_size
+    strip_at += (-strip_at) % 16
+    raw_entries[4] = (0x0111, 4, 1, struct.pack("<I", strip_at))
+    raw = ifd(raw_entries, raw_at)
+    ifd0 = ifd([(0x00FE, 4, 1, struct.pack("<I", 1)), (0x014A, 4, 1, struct.pack("<I", raw_at))],
+               ifd0_at)
+    head = b"II*\0" + struct.pack("<I", ifd0_at) + ifd0 + raw
+    return head + b"\0" * (strip_at - len(head)) + strip
