@@ -166,13 +166,22 @@ def workload():
     return spec, frame, recipe, options
 
 
-def cpu_baseline(frame, recipe, options):
-    """The oracle's restatement of the same chain on one full frame."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_chain(frame, recipe, options, threads):
+    """Seconds for the oracle's normalize + Opposed + AMaZE + RL on `frame`."""
     from oracle import oracle
     from rohditor_amd import sensor
     desc = sensor.SensorDevelopmentDescription.from_frame(frame, recipe, options)
     c = desc.crop
-    threads = oracle.cpu_threads()
     t0 = time.perf_counter()
     norm = oracle.normalize(frame.mosaic, desc.sensor_pattern, desc.black, *desc.black_repeat,
                             desc.white, (c.x, c.y, c.width, c.height))
@@ -184,11 +193,33 @@ def cpu_baseline(frame, recipe, options):
     oracle.capture(rgb, recipe.capture.amount, recipe.capture.radius,
                    recipe.capture.noise_protection, ceil, iterations=recipe.capture_iterations,
                    threads=threads)
-    dt = time.perf_counter() - t0
-    mp = c.width * c.height / 1e6
+    return time.perf_counter() - t0, c.width * c.height / 1e6
+
+
+def cpu_baseline(frame, recipe, options):
+    """The oracle's restatement of the same chain: the full frame on every
+    host core (1 warm-up, median of 5), and a 1536x1024 crop of it on one
+    core (median of 3)."""
+    from oracle import oracle
+    from rohditor_amd import synth
+    threads = oracle.cpu_threads()
+    _cpu_chain(frame, recipe, options, threads)
+    runs = [_cpu_chain(frame, recipe, options, threads) for _ in range(5)]
+    dt = float(np.median([r[0] for r in runs]))
+    mp = runs[0][1]
+    crop = synth.frame_from_mosaic(np.ascontiguousarray(frame.mosaic[:1024, :1536]),
+                                   frame.info.cfa, float(frame.info.black_levels.values[0]),
+                                   float(frame.info.white_levels[0]))
+    single = [_cpu_chain(crop, recipe, options, 1) for _ in range(3)]
+    dt1 = float(np.median([r[0] for r in single]))
     return {"value": round(mp / dt, 3), "unit": "MP/s", "cores": threads, "kind": "port",
-            "sample": f"one full {c.width}x{c.height} frame, normalize+Opposed+AMaZE+RL(10), "
-                      f"{dt:.2f} s"}
+            "sample": f"one full {frame.info.width}x{frame.info.height} frame, "
+                      f"normalize+Opposed+AMaZE+RL({recipe.capture_iterations}), median of 5 "
+                      f"after 1 warm-up: {dt:.2f} s",
+            "single_core": {"value": round(single[0][1] / dt1, 3), "unit": "MP/s", "cores": 1,
+                            "sample": f"{crop.info.width}x{crop.info.height} crop of the frame, "
+                                      f"median of 3: {dt1:.2f} s"},
+            "cpu_model": _cpu_model()}
 
 
 def main():
