@@ -422,6 +422,10 @@ void rhd_server_destroy(rhd_server* server);
 /* Makes a decoded frame resident under `frame_id` (replacing a previous one). */
 rhd_status rhd_server_put_mosaic(rhd_server* server, uint64_t frame_id, const uint16_t* host,
                                  uint64_t row_stride, uint32_t width, uint32_t height);
+/* The same from an ARW file: the raw strip found by rhd_arw_probe is uploaded
+ * and decoded on the device (rhd_mosaic_decode_arw). */
+rhd_status rhd_server_put_arw(rhd_server* server, uint64_t frame_id, const uint8_t* file,
+                              uint64_t size, const rhd_arw_layout* layout);
 rhd_status rhd_server_drop_mosaic(rhd_server* server, uint64_t frame_id);
 /* Queues a develop of a resident frame; the descriptions are copied. */
 rhd_status rhd_server_submit(rhd_server* server, uint64_t frame_id, const rhd_sensor_desc* desc,

@@ -164,6 +164,7 @@ SIGNATURES = {
     "rhd_server_create": (c_int, [_P, POINTER(ServerConfig), POINTER(_P)]),
     "rhd_server_destroy": (None, [_P]),
     "rhd_server_put_mosaic": (c_int, [_P, c_uint64, _P, c_uint64, c_uint32, c_uint32]),
+    "rhd_server_put_arw": (c_int, [_P, c_uint64, _P, c_uint64, POINTER(ArwLayout)]),
     "rhd_server_drop_mosaic": (c_int, [_P, c_uint64]),
     "rhd_server_submit": (c_int, [_P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc),
                                   POINTER(c_uint64)]),

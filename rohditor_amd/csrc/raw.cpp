@@ -73,6 +73,91 @@ hipError_t launch_arw2(const uint8_t* strip, uint64_t bytes, int width, int heig
 
 }  // namespace rhd
 
+namespace rhd {
+
+// Uploads the raw strip of `l` and decodes it on lane L into a new mosaic.
+rhd_status decode_arw_on(rhd_context* ctx, Lane& L, const uint8_t* file, uint64_t size,
+                         const rhd_arw_layout* l, rhd_mosaic** out, float* gpu_ms) {
+    *out = nullptr;
+    if (gpu_ms) *gpu_ms = 0.0f;
+    if (l->width == 0 || l->height == 0 || l->width > (1u << 20) || l->height > 65535)
+        return fail(ctx, RHD_INVALID_DIMENSIONS, "raw dimensions out of range");
+    if (l->offset > size || l->bytes > size - l->offset)
+        return fail(ctx, RHD_RAW_CORRUPT, "raw strip outside the file");
+    const uint64_t px = (uint64_t)l->width * l->height;
+    if (l->kind == RHD_ARW_UNCOMPRESSED16) {
+        if (l->bytes != 2 * px || (l->offset & 1))
+            return fail(ctx, RHD_RAW_CORRUPT, "uncompressed strip size does not match");
+        (void)hipSetDevice(ctx->device);
+        return mosaic_upload(ctx, L, (const uint16_t*)(file + l->offset), l->width, l->width,
+                             l->height, out);
+    }
+    if (l->kind != RHD_ARW_COMPRESSED) return fail(ctx, RHD_RAW_UNSUPPORTED, "unknown raw kind");
+    if (l->bytes != px) return fail(ctx, RHD_RAW_CORRUPT, "compressed strip size does not match");
+
+    // Tone curve (4096 entries, segment i climbs 2^i) and its dithering
+    // entries (base | delta << 16); the stream's jump per 16-pixel block.
+    uint32_t knots[6] = {0, 0, 0, 0, 0, 4095};
+    for (int i = 0; i < 4; ++i) knots[i + 1] = (l->curve[i] >> 2) & 0xfff;
+    std::vector<uint16_t> curve(4096, 0);
+    for (int i = 0; i < 5; ++i)
+        for (uint32_t j = knots[i] + 1; j <= knots[i + 1]; ++j)
+            curve[j] = (uint16_t)(curve[j - 1] + (1u << i));
+    const int blocks = (int)(l->width / 32) * 2;
+    std::vector<uint32_t> tables(4096 + (size_t)blocks);
+    for (int i = 0; i < 4096; ++i) {
+        const uint16_t c = curve[i];
+        const uint16_t lo = i > 0 ? curve[i - 1] : c, up = i < 4095 ? curve[i + 1] : c;
+        const uint16_t spread = (uint16_t)(up - lo);
+        const uint16_t base = c == 0 ? 0 : (uint16_t)(c - (uint16_t)((uint16_t)(spread + 2) / 4));
+        tables[i] = (uint32_t)base | (uint32_t)spread << 16;
+    }
+    const uint64_t m = 15700ull * 65536ull - 1ull;
+    uint64_t step16 = 1;
+    for (int k = 0; k < 16; ++k) step16 = step16 * 15700ull % m;
+    uint64_t jump = 1;
+    for (int b = 0; b < blocks; ++b) {
+        tables[4096 + b] = (uint32_t)jump;
+        jump = jump * step16 % m;
+    }
+
+    (void)hipSetDevice(ctx->device);
+    const uint64_t strip_bytes = (l->bytes + 255) & ~255ull;
+    const uint64_t table_bytes = (tables.size() * sizeof(uint32_t) + 255) & ~255ull;
+    rhd_status st =
+        ensure_scratch(ctx, L, L.raw, strip_bytes + table_bytes + 4ull * l->height);
+    if (st != RHD_OK) return st;
+    void* mosaic = nullptr;
+    if ((st = device_alloc(ctx, &L, px * sizeof(uint16_t), &mosaic)) != RHD_OK) return st;
+    uint8_t* d_strip = (uint8_t*)L.raw.ptr;
+    uint32_t* d_tables = (uint32_t*)(d_strip + strip_bytes);
+    uint32_t* d_first_bad = (uint32_t*)(d_strip + strip_bytes + table_bytes);
+    hipError_t e = hipMemcpyAsync(d_strip, file + l->offset, l->bytes, hipMemcpyHostToDevice,
+                                  L.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_tables, tables.data(), tables.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess) e = hipEventRecord(L.ev[3], L.stream);
+    if (e == hipSuccess)
+        e = launch_arw2(d_strip, l->bytes, (int)l->width, (int)l->height, d_tables,
+                        d_tables + 4096, d_first_bad, (uint16_t*)mosaic, L.stream, PROF(L));
+    if (e == hipSuccess) e = hipEventRecord(L.ev[4], L.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(L.stream);
+        device_free(ctx, mosaic, px * sizeof(uint16_t));
+        return hip_fail(ctx, e, "compressed ARW decode");
+    }
+    if (gpu_ms) *gpu_ms = elapsed(L.ev[3], L.ev[4]);
+    ctx->refs.fetch_add(1);
+    *out = new rhd_mosaic{ctx, (uint16_t*)mosaic, l->width, l->width, l->height,
+                          px * sizeof(uint16_t)};
+    return RHD_OK;
+}
+
+
+}  // namespace rhd
+
 using namespace rhd;
 
 extern "C" {
@@ -141,81 +226,7 @@ rhd_status rhd_arw_probe(const uint8_t* file, uint64_t size, rhd_arw_layout* out
 rhd_status rhd_mosaic_decode_arw(rhd_context* ctx, const uint8_t* file, uint64_t size,
                                  const rhd_arw_layout* l, rhd_mosaic** out, float* gpu_ms) {
     if (!ctx || !file || !l || !out) return RHD_INVALID_INPUT;
-    *out = nullptr;
-    if (gpu_ms) *gpu_ms = 0.0f;
-    if (l->width == 0 || l->height == 0 || l->width > (1u << 20) || l->height > 65535)
-        return fail(ctx, RHD_INVALID_DIMENSIONS, "raw dimensions out of range");
-    if (l->offset > size || l->bytes > size - l->offset)
-        return fail(ctx, RHD_RAW_CORRUPT, "raw strip outside the file");
-    const uint64_t px = (uint64_t)l->width * l->height;
-    if (l->kind == RHD_ARW_UNCOMPRESSED16) {
-        if (l->bytes != 2 * px || (l->offset & 1))
-            return fail(ctx, RHD_RAW_CORRUPT, "uncompressed strip size does not match");
-        return rhd_mosaic_upload(ctx, (const uint16_t*)(file + l->offset), l->width, l->width,
-                                 l->height, out);
-    }
-    if (l->kind != RHD_ARW_COMPRESSED) return fail(ctx, RHD_RAW_UNSUPPORTED, "unknown raw kind");
-    if (l->bytes != px) return fail(ctx, RHD_RAW_CORRUPT, "compressed strip size does not match");
-
-    // Tone curve (4096 entries, segment i climbs 2^i) and its dithering
-    // entries (base | delta << 16); the stream's jump per 16-pixel block.
-    uint32_t knots[6] = {0, 0, 0, 0, 0, 4095};
-    for (int i = 0; i < 4; ++i) knots[i + 1] = (l->curve[i] >> 2) & 0xfff;
-    std::vector<uint16_t> curve(4096, 0);
-    for (int i = 0; i < 5; ++i)
-        for (uint32_t j = knots[i] + 1; j <= knots[i + 1]; ++j)
-            curve[j] = (uint16_t)(curve[j - 1] + (1u << i));
-    const int blocks = (int)(l->width / 32) * 2;
-    std::vector<uint32_t> tables(4096 + (size_t)blocks);
-    for (int i = 0; i < 4096; ++i) {
-        const uint16_t c = curve[i];
-        const uint16_t lo = i > 0 ? curve[i - 1] : c, up = i < 4095 ? curve[i + 1] : c;
-        const uint16_t spread = (uint16_t)(up - lo);
-        const uint16_t base = c == 0 ? 0 : (uint16_t)(c - (uint16_t)((uint16_t)(spread + 2) / 4));
-        tables[i] = (uint32_t)base | (uint32_t)spread << 16;
-    }
-    const uint64_t m = 15700ull * 65536ull - 1ull;
-    uint64_t step16 = 1;
-    for (int k = 0; k < 16; ++k) step16 = step16 * 15700ull % m;
-    uint64_t jump = 1;
-    for (int b = 0; b < blocks; ++b) {
-        tables[4096 + b] = (uint32_t)jump;
-        jump = jump * step16 % m;
-    }
-
-    Lane& L = ctx->lanes[0];
-    (void)hipSetDevice(ctx->device);
-    const uint64_t strip_bytes = (l->bytes + 255) & ~255ull;
-    const uint64_t table_bytes = (tables.size() * sizeof(uint32_t) + 255) & ~255ull;
-    rhd_status st =
-        ensure_scratch(ctx, L, L.raw, strip_bytes + table_bytes + 4ull * l->height);
-    if (st != RHD_OK) return st;
-    void* mosaic = nullptr;
-    if ((st = device_alloc(ctx, &L, px * sizeof(uint16_t), &mosaic)) != RHD_OK) return st;
-    uint8_t* d_strip = (uint8_t*)L.raw.ptr;
-    uint32_t* d_tables = (uint32_t*)(d_strip + strip_bytes);
-    uint32_t* d_first_bad = (uint32_t*)(d_strip + strip_bytes + table_bytes);
-    hipError_t e = hipMemcpyAsync(d_strip, file + l->offset, l->bytes, hipMemcpyHostToDevice,
-                                  L.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_tables, tables.data(), tables.size() * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, L.stream);
-    if (e == hipSuccess) e = hipEventRecord(L.ev[3], L.stream);
-    if (e == hipSuccess)
-        e = launch_arw2(d_strip, l->bytes, (int)l->width, (int)l->height, d_tables,
-                        d_tables + 4096, d_first_bad, (uint16_t*)mosaic, L.stream, PROF(L));
-    if (e == hipSuccess) e = hipEventRecord(L.ev[4], L.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
-    if (e != hipSuccess) {
-        (void)hipStreamSynchronize(L.stream);
-        device_free(ctx, mosaic, px * sizeof(uint16_t));
-        return hip_fail(ctx, e, "compressed ARW decode");
-    }
-    if (gpu_ms) *gpu_ms = elapsed(L.ev[3], L.ev[4]);
-    ctx->refs.fetch_add(1);
-    *out = new rhd_mosaic{ctx, (uint16_t*)mosaic, l->width, l->width, l->height,
-                          px * sizeof(uint16_t)};
-    return RHD_OK;
+    return decode_arw_on(ctx, ctx->lanes[0], file, size, l, out, gpu_ms);
 }
 
 rhd_status rhd_mosaic_read(rhd_context* ctx, const rhd_mosaic* m, uint16_t* dst, uint64_t count) {

@@ -121,6 +121,9 @@ bool detail_active(const rhd_detail_desc* dd);
 bool noise_active(const rhd_detail_desc* dd);
 // Lane scratch the detail stage needs for a w x h frame.
 uint64_t detail_scratch_bytes(int w, int h, const rhd_detail_desc* dd);
+// Uploads an ARW file's raw strip and decodes it on lane L (raw.cpp).
+rhd_status decode_arw_on(rhd_context* ctx, Lane& L, const uint8_t* file, uint64_t size,
+                         const rhd_arw_layout* l, rhd_mosaic** out, float* gpu_ms);
 // False colour then capture sharpening on `src` (in place).
 rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
 rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd);

@@ -388,6 +388,10 @@ rhd_status submit(rhd_server* s, uint64_t frame_id, const rhd_sensor_desc* d,
 
 }  // namespace
 
+namespace rhd {
+rhd_status install_mosaic(rhd_server* s, uint64_t frame_id, rhd_mosaic* m);
+}  // namespace rhd
+
 extern "C" {
 
 rhd_status rhd_server_create(rhd_context* ctx, const rhd_server_config* cfg, rhd_server** out) {
@@ -451,6 +455,29 @@ rhd_status rhd_server_put_mosaic(rhd_server* s, uint64_t frame_id, const uint16_
         st = mosaic_upload(s->ctx, s->ctx->lanes[UPLOAD_LANE], host, row_stride, width, height, &m);
     }
     if (st != RHD_OK) return st;
+    return install_mosaic(s, frame_id, m);
+}
+
+rhd_status rhd_server_put_arw(rhd_server* s, uint64_t frame_id, const uint8_t* file,
+                              uint64_t size, const rhd_arw_layout* layout) {
+    if (!s || !file || !layout) return RHD_INVALID_INPUT;
+    rhd_mosaic* m = nullptr;
+    rhd_status st;
+    {
+        std::lock_guard<std::mutex> g(s->upload_mu);
+        st = decode_arw_on(s->ctx, s->ctx->lanes[UPLOAD_LANE], file, size, layout, &m, nullptr);
+    }
+    if (st != RHD_OK) return st;
+    return install_mosaic(s, frame_id, m);
+}
+
+}  // extern "C"
+
+namespace rhd {
+
+// Publishes `m` as frame `frame_id`, replacing an idle previous mosaic and
+// dropping the bases developed from it.
+rhd_status install_mosaic(rhd_server* s, uint64_t frame_id, rhd_mosaic* m) {
     rhd_mosaic* old = nullptr;
     {
         std::lock_guard<std::mutex> g(s->mu);
@@ -477,6 +504,10 @@ rhd_status rhd_server_put_mosaic(rhd_server* s, uint64_t frame_id, const uint16_
     if (old) rhd_mosaic_release(old);
     return RHD_OK;
 }
+
+}  // namespace rhd
+
+extern "C" {
 
 rhd_status rhd_server_drop_mosaic(rhd_server* s, uint64_t frame_id) {
     if (!s) return RHD_INVALID_INPUT;

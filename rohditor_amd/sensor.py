@@ -672,6 +672,14 @@ class GpuServer:
             self.ptr, frame_id, buf.ctypes.data_as(ctypes.c_void_p), frame.row_stride,
             frame.info.width, h))
 
+    def put_arw(self, frame_id: int, data: bytes, layout=None) -> None:
+        """Makes an ARW file's frame resident, decoded on the device."""
+        layout = layout if layout is not None else arw_probe(data)
+        buf = np.frombuffer(data, dtype=np.uint8)
+        self.proc._check(self.proc.lib.rhd_server_put_arw(
+            self.ptr, frame_id, buf.ctypes.data_as(ctypes.c_void_p), len(data),
+            ctypes.byref(layout)))
+
     def drop_frame(self, frame_id: int) -> None:
         self.proc._check(self.proc.lib.rhd_server_drop_mosaic(self.ptr, frame_id))
 

@@ -100,3 +100,34 @@ def test_damaged_layouts_are_rejected(gpu):
     assert gpu.reserved_bytes() <= before + (1 << 20)
     m, _ = gpu.decode_arw(data, layout)
     m.release()
+
+
+def test_server_takes_arw_frames(gpu):
+    """rhd_server_put_arw: a served develop of a frame made resident from its
+    ARW file equals the plain develop of the decoded samples."""
+    spec = synth.SceneSpec(width=1024, height=768, seed=0xA4A2, cfa="RGGB")
+    codes = synth.sony_codes(synth.mosaic(spec))
+    data = synth.arw_file(synth.encode_arw2(codes), spec.width, spec.height)
+    resident, _ = gpu.decode_arw(data)
+    frame = synth.frame_from_mosaic(resident.read(spec.width, spec.height), "RGGB")
+    recipe = sensor.EditRecipe(capture=sensor.CaptureSharpening(enabled=True))
+    desc = sensor.SensorDevelopmentDescription.from_frame(frame, recipe, sensor.RenderOptions())
+    detail = sensor.detail_desc(recipe, desc.demosaic_balance)
+    src = gpu.develop_resident(resident, desc, detail)
+    exp = src.read_rgb()
+    src.release()
+    resident.release()
+    proc = sensor.GpuSensorProcessor(0)
+    server = sensor.GpuServer(proc, lanes=1)
+    server.put_arw(7, data)
+    r = server.wait(server.submit(7, desc, detail))
+    assert r.ok
+    np.testing.assert_array_equal(r.source.read_rgb(), exp)
+    r.source.release()
+    with pytest.raises(sensor.GpuPreviewError) as e:
+        bad = sensor.arw_probe(data)
+        bad.bytes -= 1
+        server.put_arw(8, data, bad)
+    assert e.value.status == _lib.RAW_CORRUPT
+    server.close()
+    proc.close()
