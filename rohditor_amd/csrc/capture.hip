@@ -10,7 +10,7 @@
 //
 // Kernels: init (guide, highlight mask), two single blurs with the mask
 // epilogues, one fused kernel per RL iteration (blur estimate -> ratio ->
-// blur ratio -> clamped update, all in LDS for a 64x32 output tile with a
+// blur ratio -> clamped update, all in LDS for a 64x16 output tile with a
 // 2r halo), and the gain apply. Iterations ping-pong between two estimate
 // planes. HBM-bound: per iteration the estimate (with halo) and the guide are
 // read and the estimate written.
@@ -19,7 +19,7 @@
 namespace rhd {
 namespace {
 
-constexpr int TW = 64, TH = 32, MAXR = 4, THREADS = 256;
+constexpr int TW = 64, TH = 16, MAXR = 4, THREADS = 256;
 constexpr float FLOOR = 1.0e-6f;
 
 struct Weights {
@@ -97,28 +97,42 @@ __device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, 
 // A wavefront stages whole rows, so each row's address is wave-uniform and
 // formed in scalar registers.
 template <int MAXROWS, int MAXCOLS>
-__device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ src,
-                                      size_t src_pitch, int rows, int cols) {
-    constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
-    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
+struct Staged {
+    static constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
     float v[RI * CI];
+
+    __device__ __forceinline__ void load(const float* __restrict__ src, size_t src_pitch, int rows,
+                                         int cols) {
+        const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < RI; ++i) {
-        const int r = ty + 4 * i;
-        const float* row = src + (size_t)r * src_pitch;
+        for (int i = 0; i < RI; ++i) {
+            const int r = ty + 4 * i;
+            const float* row = src + (size_t)r * src_pitch;
 #pragma unroll
-        for (int j = 0; j < CI; ++j) {
-            const int c = tx + 64 * j;
-            v[i * CI + j] = (r < rows && c < cols) ? row[c] : 0.0f;
+            for (int j = 0; j < CI; ++j) {
+                const int c = tx + 64 * j;
+                v[i * CI + j] = (r < rows && c < cols) ? row[c] : 0.0f;
+            }
         }
     }
+    __device__ __forceinline__ void store(float* lds, int pitch, int rows, int cols) const {
+        const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < RI; ++i)
+        for (int i = 0; i < RI; ++i)
 #pragma unroll
-        for (int j = 0; j < CI; ++j) {
-            const int r = ty + 4 * i, c = tx + 64 * j;
-            if (r < rows && c < cols) lds[r * pitch + c] = v[i * CI + j];
-        }
+            for (int j = 0; j < CI; ++j) {
+                const int r = ty + 4 * i, c = tx + 64 * j;
+                if (r < rows && c < cols) lds[r * pitch + c] = v[i * CI + j];
+            }
+    }
+};
+
+template <int MAXROWS, int MAXCOLS>
+__device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ src,
+                                      size_t src_pitch, int rows, int cols) {
+    Staged<MAXROWS, MAXCOLS> w;
+    w.load(src, src_pitch, rows, cols);
+    w.store(lds, pitch, rows, cols);
 }
 
 // One blur of `in` with an epilogue:
@@ -171,12 +185,13 @@ __global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in
         blur_tile<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
 }
 
-// One RL iteration (sharpening.rs:268-304) for a 64x32 tile: blur(estimate)
+// One RL iteration (sharpening.rs:268-304) for a 64x16 tile: blur(estimate)
 // over the tile widened by r, ratio = guide / max(b, floor), blur(ratio) over
-// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide). The LDS
-// windows are sized for the radius: the estimate with a 2r halo, its
-// horizontal blur, and the guide with an r halo, which the ratio overwrites
-// in place (the final clamp reads the guide from global memory).
+// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide). Two LDS
+// arrays sized for the radius: the first holds the estimate with a 2r halo
+// and, once its horizontal blur is done, the guide with an r halo, which the
+// ratio overwrites in place; the second holds the horizontal blurs. Sharing
+// the first array keeps eight workgroups (eight waves per SIMD) resident.
 template <int R>
 struct IterWindows {
     static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;  // estimate
@@ -187,7 +202,7 @@ struct IterWindows {
 template <int R, bool FAST>
 __device__ void iterate_tile(const float* __restrict__ est_in, const float* __restrict__ guide,
                              float* __restrict__ est_out, int width, int height, const Weights& k,
-                             float* s_e, float* s_h, float* s_g) {
+                             float* s_a, float* s_b) {
     using Win = IterWindows<R>;
     constexpr int EW = Win::EW, HW = Win::HW, GW = Win::GW;
     const uint3 blk = xcd_block();
@@ -199,24 +214,29 @@ __device__ void iterate_tile(const float* __restrict__ est_in, const float* __re
     const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
     const int ew = ex1 - ex0, eh = ey1 - ey0, rw = rx1 - rx0, rh = ry1 - ry0;
     const int ow = ox1 - x0, oh = oy1 - y0;
-    stage<Win::EH, Win::EW>(s_e, EW, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
-    stage<Win::GH, Win::GW>(s_g, GW, guide + (size_t)ry0 * width + rx0, width, rh, rw);
+    // The guide window waits in registers while the estimate window, which
+    // shares its LDS, is blurred.
+    Staged<Win::GH, Win::GW> g;
+    g.load(guide + (size_t)ry0 * width + rx0, width, rh, rw);
+    stage<Win::EH, Win::EW>(s_a, EW, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
     __syncthreads();
-    FOR_WINDOW(eh, rw, r, c) s_h[r * HW + c] = hblur<R, FAST>(s_e + r * EW, ex0, rx0 + c, width, k);
+    FOR_WINDOW(eh, rw, r, c) s_b[r * HW + c] = hblur<R, FAST>(s_a + r * EW, ex0, rx0 + c, width, k);
+    __syncthreads();
+    g.store(s_a, GW, rh, rw);
     __syncthreads();
     FOR_WINDOW(rh, rw, r, c) {
-        const float b = vblur<R, FAST>(s_h, HW, ey0, ry0 + r, c, height, k);
-        s_g[r * GW + c] = s_g[r * GW + c] / num_max(b, FLOOR);
+        const float b = vblur<R, FAST>(s_b, HW, ey0, ry0 + r, c, height, k);
+        s_a[r * GW + c] = s_a[r * GW + c] / num_max(b, FLOOR);
     }
     __syncthreads();
-    FOR_WINDOW(rh, ow, r, c) s_h[r * HW + c] = hblur<R, FAST>(s_g + r * GW, rx0, x0 + c, width, k);
+    FOR_WINDOW(rh, ow, r, c) s_b[r * HW + c] = hblur<R, FAST>(s_a + r * GW, rx0, x0 + c, width, k);
     __syncthreads();
     FOR_WINDOW(oh, ow, r, c) {
         const int y = y0 + r, x = x0 + c;
-        const float b = vblur<R, FAST>(s_h, HW, ry0, y, c, height, k);
+        const float b = vblur<R, FAST>(s_b, HW, ry0, y, c, height, k);
         const size_t i = (size_t)y * width + x;
         const float g = guide[i];
-        const float v = s_e[(y - ey0) * EW + (x - ex0)] * b;
+        const float v = est_in[i] * b;
         const float lo = g * 0.5f, hi = g * 2.0f;
         est_out[i] = v < lo ? lo : (v > hi ? hi : v);
     }
@@ -239,6 +259,8 @@ struct InteriorWindows {
     static constexpr int E_FLOATS = EH * EW + 8;
     static constexpr int H_FLOATS = (EH + 4) * P;
     static constexpr int G_FLOATS = GH * P;
+    // The estimate window, then the guide / ratio window, share one array.
+    static constexpr int A_FLOATS = E_FLOATS > G_FLOATS ? E_FLOATS : G_FLOATS;
     static constexpr int CHUNKS = (4 + 2 * R + 3) / 4;      // float4 reads per strip
 };
 
@@ -284,7 +306,7 @@ __device__ __forceinline__ void vstrip(const float* src, const Weights& k, float
 template <int R>
 __device__ void iterate_interior(const float* __restrict__ est_in, const float* __restrict__ guide,
                                  float* __restrict__ est_out, int width, const Weights& k,
-                                 float* s_e, float* s_h, float* s_g) {
+                                 float* s_a, float* s_b) {
     using Win = InteriorWindows<R>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
     constexpr int NS1 = (GW + 3) / 4;   // column strips of the estimate's blur
@@ -293,35 +315,48 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
     constexpr int NR4 = TH / 4;         // row strips of the output
     constexpr int OUT_STRIPS = NR4 * TW / THREADS;
     static_assert(TW == 64, "output strips map one wavefront to one tile row");
+    static_assert(OUT_STRIPS >= 1, "every thread owns at least one output strip");
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    stage<EH, EW>(s_e, EW, est_in + (size_t)(y0 - 2 * R) * width + x0 - 2 * R, width, EH, EW);
-    stage<GH, GW>(s_g, P, guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
+    // The guide window waits in registers while the estimate window, which
+    // shares its LDS, is blurred.
+    Staged<GH, GW> gw;
+    gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
+    stage<EH, EW>(s_a, EW, est_in + (size_t)(y0 - 2 * R) * width + x0 - 2 * R, width, EH, EW);
     __syncthreads();
-    float g_out[OUT_STRIPS][4];
+    float e_out[OUT_STRIPS][4], g_out[OUT_STRIPS][4];
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) g_out[q][i] = s_g[(r0 + i + R) * P + c + R];
+        for (int i = 0; i < 4; ++i) e_out[q][i] = s_a[(r0 + i + 2 * R) * EW + c + 2 * R];
     }
     for (int item = threadIdx.x; item < EH * NS1; item += THREADS) {
         const int r = item / NS1, j = item % NS1;
-        *reinterpret_cast<float4*>(s_h + r * P + 4 * j) = hstrip<R>(s_e + r * EW + 4 * j, k);
+        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * EW + 4 * j, k);
+    }
+    __syncthreads();
+    gw.store(s_a, P, GH, GW);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < OUT_STRIPS; ++q) {
+        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + R];
     }
     __syncthreads();
     for (int item = threadIdx.x; item < NR2 * GW; item += THREADS) {
         const int c = item % GW, r0 = 4 * (item / GW);
         float b[4];
-        vstrip<R, P>(s_h + r0 * P + c, k, b);
+        vstrip<R, P>(s_b + r0 * P + c, k, b);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            if (r0 + i < GH) s_g[(r0 + i) * P + c] = s_g[(r0 + i) * P + c] / num_max(b[i], FLOOR);
+            if (r0 + i < GH) s_a[(r0 + i) * P + c] = s_a[(r0 + i) * P + c] / num_max(b[i], FLOOR);
     }
     __syncthreads();
     for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
         const int r = item / NS3, j = item % NS3;
-        *reinterpret_cast<float4*>(s_h + r * P + 4 * j) = hstrip<R>(s_g + r * P + 4 * j, k);
+        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
     }
     __syncthreads();
 #pragma unroll
@@ -330,13 +365,12 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
         const int c = threadIdx.x & 63;
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
         float b[4];
-        vstrip<R, P>(s_h + r0 * P + c, k, b);
+        vstrip<R, P>(s_b + r0 * P + c, k, b);
         float* out_row = est_out + (size_t)(y0 + r0) * width + x0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int r = r0 + i;
             const float g = g_out[q][i];
-            const float v = s_e[(r + 2 * R) * EW + c + 2 * R] * b[i];
+            const float v = e_out[q][i] * b[i];
             const float lo = g * 0.5f, hi = g * 2.0f;
             out_row[(size_t)i * width + c] = v < lo ? lo : (v > hi ? hi : v);
         }
@@ -350,20 +384,19 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
                                                        int height, Weights k) {
     using Win = IterWindows<R>;
     using IWin = InteriorWindows<R>;
-    __shared__ __align__(16) float s_e[IWin::E_FLOATS];
-    __shared__ __align__(16) float s_h[IWin::H_FLOATS];
-    __shared__ __align__(16) float s_g[IWin::G_FLOATS];
-    static_assert(IWin::E_FLOATS >= Win::EH * Win::EW && IWin::H_FLOATS >= Win::EH * Win::HW &&
-                      IWin::G_FLOATS >= Win::GH * Win::GW,
+    __shared__ __align__(16) float s_a[IWin::A_FLOATS];
+    __shared__ __align__(16) float s_b[IWin::H_FLOATS];
+    static_assert(IWin::A_FLOATS >= Win::EH * Win::EW && IWin::A_FLOATS >= Win::GH * Win::GW &&
+                      IWin::H_FLOATS >= Win::EH * Win::HW,
                   "the border path's windows fit the interior path's arrays");
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool interior =
         x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
     if (interior)
-        iterate_interior<R>(est_in, guide, est_out, width, k, s_e, s_h, s_g);
+        iterate_interior<R>(est_in, guide, est_out, width, k, s_a, s_b);
     else
-        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_e, s_h, s_g);
+        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_a, s_b);
 }
 
 // sharpening.rs:105-126
