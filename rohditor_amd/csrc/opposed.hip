@@ -22,8 +22,9 @@
 //   scan       ring cells compacted in raster order, plus each region's
 //              first/last ring index
 //   contrib    every ring cell's nine correction samples, in parallel
-//   sums       one wavefront per group (0 = image-wide pool, r = region r),
-//              lane s owning slot s: both passes over the group's ring
+//   sums       one workgroup per group (0 = image-wide pool, r = region r),
+//              four wavefronts staging the ring entries and lane s of a fifth
+//              owning slot s: both passes over the group's ring
 //              entries in raster order, accumulated one f64 add at a time, so
 //              the sums are bit-identical to the reference's
 //   sites      one thread per (cell, channel) with a suspected site: the
@@ -516,12 +517,13 @@ __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float
 // The per-sample correction of a ring cell does not depend on which
 // accumulator receives it, so opp_contrib computes every ring cell's nine
 // (channel, support set) samples once, in parallel. opp_sums then gives each
-// group (0 = the image-wide pool, r = region r) one wavefront. The wavefront
-// stages SUM_BATCH consecutive ring entries at a time in LDS, with the next
-// batch's loads in flight, and lane s walks the staged entries in raster order
-// for slot s, one f64 add at a time, exactly as the reference's sequential
-// loop does. Entries that do not count add +0.0, which leaves an accumulator
-// that starts at +0.0 unchanged, so the adds run without branches.
+// group (0 = the image-wide pool, r = region r) one workgroup. Producer
+// wavefronts stage SUM_BATCH consecutive ring entries at a time in LDS, with
+// the next batch's loads in flight, and lane s of the consumer wavefront walks
+// the staged entries in raster order for slot s, one f64 add at a time,
+// exactly as the reference's sequential loop does. Entries that do not count
+// add +0.0, which leaves an accumulator that starts at +0.0 unchanged, so the
+// adds run without branches.
 __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float l1, float l2) {
     const float lv[3] = {l0, l1, l2};
     const unsigned ring = L.scalars[1];
@@ -559,113 +561,103 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
 }
 
 constexpr int SUM_BATCH = 256;
-constexpr int SUM_PER_LANE = SUM_BATCH / 64;
+constexpr int SUM_PRODUCERS = SUM_BATCH / 64;
+constexpr int SUM_THREADS = 64 * (1 + SUM_PRODUCERS);
 constexpr int SUM_UNROLL = 8;
 
-// One batch of ring entries in registers, entry q * 64 + lane of the batch in
-// slot q: its slot flags (zero when it belongs to another region), nine
-// samples and three ratios.
-struct SumBatch {
-    unsigned fl[SUM_PER_LANE];
-    float val[SUM_PER_LANE][SLOTS];
-    float rat[SUM_PER_LANE][3];
+// One ring entry in a producer lane's registers: its slot flags (zero when it
+// belongs to another region), nine samples and three ratios.
+struct SumEntry {
+    unsigned fl;
+    float val[SLOTS];
+    float rat[3];
 };
 
-__device__ __forceinline__ void sum_load(const Layout& L, unsigned g, unsigned base,
-                                         unsigned end, int lane, SumBatch& b) {
+__device__ __forceinline__ void sum_load(const Layout& L, unsigned g, unsigned k, unsigned end,
+                                         SumEntry& b) {
+    const bool in = k < end;
+    const unsigned lab = in ? L.ring_label[k] : 0u;
+    const unsigned f = in ? L.contrib_flags[k] : 0u;
 #pragma unroll
-    for (int q = 0; q < SUM_PER_LANE; ++q) {
-        const unsigned k = base + q * 64 + lane;
-        const bool in = k < end;
-        const unsigned lab = in ? L.ring_label[k] : 0u;
-        const unsigned f = in ? L.contrib_flags[k] : 0u;
+    for (int s = 0; s < SLOTS; ++s) b.val[s] = in ? L.contrib_value[(size_t)k * SLOTS + s] : 0.0f;
 #pragma unroll
-        for (int s = 0; s < SLOTS; ++s)
-            b.val[q][s] = in ? L.contrib_value[(size_t)k * SLOTS + s] : 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) b.rat[q][c] = in ? L.contrib_ratio[(size_t)k * 3 + c] : 0.0f;
-        b.fl[q] = in && (g == 0 || lab == g) ? f : 0u;
-    }
+    for (int c = 0; c < 3; ++c) b.rat[c] = in ? L.contrib_ratio[(size_t)k * 3 + c] : 0.0f;
+    b.fl = in && (g == 0 || lab == g) ? f : 0u;
 }
 
-// Staging buffers of one batch: per slot the exact f64 terms it adds, the
-// number of contributing entries, and whether any entry of the batch belongs
-// to the group.
+// Staging buffers of one batch: per slot the exact f64 terms it adds, and per
+// producer wavefront the number of contributing entries of each slot and
+// whether any of its entries belongs to the group.
 // Slot rows are SUM_PITCH doubles apart (2 more than a batch) so that the
 // nine lanes reading them start in different LDS banks.
 constexpr int SUM_PITCH = SUM_BATCH + 2;
 struct SumStage {
     double a[SLOTS * SUM_PITCH];
     double b[SLOTS * SUM_PITCH];
-    unsigned count[SLOTS];
-    unsigned staged;
+    unsigned count[SUM_PRODUCERS][SLOTS];
+    unsigned staged[SUM_PRODUCERS];
 };
 
-// One pass of every slot of one group over ring entries [begin, end), by two
-// wavefronts. Wave 1 loads batch i (the next batch's loads in flight behind
-// it) and turns it into the terms each slot adds: +0.0 for an entry that does
-// not count, which leaves an accumulator that starts at +0.0 unchanged, and
-// ballot counts. Meanwhile lane s of wave 0 adds slot s's terms of batch i - 1
-// in raster order, one f64 add at a time, as the reference's sequential loop
-// does. The two stage buffers alternate.
+// One pass of every slot of one group over ring entries [begin, end). Waves
+// 1..SUM_PRODUCERS each load a quarter of batch i (their share of the next
+// batch's loads in flight behind it) and turn it into the terms each slot
+// adds: +0.0 for an entry that does not count, which leaves an accumulator
+// that starts at +0.0 unchanged, and ballot counts. Meanwhile lane s of wave 0
+// adds slot s's terms of batch i - 1 in raster order, one f64 add at a time,
+// as the reference's sequential loop does. The two stage buffers alternate.
 template <int PASS>
 __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned begin,
                                          unsigned end, const float* means, const float* tols,
                                          SumStage* st, unsigned& count, double& accum,
                                          double& second) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = wave - 1, e = q * 64 + lane;
     const unsigned batches = begin < end ? (end - begin + SUM_BATCH - 1) / SUM_BATCH : 0u;
-    SumBatch b;
-    if (wave == 1 && batches != 0) sum_load(L, g, begin, end, lane, b);
+    SumEntry b;
+    if (wave != 0 && batches != 0) sum_load(L, g, begin + e, end, b);
     for (unsigned i = 0; i <= batches; ++i) {
-        if (wave == 1 && i < batches) {
+        if (wave != 0 && i < batches) {
             SumStage& out = st[i & 1];
             const unsigned base = begin + i * SUM_BATCH;
-            unsigned mine = 0;
+            // Every producer writes its terms, so a batch that another
+            // producer stages never holds stale ones.
+            unsigned counts[SLOTS];
 #pragma unroll
-            for (int q = 0; q < SUM_PER_LANE; ++q) mine |= b.fl[q];
-            const bool staged = __ballot(mine != 0) != 0;
-            if (staged) {
-                unsigned counts[SLOTS];
-#pragma unroll
-                for (int t = 0; t < SLOTS; ++t) counts[t] = 0;
-#pragma unroll
-                for (int q = 0; q < SUM_PER_LANE; ++q) {
-                    const int e = q * 64 + lane;
-                    const unsigned f = b.fl[q];
-#pragma unroll
-                    for (int t = 0; t < SLOTS; ++t) {
-                        const float v = b.val[q][t];
-                        bool take = (f >> t) & 1u;
-                        double ta, tb;
-                        if (PASS == 0) {
-                            ta = take ? (double)v : 0.0;
-                            tb = ta * ta;
-                        } else {
-                            take = take && fabsf(v - means[t]) <= tols[t];
-                            const bool ratio = t % 3 == 2 && ((f >> (SLOTS + t / 3)) & 1u);
-                            ta = take ? (double)v : 0.0;
-                            tb = take && ratio ? (double)b.rat[q][t / 3] : 0.0;
-                        }
-                        out.a[t * SUM_PITCH + e] = ta;
-                        out.b[t * SUM_PITCH + e] = tb;
-                        counts[t] += __popcll(__ballot(take));
-                    }
+            for (int t = 0; t < SLOTS; ++t) {
+                const float v = b.val[t];
+                bool take = (b.fl >> t) & 1u;
+                double ta, tb;
+                if (PASS == 0) {
+                    ta = take ? (double)v : 0.0;
+                    tb = ta * ta;
+                } else {
+                    take = take && fabsf(v - means[t]) <= tols[t];
+                    const bool ratio = t % 3 == 2 && ((b.fl >> (SLOTS + t / 3)) & 1u);
+                    ta = take ? (double)v : 0.0;
+                    tb = take && ratio ? (double)b.rat[t / 3] : 0.0;
                 }
-                if (lane < SLOTS) {
-                    unsigned c = 0;
-#pragma unroll
-                    for (int t = 0; t < SLOTS; ++t) c = lane == t ? counts[t] : c;
-                    out.count[lane] = c;
-                }
+                out.a[t * SUM_PITCH + e] = ta;
+                out.b[t * SUM_PITCH + e] = tb;
+                counts[t] = __popcll(__ballot(take));
             }
-            if (lane == 0) out.staged = staged;
-            if (base + SUM_BATCH < end) sum_load(L, g, base + SUM_BATCH, end, lane, b);
+            if (lane < SLOTS) {
+                unsigned c = 0;
+#pragma unroll
+                for (int t = 0; t < SLOTS; ++t) c = lane == t ? counts[t] : c;
+                out.count[q][lane] = c;
+            }
+            const bool staged = __ballot(b.fl != 0) != 0;
+            if (lane == 0) out.staged[q] = staged;
+            if (base + SUM_BATCH < end) sum_load(L, g, base + SUM_BATCH + e, end, b);
         }
         if (wave == 0 && i >= 1 && lane < SLOTS) {
             const SumStage& in = st[(i - 1) & 1];
-            if (in.staged) {
-                count += in.count[lane];
+            unsigned staged = 0;
+#pragma unroll
+            for (int p = 0; p < SUM_PRODUCERS; ++p) staged |= in.staged[p];
+            if (staged) {
+#pragma unroll
+                for (int p = 0; p < SUM_PRODUCERS; ++p) count += in.count[p][lane];
                 const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_PITCH);
                 const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_PITCH);
                 for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
@@ -689,9 +681,10 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
     }
 }
 
-// One workgroup of two wavefronts per group (0 = the image-wide pool, r =
-// region r); lane s of wave 0 owns slot s (region_correction.rs:196-261).
-__global__ __launch_bounds__(128) void opp_sums(Layout L) {
+// One workgroup (a consumer wavefront and SUM_PRODUCERS producers) per group
+// (0 = the image-wide pool, r = region r); lane s of wave 0 owns slot s
+// (region_correction.rs:196-261).
+__global__ __launch_bounds__(SUM_THREADS) void opp_sums(Layout L) {
     __shared__ __align__(16) SumStage s_stage[2];
     __shared__ float s_mean[SLOTS], s_tol[SLOTS];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1107,7 +1100,7 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     RHD_TIMED(prof, s, "opp_contrib", 0,
               hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
     RHD_TIMED(prof, s, "opp_sums", 0,
-              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(128), 0, s, L));
+              hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(SUM_THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_sites", 8 * px + 16 * cells,
               hipLaunchKernelGGL(opp_sites, dim3((L.cw + SX - 1) / SX, (L.ch + SY - 1) / SY, 3),
                                  dim3(THREADS), 0, s, a.mosaic, L, a.pattern, l0, l1, l2,
