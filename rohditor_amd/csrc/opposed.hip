@@ -563,7 +563,8 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
 constexpr int SUM_BATCH = 256;
 constexpr int SUM_PRODUCERS = SUM_BATCH / 64;
 constexpr int SUM_THREADS = 64 * (1 + SUM_PRODUCERS);
-constexpr int SUM_UNROLL = 8;
+constexpr int SUM_UNROLL = 4;
+static_assert((SUM_BATCH / 2) % (2 * SUM_UNROLL) == 0, "whole register groups per batch");
 
 // One ring entry in a producer lane's registers: its slot flags (zero when it
 // belongs to another region), nine samples and three ratios.
@@ -660,19 +661,42 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
                 for (int p = 0; p < SUM_PRODUCERS; ++p) count += in.count[p][lane];
                 const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_PITCH);
                 const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_PITCH);
-                for (int j = 0; j < SUM_BATCH / 2; j += SUM_UNROLL) {
-                    double2 a[SUM_UNROLL], c2[SUM_UNROLL];
+                // Two register groups alternate: the next group's LDS reads
+                // are in flight while the current group's terms are added.
+                double2 xa[SUM_UNROLL], xb[SUM_UNROLL], ya[SUM_UNROLL], yb[SUM_UNROLL];
+#pragma unroll
+                for (int u = 0; u < SUM_UNROLL; ++u) {
+                    xa[u] = pa[u];
+                    xb[u] = pb[u];
+                }
+                for (int j = 0; j < SUM_BATCH / 2; j += 2 * SUM_UNROLL) {
 #pragma unroll
                     for (int u = 0; u < SUM_UNROLL; ++u) {
-                        a[u] = pa[j + u];
-                        c2[u] = pb[j + u];
+                        ya[u] = pa[j + SUM_UNROLL + u];
+                        yb[u] = pb[j + SUM_UNROLL + u];
                     }
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < SUM_UNROLL; ++u) {
-                        accum += a[u].x;
-                        second += c2[u].x;
-                        accum += a[u].y;
-                        second += c2[u].y;
+                        accum += xa[u].x;
+                        second += xb[u].x;
+                        accum += xa[u].y;
+                        second += xb[u].y;
+                    }
+                    if (j + 2 * SUM_UNROLL < SUM_BATCH / 2) {
+#pragma unroll
+                        for (int u = 0; u < SUM_UNROLL; ++u) {
+                            xa[u] = pa[j + 2 * SUM_UNROLL + u];
+                            xb[u] = pb[j + 2 * SUM_UNROLL + u];
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < SUM_UNROLL; ++u) {
+                        accum += ya[u].x;
+                        second += yb[u].x;
+                        accum += ya[u].y;
+                        second += yb[u].y;
                     }
                 }
             }
