@@ -709,8 +709,8 @@ __global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
     hv_prev[p] = hvwt[7 * TSH + p];
     pm_prev[p] = pmwt[9 * TSH + p];
     // Inputs of one row. Every value is final before the kernel starts except
-    // the current row's weights, which only this row's pass writes, so the
-    // next row's inputs are loaded while the current row is processed.
+    // the current row's weights, which only this row's pass writes, so later
+    // rows' inputs are loaded while the current row is processed.
     struct RowIn {
         float hv, hv_dl, hv_dr, vcd, hcd, cfa, cfa_l, cfa_r, cfa_u, cfa_d;
         float pm, pm_dl, pm_dr, rbm, rbp;
@@ -739,10 +739,20 @@ __global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
         in.rbp = rbp[h];
         return in;
     };
-    RowIn cur = load(8);
+    // CHAIN_AHEAD rows of inputs are in flight: row rr + CHAIN_AHEAD is
+    // requested as row rr is processed (re-reading row rr past the end).
+    constexpr int CHAIN_AHEAD = 8;
+    static_assert((TS - 16) % CHAIN_AHEAD == 0, "whole groups of rows");
+    RowIn buf[CHAIN_AHEAD];
+#pragma unroll
+    for (int k = 0; k < CHAIN_AHEAD; ++k) buf[k] = load(8 + k);
     __syncthreads();
-    for (int rr = 8; rr < TS - 8; ++rr) {
-        const RowIn next = load(rr + 1 < TS - 8 ? rr + 1 : rr);
+    for (int r0 = 8; r0 < TS - 8; r0 += CHAIN_AHEAD)
+#pragma unroll
+    for (int k = 0; k < CHAIN_AHEAD; ++k) {
+        const int rr = r0 + k;
+        const RowIn cur = buf[k];
+        buf[k] = load(rr + CHAIN_AHEAD < TS - 8 ? rr + CHAIN_AHEAD : rr);
         const int cc = 2 * p + gae(a.pattern, rr);
         const int h = rr * TSH + p;
         float hv = cur.hv;
@@ -780,7 +790,6 @@ __global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
         hv_prev[p] = hv;
         if (pm_row) pm_prev[p] = pm;
         __syncthreads();
-        cur = next;
     }
 }
 
