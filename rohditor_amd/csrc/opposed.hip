@@ -901,9 +901,9 @@ struct Site {
     }
 };
 
-// opposed.rs:97-162: one thread per (cell, channel) with a suspected site of
-// that channel (two cells per thread); the estimate is computed once and
-// applied to each such site. Blocks without a suspected cell of their channel
+// opposed.rs:97-162: one (cell, channel) with a suspected site of that
+// channel at a time; the estimate is computed once and applied to each such
+// site. Blocks without a suspected cell of their channel
 // leave before staging anything.
 __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
                                                      float l0, float l1, float l2, int has_ceiling,
@@ -913,6 +913,8 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     __shared__ float s_pref[WY * WX];
     __shared__ float s_plr[WY * WX];
     __shared__ int s_keys[MAX_LOCAL * THREADS];
+    __shared__ unsigned short s_task[SX * SY];
+    __shared__ int s_ntask;
     const float lv[3] = {l0, l1, l2};
     const float ceiling[3] = {c0, c1, c2};
     const uint3 blk = xcd_block();
@@ -948,18 +950,37 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
         s_pref[k] = pr;
         s_plr[k] = pl;
     }
+    if (threadIdx.x == 0) s_ntask = 0;
     __syncthreads();
+    // Cells whose suspected sites need an estimate (a supporting channel)
+    // are listed in LDS and taken densely below, so the ring searches run in
+    // full wavefronts; the others fall back at once.
     unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
-    for (int q = 0; q < ROWS_PER_THREAD; ++q) {
-        const int cx = bx0 + tx, cy = by0 + ty + 4 * q;
+    for (int pass = 0; pass < 2; ++pass) {
+    const int items = pass == 0 ? ROWS_PER_THREAD : (s_ntask + THREADS - 1 - (int)threadIdx.x) / THREADS;
+    for (int item = 0; item < items; ++item) {
+        int r, c;
+        if (pass == 0) {
+            r = ty + 4 * item;
+            c = tx;
+        } else {
+            const int code = s_task[threadIdx.x + item * THREADS];
+            r = code / SX;
+            c = code % SX;
+        }
+        const int cx = bx0 + c, cy = by0 + r;
         if (cx >= L.cw || cy >= L.ch) continue;
-        const int centre = (ty + 4 * q + SR) * WX + tx + SR;
+        const int centre = (r + SR) * WX + c + SR;
         const float4 target = s_cell[centre];
         const unsigned f = flags_of(target);
         if (!(f & (1u << (channel + 3)))) continue;
         int support_count = 0;
         for (int o = 0; o < 3; ++o)
             if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
+        if (pass == 0 && support_count != 0) {
+            s_task[atomicAdd(&s_ntask, 1)] = (unsigned short)(r * SX + c);
+            continue;
+        }
         float est = 0.0f;
         bool has = false;
         if (support_count != 0) {
@@ -997,6 +1018,8 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
                     neutralised += 1;
                 }
             }
+    }
+    __syncthreads();
     }
     // Per-block totals go to the block's own slot (no shared counters);
     // opp_site_stats adds them up.
