@@ -387,19 +387,7 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
 
 // ---------------------------------------------------------------- weights, Nyquist test, diagonals
 // green.rs:229-292 (direction_weights), nyquist.rs:25-89 (test and bounding
-// box), diagonal.rs:98-144 (diagonal_estimates). One thread per red/blue
-// site: cc = 2 * pair + green_at_even(row).
-__device__ __forceinline__ float even_kernel(const float* p, int rr, int cc) {
-    const float* up2 = p + (rr - 2) * TSH;
-    const float* up1 = p + (rr - 1) * TSH;
-    const float* mid = p + rr * TSH;
-    const float* down1 = p + (rr + 1) * TSH;
-    const float* down2 = p + (rr + 2) * TSH;
-    return GAUSSEVEN[0] * (up1[cc >> 1] + mid[(cc - 1) >> 1] + mid[(cc + 1) >> 1] + down1[cc >> 1]) +
-           GAUSSEVEN[1] * (up2[(cc - 1) >> 1] + up2[(cc + 1) >> 1] + up1[(cc - 2) >> 1] +
-                           up1[(cc + 2) >> 1] + down1[(cc - 2) >> 1] + down1[(cc + 2) >> 1] +
-                           down2[(cc - 1) >> 1] + down2[(cc + 1) >> 1]);
-}
+// box), diagonal.rs:98-144 (diagonal_estimates).
 __device__ __forceinline__ float diagonal_estimate(float centre, float near_, float far_) {
     float ratio = 2.0f * near_ / (EPS + centre + far_);
     return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : near_ + 0.5f * (centre - far_);
@@ -419,100 +407,52 @@ __device__ __forceinline__ float bound_towards(float estimate, float centre, flo
     return value > clip ? median3(value, a, b) : value;
 }
 
-// The three parts read disjoint planes, so they run as three kernels over
-// the red/blue sites: HV weights, the Nyquist test with the tile's bounding
-// box, and the diagonal estimates.
-struct Site {
-    int rr, cc, i;
+// One thread per red/blue site: cc = 2 * pair + green_at_even(row). A
+// wavefront holds 64 consecutive pairs of one row, so the horizontal taps of
+// a plane row come from one float2 load per lane (columns 2p, 2p + 1) and
+// the neighbouring lanes; only lanes at the wavefront's ends load theirs.
+// The arithmetic of each site is the reference's, term for term.
+struct Lanes {
+    int lane, base, par;  // base = 2 * pair
+    // Column cc + dx of a plane row of which this lane holds columns
+    // (base, base + 1) in v; `fetch` says whether this lane may read the row
+    // itself when the neighbour lies outside the wavefront.
+    __device__ __forceinline__ float tap(float2 v, const float* row, int dx, bool fetch) const {
+        const int o = par + dx;
+        const int po = o >> 1;  // pair offset (floor)
+        const float mine = (o & 1) ? v.y : v.x;
+        if (po == 0) return mine;
+        const int src = lane + po;
+        float r = __shfl(mine, src, 64);
+        if ((src < 0 || src > 63) && fetch) r = row[base + o];
+        return r;
+    }
+    // Half column pair + s of a half-plane row of which this lane holds
+    // half column `pair` in h.
+    __device__ __forceinline__ float half_tap(float h, const float* row, int s, bool fetch) const {
+        if (s == 0) return h;
+        const int src = lane + s;
+        float r = __shfl(h, src, 64);
+        if ((src < 0 || src > 63) && fetch) r = row[(base >> 1) + s];
+        return r;
+    }
 };
-__device__ __forceinline__ Site site_of(const AmazeArgs& a, const uint3& blk) {
-    const int p = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int cc = 2 * p + gae(a.pattern, rr);
-    return Site{rr, cc, rr * TS + cc};
+
+__device__ __forceinline__ float2 pair_load(const float* plane, int row, int base) {
+    return *reinterpret_cast<const float2*>(plane + row * TS + base);
 }
 
-__device__ __forceinline__ void part_hvwt(const AmazeArgs& a, const uint3& blk) {
-    
-    const Tile t = tile_of(a, blk.z);
-    const Site s = site_of(a, blk);
-    if (!(s.rr >= 6 && s.rr < TS - 6 && s.cc >= 6 && s.cc < TS - 6)) return;
-    const int i = s.i;
-    const float* __restrict__ vcd = t.full(VCD);
-    const float* __restrict__ hcd = t.full(HCD);
-    const float* __restrict__ dgv = t.full(DGV);
-    const float* __restrict__ dgh = t.full(DGH);
-    const float* __restrict__ w0 = t.full(DW0);
-    const float* __restrict__ w1 = t.full(DW1);
-    float v[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
-    float uave = v[3] + v[2] + v[1] + v[0];
-    float dave = v[3] + v[4] + v[5] + v[6];
-    float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
-    float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
-    float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
-    float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
-    float varl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) +
-                 sqr(hcd[i - 3] - lave);
-    float varr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) +
-                 sqr(hcd[i + 3] - rave);
-    float hwt = w1[i - 1] / (w1[i - 1] + w1[i + 1]);
-    float vwt = w0[i - TS] / (w0[i + TS] + w0[i - TS]);
-    float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
-    float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
-    float fluct_u = dgv[i] + dgv[i - TS] + dgv[i - 2 * TS];
-    float fluct_d = dgv[i] + dgv[i + TS] + dgv[i + 2 * TS];
-    float fluct_l = dgh[i] + dgh[i - 1] + dgh[i - 2];
-    float fluct_r = dgh[i] + dgh[i + 1] + dgh[i + 2];
-    float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
-    float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
-    float varwt = hcdvar / (vcdvar + hcdvar);
-    float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
-    t.half(HVWT)[i >> 1] =
-        ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
-            ? varwt
-            : diffwt;
-}
-
-// Nyquist test statistic (nyquist.rs:28-57) and the tile's bounding box of
-// flagged sites (:59-89).
-__device__ __forceinline__ void part_nyquist(const AmazeArgs& a, const uint3& blk) {
-    
-    const Tile t = tile_of(a, blk.z);
-    const Site s = site_of(a, blk);
-    bool flagged = false;
-    if (s.rr >= 6 && s.rr < TS - 6 && s.cc >= 6 && s.cc < TS - 6) {
-        const float* __restrict__ c = t.full(CDS) + s.i;
-        const float* __restrict__ d = t.full(DHV) + s.i;
-        float test =
-            (GAUSSODD[0] * c[0] +
-             GAUSSODD[1] * (c[-TS - 1] + c[-TS + 1] + c[TS - 1] + c[TS + 1]) +
-             GAUSSODD[2] * (c[-2 * TS] + c[-2] + c[2] + c[2 * TS]) +
-             GAUSSODD[3] * (c[-2 * TS - 2] + c[-2 * TS + 2] + c[2 * TS - 2] + c[2 * TS + 2])) -
-            (GAUSSGRAD[0] * d[0] + GAUSSGRAD[1] * (d[-TS] + d[1] + d[-1] + d[TS]) +
-             GAUSSGRAD[2] * (d[-TS - 1] + d[-TS + 1] + d[TS - 1] + d[TS + 1]) +
-             GAUSSGRAD[3] * (d[-2 * TS] + d[-2] + d[2] + d[2 * TS]) +
-             GAUSSGRAD[4] * (d[-2 * TS - 1] + d[-2 * TS + 1] + d[-TS - 2] + d[-TS + 2] +
-                             d[TS - 2] + d[TS + 2] + d[2 * TS - 1] + d[2 * TS + 1]) +
-             GAUSSGRAD[5] * (d[-2 * TS - 2] + d[-2 * TS + 2] + d[2 * TS - 2] + d[2 * TS + 2]));
-        if (test > 0.0f) {
-            t.nyq()[s.i >> 1] = 1;
-            flagged = true;
-        }
-    }
-    // A wavefront is one half row of the tile, with cc increasing by lane, so
-    // its flagged sites span [cc of the first flagged lane, cc of the last].
-    // Every wavefront writes its slot (empty: INT_MAX, -1, INT_MAX, -1);
-    // amz_box reduces the slots, so no two waves touch the same address.
-    const unsigned long long m = __ballot(flagged);
-    int4 box = make_int4(INT_MAX, -1, INT_MAX, -1);
-    if (m) {
-        const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
-        box = make_int4(s.rr, s.rr, __shfl(s.cc, first, 64), __shfl(s.cc, last, 64));
-    }
-    if ((threadIdx.x & 63) == 0)
-        reinterpret_cast<int4*>(t.boxes())[s.rr * 2 + blk.x] = box;
+// even_kernel over half-plane rows rr - 2 .. rr + 2 held one half column per
+// lane (e[0] .. e[4]); (cc >> 1) is the lane's own half column, ((cc + 1) >> 1)
+// is + par, ((cc - 1) >> 1) is + par - 1, ((cc + 2) >> 1) is + 1 and
+// ((cc - 2) >> 1) is - 1.
+__device__ __forceinline__ float even_lanes(const Lanes& L, const float* plane, int rr,
+                                            const float* e, bool fetch) {
+    auto at = [&](int m, int s) { return L.half_tap(e[m], plane + (rr + m - 2) * TSH, s, fetch); };
+    const int par = L.par;
+    return GAUSSEVEN[0] * (e[1] + at(2, par - 1) + at(2, par) + e[3]) +
+           GAUSSEVEN[1] * (at(0, par - 1) + at(0, par) + at(1, -1) + at(1, 1) + at(3, -1) +
+                           at(3, 1) + at(4, par - 1) + at(4, par));
 }
 
 // Reduces a tile's slots into its Nyquist box (min/max over flagged sites).
@@ -547,48 +487,172 @@ __global__ __launch_bounds__(256) void amz_box(AmazeArgs a) {
     }
 }
 
-// Diagonal estimates (diagonal.rs:98-144).
-__device__ __forceinline__ void part_diagonal(const AmazeArgs& a, const uint3& blk, float clip_pt) {
-    
-    const Tile t = tile_of(a, blk.z);
-    const Site s = site_of(a, blk);
-    const int rr = s.rr, cc = s.cc, i = s.i;
-    if (!(rr >= 8 && rr < TS - 8 && cc >= 8 && cc < TS - 8)) return;
-    const float* __restrict__ cfa = t.full(CFA);
-    const int h0 = rr * TSH, i1 = cc >> 1;
-    float centre = cfa[i];
-    float se1 = cfa[i + TS + 1], se2 = cfa[i + 2 * TS + 2];
-    float nw1 = cfa[i - TS - 1], nw2 = cfa[i - 2 * TS - 2];
-    float ne1 = cfa[i - TS + 1], ne2 = cfa[i - 2 * TS + 2];
-    float sw1 = cfa[i + TS - 1], sw2 = cfa[i + 2 * TS - 2];
-    float rbse = diagonal_estimate(centre, se1, se2);
-    float rbnw = diagonal_estimate(centre, nw1, nw2);
-    float rbne = diagonal_estimate(centre, ne1, ne2);
-    float rbsw = diagonal_estimate(centre, sw1, sw2);
-    const float* __restrict__ delm = t.half(DELM);
-    const float* __restrict__ delp = t.half(DELP);
-    float wtse = EPS + delm[h0 + i1] + delm[h0 + TSH + ((cc + 1) >> 1)] +
-                 delm[h0 + 2 * TSH + ((cc + 2) >> 1)];
-    float wtnw = EPS + delm[h0 + i1] + delm[h0 - TSH + ((cc - 1) >> 1)] +
-                 delm[h0 - 2 * TSH + ((cc - 2) >> 1)];
-    float wtne = EPS + delp[h0 + i1] + delp[h0 - TSH + ((cc + 1) >> 1)] +
-                 delp[h0 - 2 * TSH + ((cc + 2) >> 1)];
-    float wtsw = EPS + delp[h0 + i1] + delp[h0 + TSH + ((cc - 1) >> 1)] +
-                 delp[h0 + 2 * TSH + ((cc - 2) >> 1)];
-    float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
-    float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
-    float rbvarm = EPSSQ + even_kernel(t.half(SQ1M), rr, cc);
-    t.half(PMWT)[h0 + i1] = rbvarm / ((EPSSQ + even_kernel(t.half(SQ1P), rr, cc)) + rbvarm);
-    t.half(RBP)[h0 + i1] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
-    t.half(RBM)[h0 + i1] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
-}
-
 __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
     const uint3 blk = xcd_block();
-    part_hvwt(a, blk);
-    part_nyquist(a, blk);
-    part_diagonal(a, blk, clip_pt);
+    const Tile t = tile_of(a, blk.z);
+    const int lane = threadIdx.x & 63;
+    const int p = blk.x * 64 + lane;
+    const int rr = blk.y * 4 + (threadIdx.x >> 6);
+    const int par = gae(a.pattern, rr);
+    const int cc = 2 * p + par, i = rr * TS + cc;
+    const Lanes L{lane, 2 * p, par};
+    bool flagged = false;
+    // Whole wavefronts skip the rows outside [6, TS - 6); inside them every
+    // lane takes part in the exchanges and only valid sites fetch or store.
+    if (rr >= 6 && rr < TS - 6) {
+        const bool valid = cc >= 6 && cc < TS - 6;
+        // HV weights (green.rs:229-292).
+        {
+            const float* vcd = t.full(VCD);
+            const float* dgv = t.full(DGV);
+            const float* w0 = t.full(DW0);
+            const float* hcd_row = t.full(HCD) + rr * TS;
+            const float* w1_row = t.full(DW1) + rr * TS;
+            const float* dgh_row = t.full(DGH) + rr * TS;
+            const float2 HC = pair_load(t.full(HCD), rr, L.base);
+            const float2 W1 = pair_load(t.full(DW1), rr, L.base);
+            const float2 DH = pair_load(t.full(DGH), rr, L.base);
+            float v[7], dv[5];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) dv[k] = dgv[i + (k - 2) * TS];
+            const float w0u = w0[i - TS], w0d = w0[i + TS];
+            float hcd[7], dgh[5];
+#pragma unroll
+            for (int d = -3; d <= 3; ++d) hcd[d + 3] = L.tap(HC, hcd_row, d, valid);
+#pragma unroll
+            for (int d = -2; d <= 2; ++d) dgh[d + 2] = L.tap(DH, dgh_row, d, valid);
+            const float w1l = L.tap(W1, w1_row, -1, valid), w1r = L.tap(W1, w1_row, 1, valid);
+            if (valid) {
+                float uave = v[3] + v[2] + v[1] + v[0];
+                float dave = v[3] + v[4] + v[5] + v[6];
+                float lave = hcd[3] + hcd[2] + hcd[1] + hcd[0];
+                float rave = hcd[3] + hcd[4] + hcd[5] + hcd[6];
+                float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
+                float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
+                float varl = sqr(hcd[3] - lave) + sqr(hcd[2] - lave) + sqr(hcd[1] - lave) +
+                             sqr(hcd[0] - lave);
+                float varr = sqr(hcd[3] - rave) + sqr(hcd[4] - rave) + sqr(hcd[5] - rave) +
+                             sqr(hcd[6] - rave);
+                float hwt = w1l / (w1l + w1r);
+                float vwt = w0u / (w0d + w0u);
+                float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
+                float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
+                float fluct_u = dv[2] + dv[1] + dv[0];
+                float fluct_d = dv[2] + dv[3] + dv[4];
+                float fluct_l = dgh[2] + dgh[1] + dgh[0];
+                float fluct_r = dgh[2] + dgh[3] + dgh[4];
+                float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
+                float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
+                float varwt = hcdvar / (vcdvar + hcdvar);
+                float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
+                t.half(HVWT)[i >> 1] =
+                    ((0.5f - varwt) * (0.5f - diffwt) > 0.0f &&
+                     fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
+                        ? varwt
+                        : diffwt;
+            }
+        }
+        // Nyquist test statistic (nyquist.rs:28-57).
+        {
+            float2 C[5], D[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                C[m] = pair_load(t.full(CDS), rr + m - 2, L.base);
+                D[m] = pair_load(t.full(DHV), rr + m - 2, L.base);
+            }
+            auto c = [&](int dy, int dx) {
+                return L.tap(C[dy + 2], t.full(CDS) + (rr + dy) * TS, dx, valid);
+            };
+            auto d = [&](int dy, int dx) {
+                return L.tap(D[dy + 2], t.full(DHV) + (rr + dy) * TS, dx, valid);
+            };
+            float test =
+                (GAUSSODD[0] * c(0, 0) +
+                 GAUSSODD[1] * (c(-1, -1) + c(-1, 1) + c(1, -1) + c(1, 1)) +
+                 GAUSSODD[2] * (c(-2, 0) + c(0, -2) + c(0, 2) + c(2, 0)) +
+                 GAUSSODD[3] * (c(-2, -2) + c(-2, 2) + c(2, -2) + c(2, 2))) -
+                (GAUSSGRAD[0] * d(0, 0) + GAUSSGRAD[1] * (d(-1, 0) + d(0, 1) + d(0, -1) + d(1, 0)) +
+                 GAUSSGRAD[2] * (d(-1, -1) + d(-1, 1) + d(1, -1) + d(1, 1)) +
+                 GAUSSGRAD[3] * (d(-2, 0) + d(0, -2) + d(0, 2) + d(2, 0)) +
+                 GAUSSGRAD[4] * (d(-2, -1) + d(-2, 1) + d(-1, -2) + d(-1, 2) + d(1, -2) +
+                                 d(1, 2) + d(2, -1) + d(2, 1)) +
+                 GAUSSGRAD[5] * (d(-2, -2) + d(-2, 2) + d(2, -2) + d(2, 2)));
+            if (valid && test > 0.0f) {
+                t.nyq()[i >> 1] = 1;
+                flagged = true;
+            }
+        }
+        // Diagonal estimates (diagonal.rs:98-144).
+        if (rr >= 8 && rr < TS - 8) {
+            const bool valid8 = cc >= 8 && cc < TS - 8;
+            float2 F[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) F[m] = pair_load(t.full(CFA), rr + m - 2, L.base);
+            auto f = [&](int dy, int dx) {
+                return L.tap(F[dy + 2], t.full(CFA) + (rr + dy) * TS, dx, valid8);
+            };
+            const float centre = f(0, 0);
+            const float se1 = f(1, 1), se2 = f(2, 2);
+            const float nw1 = f(-1, -1), nw2 = f(-2, -2);
+            const float ne1 = f(-1, 1), ne2 = f(-2, 2);
+            const float sw1 = f(1, -1), sw2 = f(2, -2);
+            const int hp = rr * TSH + p;
+            float dm[5], dp[5], sm[5], sp[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                dm[m] = t.half(DELM)[hp + (m - 2) * TSH];
+                dp[m] = t.half(DELP)[hp + (m - 2) * TSH];
+                sm[m] = t.half(SQ1M)[hp + (m - 2) * TSH];
+                sp[m] = t.half(SQ1P)[hp + (m - 2) * TSH];
+            }
+            // ((cc + 1) >> 1) is half column p + par, ((cc - 1) >> 1) is
+            // p + par - 1, ((cc + 2) >> 1) is p + 1, ((cc - 2) >> 1) is p - 1.
+            auto hm = [&](int m, int s) {
+                return L.half_tap(dm[m], t.half(DELM) + (rr + m - 2) * TSH, s, valid8);
+            };
+            auto hq = [&](int m, int s) {
+                return L.half_tap(dp[m], t.half(DELP) + (rr + m - 2) * TSH, s, valid8);
+            };
+            const float dm_se1 = hm(3, par), dm_se2 = hm(4, 1);
+            const float dm_nw1 = hm(1, par - 1), dm_nw2 = hm(0, -1);
+            const float dp_ne1 = hq(1, par), dp_ne2 = hq(0, 1);
+            const float dp_sw1 = hq(3, par - 1), dp_sw2 = hq(4, -1);
+            const float rbvarm_even = even_lanes(L, t.half(SQ1M), rr, sm, valid8);
+            const float rbvarp_even = even_lanes(L, t.half(SQ1P), rr, sp, valid8);
+            if (valid8) {
+                float rbse = diagonal_estimate(centre, se1, se2);
+                float rbnw = diagonal_estimate(centre, nw1, nw2);
+                float rbne = diagonal_estimate(centre, ne1, ne2);
+                float rbsw = diagonal_estimate(centre, sw1, sw2);
+                float wtse = EPS + dm[2] + dm_se1 + dm_se2;
+                float wtnw = EPS + dm[2] + dm_nw1 + dm_nw2;
+                float wtne = EPS + dp[2] + dp_ne1 + dp_ne2;
+                float wtsw = EPS + dp[2] + dp_sw1 + dp_sw2;
+                float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
+                float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
+                float rbvarm = EPSSQ + rbvarm_even;
+                t.half(PMWT)[hp] = rbvarm / ((EPSSQ + rbvarp_even) + rbvarm);
+                t.half(RBP)[hp] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
+                t.half(RBM)[hp] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
+            }
+        }
+    }
+    // The tile's bounding box of flagged sites (nyquist.rs:59-89). A
+    // wavefront is one half row of the tile, with cc increasing by lane, so
+    // its flagged sites span [cc of the first flagged lane, cc of the last].
+    // Every wavefront writes its slot (empty: INT_MAX, -1, INT_MAX, -1);
+    // amz_box reduces the slots, so no two waves touch the same address.
+    const unsigned long long m = __ballot(flagged);
+    int4 box = make_int4(INT_MAX, -1, INT_MAX, -1);
+    if (m) {
+        const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
+        box = make_int4(rr, rr, __shfl(cc, first, 64), __shfl(cc, last, 64));
+    }
+    if (lane == 0) reinterpret_cast<int4*>(t.boxes())[rr * 2 + blk.x] = box;
 }
+
 
 // Nyquist area of a tile (nyquist.rs:59-89); false when there is none.
 struct Area {
