@@ -461,10 +461,15 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L, DeviceStats
 
 // regions.rs:44-55 (clipped channels per region) and region_correction.rs:
 // 125-187 (WantedSlots::scan).
+// Grid-stride: each workgroup walks REGION_STRIDE blocks of cells, so the
+// short per-cell work is not dominated by wavefront launches.
+constexpr int REGION_STRIDE = 32;
 __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float l1, float l2) {
-    int i = blockIdx.x * THREADS + threadIdx.x;
     const float lv[3] = {l0, l1, l2};
     const int lane = threadIdx.x & 63;
+    unsigned any = 0;
+    for (int i = blockIdx.x * THREADS + threadIdx.x; i - (int)threadIdx.x < L.cells;
+         i += gridDim.x * THREADS) {
     unsigned lab = 0, clipped = 0, bits = 0;
     bool has = false;
     if (i < L.cells) {
@@ -489,7 +494,6 @@ __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float
     // the wavefront and let one lane per distinct label touch memory; skip the
     // atomic when every bit is already set.
     unsigned long long active = __ballot(has);
-    unsigned any = 0;
     while (active) {
         const int leader = __ffsll((long long)active) - 1;
         const unsigned target = __shfl(lab, leader, 64);
@@ -507,6 +511,7 @@ __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float
         }
         any |= vw;
         active &= ~__ballot(mine);
+    }
     }
     if (lane == 0 && any && (L.scalars[2] | any) != L.scalars[2]) atomicOr(L.scalars + 2, any);
 }
@@ -1135,7 +1140,8 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_relabel, dim3(cell_blocks), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_regions", 20 * cells,
               hipLaunchKernelGGL(opp_region_init, dim3(256), dim3(THREADS), 0, s, L, a.stats);
-              hipLaunchKernelGGL(opp_regions, dim3(cell_blocks), dim3(THREADS), 0, s, L, l0, l1,
+              hipLaunchKernelGGL(opp_regions, dim3((cell_blocks + REGION_STRIDE - 1) / REGION_STRIDE),
+                                 dim3(THREADS), 0, s, L, l0, l1,
                                  l2));
     RHD_TIMED(prof, s, "opp_scan_ring", 40 * cells,
               hipLaunchKernelGGL(opp_scan_count, dim3(scan_blocks), dim3(THREADS), 0, s, L,
