@@ -97,16 +97,19 @@ __device__ __forceinline__ float log2_f32(float v) { return (float)log2((double)
 // set of a target channel is always the other two channels. opp_cells
 // therefore evaluates them once per cell and excluded channel; NaN marks the
 // cases where the reference returns None.
-__device__ __forceinline__ void pair_values(const float4& cell, int excluded, float& ref,
-                                            float& lr) {
+__device__ __forceinline__ bool positive_finite(float v) { return is_finite(v) && v > 0.0f; }
+// `roots[c]` is cbrt_f32 of channel c's mean wherever that mean is positive
+// and finite (each channel's root serves two of the pairs).
+__device__ __forceinline__ void pair_values(const float4& cell, const float* roots, int excluded,
+                                            float& ref, float& lr) {
     const int a = excluded == 0 ? 1 : 0, b = excluded == 2 ? 1 : 2;
     const float ma = mean_of(cell, a), mb = mean_of(cell, b);
     const float nan = __int_as_float(0x7fc00000);
     ref = nan;
-    if (is_finite(ma) && ma > 0.0f && is_finite(mb) && mb > 0.0f) {
+    if (positive_finite(ma) && positive_finite(mb)) {
         float sum = 0.0f;
-        sum += cbrt_f32(ma);
-        sum += cbrt_f32(mb);
+        sum += roots[a];
+        sum += roots[b];
         const float root = sum / 2.0f;
         ref = root * root * root;
     }
@@ -198,9 +201,11 @@ __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m
         }
     const float4 cell = make_float4(means[0], means[1], means[2], __uint_as_float(flags));
     L.cell[i] = cell;
+    float roots[3];
+    for (int c = 0; c < 3; ++c) roots[c] = positive_finite(means[c]) ? cbrt_f32(means[c]) : 0.0f;
     for (int e = 0; e < 3; ++e) {
         float ref, lr;
-        pair_values(cell, e, ref, lr);
+        pair_values(cell, roots, e, ref, lr);
         L.pair_ref[(size_t)e * L.cells + i] = ref;
         L.pair_lr[(size_t)e * L.cells + i] = lr;
     }
