@@ -169,18 +169,23 @@ __device__ void blur_tile(const float* __restrict__ in, const float* __restrict_
 }
 
 template <int R>
+__device__ void blur_interior(const float* __restrict__ in, const float* __restrict__ aux,
+                              float* __restrict__ out, int width, const Weights& k, int mode,
+                              float noise_protection, float* s_in, float* s_h);
+
+template <int R>
 __global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
                                                     const float* __restrict__ aux,
                                                     float* __restrict__ out, int width,
                                                     int height, Weights k, int mode,
                                                     float noise_protection) {
-    __shared__ float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
-    __shared__ float s_h[(TH + 2 * MAXR) * TW];
+    __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
+    __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool fast = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
     if (fast)
-        blur_tile<R, true>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+        blur_interior<R>(in, aux, out, width, k, mode, noise_protection, s_in, s_h);
     else
         blur_tile<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
 }
@@ -300,6 +305,47 @@ __device__ __forceinline__ void vstrip(const float* src, const Weights& k, float
 #pragma unroll
         for (int t = 0; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
         o[q] = v;
+    }
+}
+
+// blur_tile for a tile whose r halo lies inside the image (no mirroring, the
+// horizontal sums start from 0.0): strips of four outputs per thread, as in
+// the RL iteration.
+template <int R>
+__device__ void blur_interior(const float* __restrict__ in, const float* __restrict__ aux,
+                              float* __restrict__ out, int width, const Weights& k, int mode,
+                              float noise_protection, float* s_in, float* s_h) {
+    constexpr int W = TW + 2 * R, H = TH + 2 * R, P = (W + 3) / 4 * 4;
+    static_assert(4 * (TW / 4 - 1) + InteriorWindows<R>::CHUNKS * 4 <= P, "strips stay in a row");
+    static_assert(H * P <= (TH + 2 * MAXR) * (TW + 2 * MAXR), "the window fits s_in");
+    static_assert(TH * TW == 4 * THREADS, "one vertical strip per thread");
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    stage<TH + 2 * MAXR, TW + 2 * MAXR>(s_in, P, in + (size_t)(y0 - R) * width + x0 - R, width, H,
+                                       W);
+    __syncthreads();
+    for (int item = threadIdx.x; item < H * (TW / 4); item += THREADS) {
+        const int r = item / (TW / 4), j = item % (TW / 4);
+        *reinterpret_cast<float4*>(s_h + r * TW + 4 * j) = hstrip<R>(s_in + r * P + 4 * j, k);
+    }
+    __syncthreads();
+    const int c = threadIdx.x & 63;
+    const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    float b[4];
+    vstrip<R, TW>(s_h + r0 * TW + c, k, b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const size_t i = (size_t)(y0 + r0 + q) * width + x0 + c;
+        if (mode == 0) {
+            out[i] = num_min(aux[i], (b[q] * b[q]) * (b[q] * b[q]));
+        } else {
+            const float g = in[i];
+            const float threshold =
+                0.0005f + noise_protection * 0.008f + b[q] * noise_protection * 0.02f;
+            const float detail = fabsf(g - b[q]);
+            out[i] = aux[i] * (smoothstep(threshold, threshold * 3.0f, detail) *
+                               smoothstep(0.001f, 0.01f, g));
+        }
     }
 }
 
