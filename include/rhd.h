@@ -291,6 +291,10 @@ rhd_status rhd_synchronize(rhd_context* ctx);
  * reports a kernel group's total milliseconds, launch count and algorithmic
  * bytes per launch. For benchmarking; off by default. */
 rhd_status rhd_profile_enable(rhd_context* ctx, int enable);
+/* Times only the launches of one kernel group (`kernel`, a name as
+ * rhd_profile_entry reports it), so that the events leave the rest of the
+ * stage undisturbed; NULL or "" times every group again. */
+rhd_status rhd_profile_only(rhd_context* ctx, const char* kernel);
 rhd_status rhd_profile_reset(rhd_context* ctx);
 uint32_t rhd_profile_count(rhd_context* ctx);
 rhd_status rhd_profile_entry(rhd_context* ctx, uint32_t index, const char** name, double* total_ms,

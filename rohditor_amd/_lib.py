@@ -140,6 +140,7 @@ SIGNATURES = {
     "rhd_synchronize": (c_int, [_P]),
     "rhd_profile_enable": (c_int, [_P, c_int]),
     "rhd_profile_reset": (c_int, [_P]),
+    "rhd_profile_only": (c_int, [_P, c_char_p]),
     "rhd_profile_count": (c_uint32, [_P]),
     "rhd_profile_entry": (c_int, [_P, c_uint32, POINTER(c_char_p), POINTER(ctypes.c_double),
                                   POINTER(c_uint64), POINTER(c_uint64)]),

@@ -54,6 +54,10 @@ hipEvent_t Profiler::take_event() {
 }
 
 void Profiler::begin(hipStream_t s, const char* name, uint64_t bytes) {
+    if (!only.empty() && only != name) {
+        skipped_ = true;
+        return;
+    }
     size_t k = 0;
     while (k < totals.size() && totals[k].name != name) ++k;
     if (k == totals.size()) totals.push_back(Total{name, 0.0, 0, 0});
@@ -64,8 +68,17 @@ void Profiler::begin(hipStream_t s, const char* name, uint64_t bytes) {
 }
 
 void Profiler::end(hipStream_t s) {
+    if (skipped_) {
+        skipped_ = false;
+        return;
+    }
+    if (!open_) return;
     hipEvent_t stop = take_event();
-    if (!open_ || !stop) return;
+    if (!stop) {
+        free_.push_back(open_);
+        open_ = nullptr;
+        return;
+    }
     (void)hipEventRecord(stop, s);
     pending_.push_back(Pending{open_total_, open_, stop, open_bytes_});
     open_ = nullptr;
@@ -1400,6 +1413,12 @@ rhd_status rhd_synchronize(rhd_context* ctx) {
 rhd_status rhd_profile_enable(rhd_context* ctx, int enable) {
     if (!ctx) return RHD_INVALID_INPUT;
     ctx->lanes[0].profiling = enable != 0;
+    return RHD_OK;
+}
+
+rhd_status rhd_profile_only(rhd_context* ctx, const char* kernel) {
+    if (!ctx) return RHD_INVALID_INPUT;
+    ctx->lanes[0].profiler.only = kernel ? kernel : "";
     return RHD_OK;
 }
 

@@ -26,6 +26,9 @@ class Profiler {
     void reset();
     ~Profiler();
 
+    // When set, only launches of this kernel group are timed.
+    std::string only;
+
     struct Total {
         std::string name;
         double ms = 0.0;
@@ -44,6 +47,7 @@ class Profiler {
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> free_;
     hipEvent_t open_ = nullptr;
+    bool skipped_ = false;
     size_t open_total_ = 0;
     uint64_t open_bytes_ = 0;
 };
