@@ -343,9 +343,10 @@ __global__ void nr_select_init(Selection q, SelectState* __restrict__ st,
 }
 
 // One 8-bit radix pass over HIST_CHUNK samples of one (plane, bin): the
-// histogram of the digit below the prefix, of the samples (round 0) or of
-// their distance to the median (round 1), in total order.
-__global__ __launch_bounds__(256) void nr_select_hist(Selection q, int round, int pass,
+// histogram of the digit below the prefix, of the samples (stage 0, the
+// median) or of their distance to the median (stage 1, the MAD), in total
+// order.
+__global__ __launch_bounds__(256) void nr_select_hist(Selection q, int stage, int pass,
                                                       const float* __restrict__ samples,
                                                       const SelectState* __restrict__ st,
                                                       uint32_t* __restrict__ hist) {
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(256) void nr_select_hist(Selection q, int round, in
     h[t] = 0;
     __syncthreads();
     for (int64_t i = lo + t; i < hi; i += 256) {
-        const float x = round ? fabsf(v[i] - median) : v[i];
+        const float x = stage ? fabsf(v[i] - median) : v[i];
         const uint32_t key = order_key(x);
         if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1u);
     }
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(256) void nr_select_hist(Selection q, int round, in
 
 // Picks the digit that holds the sought rank, clears the histogram for the
 // next pass and, after the last pass, stores the selected value.
-__global__ __launch_bounds__(256) void nr_select_digit(Selection q, int round, int pass,
+__global__ __launch_bounds__(256) void nr_select_digit(Selection q, int stage, int pass,
                                                        SelectState* __restrict__ st,
                                                        uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(256) void nr_select_digit(Selection q, int round, i
         st[seg].rank = rank - below;
         return;
     }
-    if (round == 0) {
+    if (stage == 0) {
         // The second selection seeks the same index, len / 2, among the
         // distances to this median.
         int64_t start, flat;
@@ -815,13 +816,13 @@ hipError_t launch_noise_profile(const float* rgb, int width, int height, const f
     RHD_TIMED(pr, s, "nr_gather", 8 * nsamples,
               nr_gather<<<dim3((unsigned)g.flat_tiles, 3), 256, 0, s>>>(q, band0, planes, keys,
                                                                        keys2, samples));
-    for (int round = 0; round < 2; ++round)
+    for (int stage = 0; stage < 2; ++stage)
         for (int pass = 0; pass < 4; ++pass) {
             RHD_TIMED(pr, s, "nr_select_hist", 4 * nsamples,
                       nr_select_hist<<<dim3((unsigned)g.hist_blocks, 3), 256, 0, s>>>(
-                          q, round, pass, samples, state, hist));
+                          q, stage, pass, samples, state, hist));
             RHD_TIMED(pr, s, "nr_select", 0,
-                      nr_select_digit<<<3 * bins, 256, 0, s>>>(q, round, pass, state, hist));
+                      nr_select_digit<<<3 * bins, 256, 0, s>>>(q, stage, pass, state, hist));
         }
     RHD_TIMED(pr, s, "nr_select", 0,
               nr_points<<<dim3(bins, 3), 1024, 0, s>>>(q, keys, keys2, bright, state, points));
