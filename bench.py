@@ -76,6 +76,7 @@ def parse():
 # Per-kernel HBM traffic measured with rocprofv3 counters (--traffic-summary
 # output); the live roofline reports the dominant kernel's row when present.
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_develop_24mp.txt"
+COUNTERS_FILE = ROOT / "profiles" / "counters_develop_24mp.txt"
 COPY_BYTES = 1 << 30
 
 
@@ -137,6 +138,21 @@ def measured_traffic(kernel):
         parts = line.split()
         if len(parts) == 5 and not line.startswith("#") and parts[0] == kernel:
             return float(parts[4])
+    return None
+
+
+def measured_valu_busy(kernel):
+    """VALUBusy (% of the kernel's time the vector ALUs issue) of `kernel`
+    from the committed counter summary."""
+    if not COUNTERS_FILE.exists():
+        return None
+    for line in COUNTERS_FILE.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and not line.startswith("#") and parts[0] == kernel:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
     return None
 
 
@@ -311,8 +327,10 @@ def main():
     # rocprof names the dominant kernel's instantiation (cap_iterate<3> for
     # sigma 0.7); the traffic file is keyed by that name.
     traffic = None
+    valu_busy = None
     for name in (dominant, dominant + "<3>"):
         traffic = traffic if traffic is not None else measured_traffic(name)
+        valu_busy = valu_busy if valu_busy is not None else measured_valu_busy(name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": None if traffic is None else int(traffic),
@@ -321,7 +339,10 @@ def main():
                 "launches_per_step": dom_launches / sampled,
                 "sampled_steps": sampled,
                 "algorithmic_bytes_per_launch": int(dom_bytes),
-                "top_kernel": top, "top_kernel_ms_per_step": round(prof[top][0] / args.steps, 4)}
+                "top_kernel": top, "top_kernel_ms_per_step": round(prof[top][0] / args.steps, 4),
+                # The kernel's actual limiter: vector-ALU issue, from a --pmc pass.
+                "valu_busy_pct": valu_busy,
+                "valu_busy_source": None if valu_busy is None else str(COUNTERS_FILE.relative_to(ROOT))}
     stage = {"stage_GBps": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9, 1),
              "stage_frac": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
 
