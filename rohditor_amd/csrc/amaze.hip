@@ -84,8 +84,8 @@ __global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
     const int cc = blk.x * 64 + (threadIdx.x & 63);
     const int rr = blk.y * 4 + (threadIdx.x >> 6);
     const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
-    const int sx = mirror_whole(tx * VALID - BORDER + cc, a.width);
-    const int sy = mirror_whole(ty * VALID - BORDER + rr, a.height);
+    const int sx = mirror_whole_near(tx * VALID - BORDER + cc, a.width);
+    const int sy = mirror_whole_near(ty * VALID - BORDER + rr, a.height);
     const int c = fc(a.pattern, rr, cc);
     const float v = a.mosaic[(size_t)sy * a.width + sx] * a.forward[c];
     const int i = rr * TS + cc;

@@ -96,8 +96,8 @@ __device__ __forceinline__ float interpolate(float d, float h, float v) {
 __global__ __launch_bounds__(256) void rcd_load(Args a) {
     RCD_SITE
     const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
-    const int sx = mirror_whole(tx * STEP - BORDER + col, a.width);
-    const int sy = mirror_whole(ty * STEP - BORDER + row, a.height);
+    const int sx = mirror_whole_near(tx * STEP - BORDER + col, a.width);
+    const int sy = mirror_whole_near(ty * STEP - BORDER + row, a.height);
     const int c = color_at(a.pattern, col, row);
     const float v = a.mosaic[(size_t)sy * a.width + sx] * a.forward[c];
     t.full(CFA)[i] = v;

@@ -45,6 +45,15 @@ __host__ __device__ __forceinline__ int mirror_near(int index, int length) {
     return mirror_half(index, length);
 }
 
+// mirror_whole for tile loads: inside the image a compare, one reflection for
+// indices up to length - 1 outside, the modulo form only beyond that.
+__host__ __device__ __forceinline__ int mirror_whole_near(int index, int length) {
+    if (index >= 0 && index < length) return index;
+    if (index < 0 && index > -length) return -index;
+    if (index >= length && index < 2 * length - 1) return 2 * (length - 1) - index;
+    return mirror_whole(index, length);
+}
+
 // Compare-select min/max (crates/demosaic/src/amaze/math.rs:95-105).
 __device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }
