@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--decode", action="store_true",
                    help="RAW ingest: the 24 MP mosaic as Sony compressed ARW, decoded on the GPU")
     p.add_argument("--batch-lanes", type=int, default=0, help="0: autotune")
-    p.add_argument("--served-lanes", type=int, default=3)
+    p.add_argument("--served-lanes", type=int, default=4)
     p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain device copy of 1 GiB (read) into 1 GiB (write), the "
