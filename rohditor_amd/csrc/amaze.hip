@@ -721,20 +721,37 @@ __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
     const float* cfa = t.full(CFA);
     for (int k = threadIdx.x; k < n; k += 256) {
         const int i = s_sites[k];
+        // Row by row of taps: the row's flags, then the flagged taps' loads
+        // (all in flight together), then the sums in the reference's order.
         float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
         for (int di = -6; di < 7; di += 2) {
-            int i1 = i + di * TS - 6;
-            for (int dj = -6; dj < 7; dj += 2) {
-                if (n2[i1 >> 1] != 0) {
-                    float c = cfa[i1];
-                    sumcfa += c;
-                    sumh += cfa[i1 - 1] + cfa[i1 + 1];
-                    sumv += cfa[i1 - TS] + cfa[i1 + TS];
-                    sumsqh += sqr(c - cfa[i1 - 1]) + sqr(c - cfa[i1 + 1]);
-                    sumsqv += sqr(c - cfa[i1 - TS]) + sqr(c - cfa[i1 + TS]);
+            const int row = i + di * TS - 6;
+            bool f[7];
+            float c[7], l[7], r[7], u[7], d[7];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) f[j] = n2[(row + 2 * j) >> 1] != 0;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int i1 = row + 2 * j;
+                c[j] = l[j] = r[j] = u[j] = d[j] = 0.0f;
+                if (f[j]) {
+                    c[j] = cfa[i1];
+                    l[j] = cfa[i1 - 1];
+                    r[j] = cfa[i1 + 1];
+                    u[j] = cfa[i1 - TS];
+                    d[j] = cfa[i1 + TS];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                if (f[j]) {
+                    sumcfa += c[j];
+                    sumh += l[j] + r[j];
+                    sumv += u[j] + d[j];
+                    sumsqh += sqr(c[j] - l[j]) + sqr(c[j] - r[j]);
+                    sumsqv += sqr(c[j] - u[j]) + sqr(c[j] - d[j]);
                     areawt += 1.0f;
                 }
-                i1 += 2;
             }
         }
         sumh = sumcfa - 0.5f * sumh;
