@@ -259,14 +259,23 @@ __global__ __launch_bounds__(THREADS) void opp_label_tiles(Layout L) {
     __shared__ unsigned s_par[LY * LX];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * LX, y0 = blk.y * LY;
+    bool any = false;
     for (int k = threadIdx.x; k < LHY * LHX; k += THREADS) {
         const int x = x0 + k % LHX - 1, y = y0 + k / LHX - 1;
         bool clipped = false;
         if (x >= 0 && y >= 0 && x < L.cw && y < L.ch)
             clipped = (flags_of(L.cell[(size_t)y * L.cw + x]) >> 3) != 0;
         s_clip[k] = clipped;
+        any |= clipped;
     }
-    __syncthreads();
+    // A tile with no clipped cell in its window has no dilated cell either.
+    if (!__syncthreads_or(any)) {
+        for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
+            const int x = x0 + k % LX, y = y0 + k / LX;
+            if (x < L.cw && y < L.ch) L.label[(size_t)y * L.cw + x] = NONE;
+        }
+        return;
+    }
     for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
         const int lx = k % LX, ly = k / LX;
         bool fg = false;
