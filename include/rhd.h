@@ -191,6 +191,21 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
  * (crates/gpu/src/capture/mod.rs:54), in place on `source`. */
 rhd_status rhd_capture(rhd_context* ctx, rhd_source* source, const rhd_detail_desc* detail,
                        const volatile uint8_t* cancel, rhd_metrics* metrics);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): capture sharpening exactly as rhd_capture does it (same
+ * kernels, same results), in place on `planar_rgb`, a caller-owned device
+ * buffer of 3 * width * height floats (R plane, then G, then B). It calls no
+ * allocator. Its working planes live in a fixed device array of
+ * RHD_NOALLOC_MAX_PIXELS pixels that each GPU holds until the process ends;
+ * larger images return RHD_INVALID_DIMENSIONS. Only capture sharpening is
+ * run: false colour or noise reduction in `detail` is RHD_INVALID_INPUT.
+ * Calls on one GPU share that array, so calls that overlap on one GPU race.
+ * Runs on the context's stream and waits for its result (the non-finite
+ * check, as rhd_capture reports it). */
+#define RHD_NOALLOC_MAX_PIXELS (2u * 1024u * 1024u)
+rhd_status rhd_capture_noalloc(rhd_context* ctx, float* planar_rgb, uint32_t width,
+                               uint32_t height, const rhd_detail_desc* detail);
 
 /*
  * RAW ingest: the Sony ARW strip that crates/raw/src/rawler_adapter.rs:

@@ -123,6 +123,7 @@ SIGNATURES = {
     "rhd_demosaic": (c_int, [_P, _P, POINTER(SensorDesc), POINTER(DetailDesc), _P, POINTER(_P),
                              POINTER(Metrics)]),
     "rhd_capture": (c_int, [_P, _P, POINTER(DetailDesc), _P, POINTER(Metrics)]),
+    "rhd_capture_noalloc": (c_int, [_P, _P, c_uint32, c_uint32, POINTER(DetailDesc)]),
     "rhd_develop_resident": (c_int, [_P, _P, POINTER(SensorDesc), POINTER(DetailDesc), _P,
                                      POINTER(_P), POINTER(Metrics)]),
     "rhd_develop": (c_int, [_P, _P, c_uint64, POINTER(SensorDesc), POINTER(DetailDesc), _P,

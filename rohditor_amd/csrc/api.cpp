@@ -943,7 +943,7 @@ rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
     ctx->device = device;
     if (budget_bytes) ctx->budget = budget_bytes;
     Lane* L = nullptr;
-    if (lane_open(ctx, 0, &L) != RHD_OK) {
+    if (lane_open(ctx, 0, &L) != RHD_OK || noalloc_setup(ctx, device) != RHD_OK) {
         context_unref(ctx);
         return RHD_UNSUPPORTED;
     }

@@ -152,6 +152,9 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
 // Optics correction and area reduction of `src` on one lane (spatial.cpp):
 // `optics` may be null; target equal to the source size with `full` set
 // gives the full-resolution corrected source.
+// Resolves the no-allocation entry point's scratch array on `device`, which
+// must be current (noalloc.hip); rhd_create calls it once per context.
+rhd_status noalloc_setup(rhd_context* ctx, int device);
 rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
                       const rhd_optics_desc* optics, uint32_t target_width,
                       uint32_t target_height, bool full, const volatile uint8_t* cancel,

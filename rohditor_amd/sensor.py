@@ -494,6 +494,19 @@ class GpuSensorProcessor:
         self._check(self.lib.rhd_capture(self.ctx, source.ptr, ctypes.byref(detail),
                                          cancellation.ptr if cancellation else None, None))
 
+    def capture_noalloc(self, planar_rgb: int, width: int, height: int, detail) -> None:
+        """Capture sharpening in place on a caller-owned planar device buffer
+        (3 * width * height floats) without allocating: rhd_capture_noalloc."""
+        self._check(self.lib.rhd_capture_noalloc(self.ctx, ctypes.c_void_p(planar_rgb), width,
+                                                 height, ctypes.byref(detail)))
+
+    def source_planes(self, source: GpuDevelopedSource):
+        """Device addresses of a source's R, G and B planes."""
+        r, g, b = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self.lib.rhd_source_planes(source.ptr, ctypes.byref(r), ctypes.byref(g),
+                                               ctypes.byref(b)))
+        return r.value, g.value, b.value
+
     def noise_profiles(self, source: GpuDevelopedSource, balance):
         """estimate_noise_profiles (crates/detail/src/denoise/mod.rs:60-110),
         on the device: [(a, b, bins)] for luma, C1, C2, or None when the image
