@@ -296,6 +296,13 @@ rhd_status rhd_source_planes(const rhd_source* source, float** red, float** gree
 void rhd_source_dims(const rhd_source* source, uint32_t* width, uint32_t* height);
 void rhd_source_release(rhd_source* source);
 
+/* Self-test, no counterpart in the reference: evaluates on the device the
+ * replicas of the platform libm's cbrtf (fn 0) and log2f (fn 1) that
+ * Opposed uses (crates/highlight/src/support.rs:65,106 call them through
+ * Rust's f32::cbrt and f32::log2), on `count` host floats into host `out`. */
+rhd_status rhd_libm_eval(rhd_context* ctx, int32_t fn, const float* in, float* out,
+                         uint64_t count);
+
 /* Blocks until the context's stream is idle. */
 rhd_status rhd_synchronize(rhd_context* ctx);
 

@@ -390,3 +390,13 @@ def arw_probe(data: bytes):
     d["curve"] = [int(v) for v in out[6:10]]
     d["has_curve"], d["kind"] = int(out[10]), int(out[11])
     return 0, d
+
+
+def libm(fn: str, values: np.ndarray) -> np.ndarray:
+    """The host libm's cbrtf ("cbrt") or log2f ("log2") over f32 values, as the
+    reference's f32::cbrt and f32::log2 compute them (support.rs:65,106)."""
+    x = np.ascontiguousarray(values, dtype=np.float32)
+    out = np.empty_like(x)
+    st = lib().rhdo_libm(ctypes.c_int({"cbrt": 0, "log2": 1}[fn]), _p(x), _p(out), _sz(x.size))
+    assert st == 0
+    return out

@@ -4,6 +4,7 @@
 //
 // Suffix `_f32` runs the bit-faithful f32 restatement; `_f64` runs the same
 // algorithm in double precision (the FP64 twin).
+#include <cmath>
 #include <cstring>
 
 #include "arw.hpp"
@@ -385,6 +386,14 @@ int rhdo_arw_probe(const uint8_t* file, size_t size, uint64_t out[12]) {
         std::memcpy(out, v, sizeof(v));
     }
     return st;
+}
+
+// The platform libm's cbrtf (fn 0) or log2f (fn 1) over n floats: what the
+// reference's f32::cbrt and f32::log2 return on this host.
+int rhdo_libm(int fn, const float* in, float* out, size_t n) {
+    if (fn != 0 && fn != 1) return 1;
+    for (size_t i = 0; i < n; ++i) out[i] = fn == 0 ? std::cbrt(in[i]) : std::log2(in[i]);
+    return 0;
 }
 
 }  // extern "C"

@@ -138,6 +138,7 @@ SIGNATURES = {
     "rhd_source_planes": (c_int, [_P, POINTER(_P), POINTER(_P), POINTER(_P)]),
     "rhd_source_dims": (None, [_P, POINTER(c_uint32), POINTER(c_uint32)]),
     "rhd_source_release": (None, [_P]),
+    "rhd_libm_eval": (c_int, [_P, c_int32, _P, _P, c_uint64]),
     "rhd_synchronize": (c_int, [_P]),
     "rhd_profile_enable": (c_int, [_P, c_int]),
     "rhd_profile_reset": (c_int, [_P]),

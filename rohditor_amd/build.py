@@ -24,7 +24,7 @@ COMMON = [
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
-    f"-I{HERE.parent / 'include'}",
+    f"-I{HERE.parent / 'include'}", f"-I{OBJ}",
 ]
 
 
@@ -42,8 +42,28 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def log2_table() -> Path:
+    """Generates the log2 exception list of csrc/libm_f32.hpp from the host
+    libm (gen_log2_table.cc), which csrc/libm.hip compiles in."""
+    OBJ.mkdir(exist_ok=True)
+    exe = OBJ / "gen_log2_table"
+    inc = OBJ / "log2f_exceptions.inc"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off",
+           "-fno-fast-math", "-Wall", f"-I{CSRC}", str(CSRC / "gen_log2_table.cc"), "-o", str(exe)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("compiling gen_log2_table failed")
+    res = subprocess.run([str(exe), str(inc)], capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("gen_log2_table failed")
+    return inc
+
+
 def build(jobs: int | None = None) -> Path:
     OBJ.mkdir(exist_ok=True)
+    log2_table()
     srcs = sources()
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs or min(8, len(srcs))) as pool:
         objs = list(pool.map(_compile, srcs))

@@ -531,6 +531,8 @@ rhd_status run_opposed(rhd_context* ctx, Lane& L, rhd_plane* plane, const rhd_se
     a.scratch = L.opposed.ptr;
     a.scratch_bytes = L.opposed.bytes;
     a.stats = L.d_stats;
+    a.log2_bitmap = libm_log2_bitmap(ctx->device);
+    if (!a.log2_bitmap) return fail(ctx, RHD_SYNCHRONIZATION, "libm tables are not set up");
     hipError_t e = launch_opposed(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "opposed");
     return RHD_OK;
@@ -943,7 +945,8 @@ rhd_status rhd_create(int device, uint64_t budget_bytes, rhd_context** out) {
     ctx->device = device;
     if (budget_bytes) ctx->budget = budget_bytes;
     Lane* L = nullptr;
-    if (lane_open(ctx, 0, &L) != RHD_OK || noalloc_setup(ctx, device) != RHD_OK) {
+    if (lane_open(ctx, 0, &L) != RHD_OK || libm_setup(ctx, device) != RHD_OK ||
+        noalloc_setup(ctx, device) != RHD_OK) {
         context_unref(ctx);
         return RHD_UNSUPPORTED;
     }

@@ -30,10 +30,10 @@
 //   sites      one thread per (cell, channel) with a suspected site: the
 //              estimate, then per site the bound, the neutral ceiling and the
 //              statistics
-// cbrt and log2 are evaluated in double and rounded to f32 (correctly
-// rounded in practice); the reference calls the platform libm's cbrtf and
-// log2f, which are within 1 ulp of that. Results therefore agree with the
-// reference within the stated f32 tolerance rather than bit for bit.
+// cbrt and log2 are the platform libm's cbrtf and log2f bit for bit
+// (libm_f32.hpp), as the reference's f32::cbrt and f32::log2 are, so every
+// output is bit-identical to the reference's.
+#include "libm_f32.hpp"
 #include "rhd_internal.hpp"
 
 namespace rhd {
@@ -67,6 +67,7 @@ struct Layout {
     float4* corrections;           // per (group, slot): offset, ratio, valid, has_ratio
     unsigned* site_counts;         // per opp_sites block: its five statistics
     unsigned* scalars;             // [0] region count, [1] ring count, [2] any wanted
+    const uint32_t* log2_bitmap;   // libm_f32.hpp exception bitmap
 };
 
 __device__ __forceinline__ unsigned flags_of(const float4& c) { return __float_as_uint(c.w); }
@@ -89,8 +90,10 @@ __device__ __forceinline__ unsigned support_mask(const float4& cell, int target,
         if (c != target && bright_clean(cell, c, lv)) m |= 1u << c;
     return m;
 }
-__device__ __forceinline__ float cbrt_f32(float v) { return (float)cbrt((double)v); }
-__device__ __forceinline__ float log2_f32(float v) { return (float)log2((double)v); }
+__device__ __forceinline__ float cbrt_f32(float v) { return libm::cbrt_libm(v); }
+__device__ __forceinline__ float log2_f32(float v, const uint32_t* bitmap) {
+    return libm::log2_libm(v, bitmap);
+}
 
 // The two-channel parts of support.rs:51-72 (reference) and :101-110
 // (support_log_ratio) depend only on a cell's means, and a two-channel support
@@ -101,7 +104,7 @@ __device__ __forceinline__ bool positive_finite(float v) { return is_finite(v) &
 // `roots[c]` is cbrt_f32 of channel c's mean wherever that mean is positive
 // and finite (each channel's root serves two of the pairs).
 __device__ __forceinline__ void pair_values(const float4& cell, const float* roots, int excluded,
-                                            float& ref, float& lr) {
+                                            const uint32_t* bitmap, float& ref, float& lr) {
     const int a = excluded == 0 ? 1 : 0, b = excluded == 2 ? 1 : 2;
     const float ma = mean_of(cell, a), mb = mean_of(cell, b);
     const float nan = __int_as_float(0x7fc00000);
@@ -114,7 +117,7 @@ __device__ __forceinline__ void pair_values(const float4& cell, const float* roo
         ref = root * root * root;
     }
     const float ratio = ma / mb;
-    lr = is_finite(ratio) && ratio > 0.0f ? log2_f32(ratio) : nan;
+    lr = is_finite(ratio) && ratio > 0.0f ? log2_f32(ratio, bitmap) : nan;
 }
 
 // support.rs:51-72, with the two-channel value taken from pair_values.
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m
     for (int c = 0; c < 3; ++c) roots[c] = positive_finite(means[c]) ? cbrt_f32(means[c]) : 0.0f;
     for (int e = 0; e < 3; ++e) {
         float ref, lr;
-        pair_values(cell, roots, e, ref, lr);
+        pair_values(cell, roots, e, L.log2_bitmap, ref, lr);
         L.pair_ref[(size_t)e * L.cells + i] = ref;
         L.pair_lr[(size_t)e * L.cells + i] = lr;
     }
@@ -1107,6 +1110,7 @@ Layout layout_of(const OpposedArgs& a) {
     L.corrections = (float4*)take(cap * SLOTS * sizeof(float4));
     L.site_counts = (unsigned*)take(site_blocks(L) * 5 * sizeof(unsigned));
     L.scalars = (unsigned*)take(64);
+    L.log2_bitmap = a.log2_bitmap;
     return L;
 }
 

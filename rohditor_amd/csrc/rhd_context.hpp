@@ -155,6 +155,11 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
 // Resolves the no-allocation entry point's scratch array on `device`, which
 // must be current (noalloc.hip); rhd_create calls it once per context.
 rhd_status noalloc_setup(rhd_context* ctx, int device);
+// Builds the device's log2 exception bitmap (libm.hip) once per device, with
+// the device current; rhd_create calls it. libm_log2_bitmap returns it (null
+// before set-up).
+rhd_status libm_setup(rhd_context* ctx, int device);
+const uint32_t* libm_log2_bitmap(int device);
 rhd_status spatial_on(rhd_context* ctx, Lane& L, const rhd_source* src,
                       const rhd_optics_desc* optics, uint32_t target_width,
                       uint32_t target_height, bool full, const volatile uint8_t* cancel,

@@ -169,6 +169,7 @@ struct OpposedArgs {
     void* scratch;
     size_t scratch_bytes;
     DeviceStats* stats;
+    const uint32_t* log2_bitmap;  // libm_f32.hpp exception bitmap on this device
 };
 size_t opposed_scratch_bytes(int width, int height);
 hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);

@@ -78,3 +78,42 @@ def test_highlight_resolution_matches_reference_fixture():
     F = np.float32
     common = (F(1.2) * F(0.987)) * min(F(1.0), F(2.0), F(1.55))
     assert desc.neutral_ceiling.tolist() == [common / F(2.0), common / F(1.0), common / F(1.55)]
+
+
+def test_every_symbol_host_code_resolves_is_in_the_device_code():
+    """Each array that host code resolves with hipGetSymbolAddress must be in
+    the gfx950 code object of its source file: the device compiler drops an
+    array that no device code reads, and asking the runtime for a symbol the
+    code object lacks takes the process down."""
+    import shutil
+    import subprocess
+    import tempfile
+    bundler = Path("/opt/rocm/llvm/bin/clang-offload-bundler")
+    readelf = Path("/opt/rocm/llvm/bin/llvm-readelf")
+    if not (bundler.exists() and readelf.exists()):
+        pytest.skip("ROCm LLVM tools not installed")
+    csrc = ROOT / "rohditor_amd" / "csrc"
+    build = ROOT / "rohditor_amd" / "_build"
+    checked = 0
+    for src in sorted(csrc.glob("*.hip")):
+        names = re.findall(r"HIP_SYMBOL\((\w+)\)", src.read_text())
+        if not names:
+            continue
+        obj = build / (src.stem + ".o")
+        assert obj.exists(), f"{obj} is not built"
+        with tempfile.TemporaryDirectory() as tmp:
+            fatbin = Path(tmp) / "fatbin"
+            dev = Path(tmp) / "dev.o"
+            objcopy = shutil.which("objcopy")
+            assert objcopy, "objcopy is required"
+            subprocess.run([objcopy, "-O", "binary", "--only-section=.hip_fatbin", str(obj),
+                            str(fatbin)], check=True)
+            subprocess.run([str(bundler), "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
+                            f"--input={fatbin}", f"--output={dev}", "--unbundle"], check=True)
+            syms = subprocess.run([str(readelf), "-s", "-W", str(dev)], check=True,
+                                  capture_output=True, text=True).stdout
+        for name in names:
+            assert re.search(rf"\b_Z\w*{len(name)}{name}E\b", syms), \
+                f"{name} ({src.name}) is missing from the gfx950 code object"
+            checked += 1
+    assert checked >= 2

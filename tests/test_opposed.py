@@ -1,7 +1,7 @@
 """Opposed highlight reconstruction: the oracle against the reference's own
 fixtures (crates/highlight/tests/opposed.rs), and the HIP path against the
-oracle (tolerance parity: the reference's cbrtf/log2f come from the platform
-libm, which is within 1 ulp of the device's correctly rounded values)."""
+oracle, bit for bit (the device's cbrt and log2 are the platform libm's,
+which the reference's f32::cbrt and f32::log2 call; tests/test_libm.py)."""
 import numpy as np
 import pytest
 
@@ -115,8 +115,16 @@ def gpu_opposed(gpu, m, pattern, levels, ceiling=None):
     return plane.read(), stats, o.regions
 
 
-def rel_err(a, b):
-    return np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+def bits_equal(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def describe_mismatch(a, b):
+    if a.shape != b.shape:
+        return f"shape {a.shape} != {b.shape}"
+    bad = np.argwhere(a.view(np.uint32) != b.view(np.uint32))
+    first = tuple(bad[0])
+    return f"{len(bad)} samples differ; first at {first}: {a[first]!r} != {b[first]!r}"
 
 
 @pytest.mark.gpu
@@ -126,7 +134,7 @@ def test_gpu_fixture_statistics_match(gpu, oracle, pattern):
     out, stats, regions = gpu_opposed(gpu, m, pattern, (0.9,) * 3)
     _, expected, ostats, _, count = oracle.opposed(m, pattern, (0.9,) * 3)
     assert stats == ostats.tolist() and regions == count
-    assert rel_err(out, expected).max() <= 1e-5
+    assert bits_equal(out, expected)
 
 
 @pytest.mark.gpu
@@ -135,7 +143,7 @@ def test_gpu_single_channel_fixture(gpu, oracle):
     out, stats, _ = gpu_opposed(gpu, m, 0, (0.9, 1.0, 1.0))
     _, expected, ostats, _, _ = oracle.opposed(m, 0, (0.9, 1.0, 1.0))
     assert stats == ostats.tolist()
-    assert rel_err(out, expected).max() <= 1e-5
+    assert bits_equal(out, expected)
 
 
 @pytest.mark.gpu
@@ -156,13 +164,8 @@ def test_gpu_natural_scene_matches_oracle(gpu, oracle, ceiling, seed):
     out, stats, regions = gpu_opposed(gpu, m, 0, levels, ceiling)
     _, expected, ostats, _, count = oracle.opposed(m, 0, levels, ceiling)
     assert regions == count
-    assert stats[0] == ostats[0] and stats[6:] == ostats[6:].tolist()
-    # Counts can move by a site or two when a 1-ulp libm difference flips a gate.
-    for a, b in zip(stats[1:6], ostats[1:6].tolist()):
-        assert abs(a - b) <= max(2, b // 10000)
-    err = rel_err(out, expected)
-    assert np.mean(err > 1e-5) <= 1e-5, np.mean(err > 1e-5)
-    assert np.mean(err) < 1e-7
+    assert stats == ostats.tolist()
+    assert bits_equal(out, expected), describe_mismatch(out, expected)
 
 
 @pytest.mark.gpu
@@ -193,9 +196,11 @@ def test_gpu_develop_full_sensor_opposed_amaze_rl(gpu, oracle):
                              threads=oracle.cpu_threads())
     ceil = sensor.capture_ceilings(recipe, desc.demosaic_balance)
     _, rgb = oracle.capture(rgb, 0.5, 0.7, 0.5, ceil, iterations=10, threads=oracle.cpu_threads())
-    assert m.opposed.suspected_clipped_sites == ostats[0] and m.opposed.regions == count
-    err = rel_err(out, rgb)
-    # Stated tolerance: <= 1e-5 relative, with AMaZE's and RL's decisions
-    # downstream of a moved Opposed estimate allowed on a tiny outlier budget.
-    assert np.mean(err > 1e-5) <= 5e-5, np.mean(err > 1e-5)
-    assert float(np.sqrt(np.mean((out - rgb) ** 2))) < 1e-5
+    o = m.opposed
+    stats = [o.suspected_clipped_sites, o.reconstructed_sites, o.changed_sites, o.fallback_sites,
+             o.fully_unsupported_sites, o.neutralised_sites, *o.suspected_by_channel]
+    assert stats == ostats.tolist() and o.regions == count
+    err = np.abs(out.astype(np.float64) - rgb) / np.maximum(np.abs(rgb), 1e-3)
+    print(f"C2 24 MP develop: {np.count_nonzero(out != rgb)} samples differ, "
+          f"max relative error {err.max():.3g}", flush=True)
+    assert bits_equal(out, rgb), describe_mismatch(out, rgb)
