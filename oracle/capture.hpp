@@ -98,14 +98,17 @@ int apply(T* rgb, size_t width, size_t height, T amount, T sigma, T noise_protec
             guide[i] = num_max(g, FLOOR);
             T peak = T(0);
             for (int c = 0; c < 3; ++c) peak = num_max(peak, p[c] / ceilings[c]);
-            mask[i] = T(1) - smoothstep(RHDO_LIT(T, 0.85), RHDO_LIT(T, 0.98), peak);
+            // 6: upper highlight edge 0.95, 8: lower edge 0.80
+            mask[i] = T(1) - smoothstep(RHDO_MUTANT(8) ? RHDO_LIT(T, 0.80) : RHDO_LIT(T, 0.85),
+                                        RHDO_MUTANT(6) ? RHDO_LIT(T, 0.95) : RHDO_LIT(T, 0.98),
+                                        peak);
         }
     });
     std::vector<T> estimate = guide, temporary(n), blurred(n), ratio(n);
     blur(mask, temporary, blurred, width, height, kernel, threads);
     for (size_t i = 0; i < n; ++i) {
         T b = blurred[i];
-        mask[i] = num_min(mask[i], (b * b) * (b * b));
+        mask[i] = num_min(mask[i], RHDO_MUTANT(5) ? b * b : (b * b) * (b * b));  // 5: power 2
     }
     blur(guide, temporary, blurred, width, height, kernel, threads);
     parallel_for(height, threads, [&](size_t y) {
@@ -124,7 +127,9 @@ int apply(T* rgb, size_t width, size_t height, T amount, T sigma, T noise_protec
         blur(ratio, temporary, blurred, width, height, kernel, threads);
         for (size_t i = 0; i < n; ++i) {
             T e = estimate[i] * blurred[i];
-            T lo = guide[i] * RHDO_LIT(T, 0.5), hi = guide[i] * T(2);
+            // 7: clamp to [0.25 g, 4 g]
+            T lo = guide[i] * (RHDO_MUTANT(7) ? RHDO_LIT(T, 0.25) : RHDO_LIT(T, 0.5)),
+              hi = guide[i] * (RHDO_MUTANT(7) ? T(4) : T(2));
             estimate[i] = e < lo ? lo : (e > hi ? hi : e);
         }
     }

@@ -14,6 +14,15 @@
 
 namespace rhdo {
 
+// Planted mistakes for the tests that show the parity pins catch them: the
+// oracle built with -DRHDO_MUTATION=k (k > 0) carries mistake k, which the
+// headers select with RHDO_MUTANT(k). The oracle used as the checker is
+// always built with 0.
+#ifndef RHDO_MUTATION
+#define RHDO_MUTATION 0
+#endif
+#define RHDO_MUTANT(k) (RHDO_MUTATION == (k))
+
 // Decimal literal rounded directly to T (no double->float double rounding).
 #define RHDO_LIT(T, x) (std::is_same<T, float>::value ? (T)(x##f) : (T)(x))
 

@@ -62,12 +62,12 @@ inline bool reference(const Cell& cell, unsigned support, float& out) {
         if (!(support & (1u << c))) continue;
         float v = cell.means[c];
         if (!finite(v) || v <= 0.0f) return false;
-        sum += std::cbrt(v);
+        sum += RHDO_MUTANT(9) ? v : std::cbrt(v);  // 9: arithmetic instead of cube-root mean
         ++count;
     }
     if (count == 0) return false;
     float root = sum / (float)count;
-    out = root * root * root;
+    out = RHDO_MUTANT(9) ? root : root * root * root;
     return true;
 }
 
@@ -99,7 +99,7 @@ inline bool support_log_ratio(const Cell& cell, unsigned support, float& out) {
 // support.rs:116-125
 inline bool chromaticity_agrees(unsigned support, bool has_t, float t, bool has_e, float e) {
     if (popcount(support) < 2) return true;
-    if (has_t && has_e) return std::fabs(t - e) <= 0.75f;
+    if (has_t && has_e) return std::fabs(t - e) <= (RHDO_MUTANT(3) ? 1.5f : 0.75f);  // 3: gate
     return false;
 }
 
@@ -211,7 +211,8 @@ struct Slot {
             double m = accum / c;
             double variance = num_max(second / c - m * m, 0.0);
             mean = (float)m;
-            tolerance = (float)(2.0 * std::sqrt(variance)) + 1e-5f;
+            // 1: a 3-sigma trim window instead of 2 sigma
+            tolerance = (float)((RHDO_MUTANT(1) ? 3.0 : 2.0) * std::sqrt(variance)) + 1e-5f;
         }
         accum = 0;
         second = 0;
@@ -269,7 +270,9 @@ struct Context {
             if (count >= MIN_CANDIDATES) {
                 std::sort(corr, corr + count, total_less);
                 int middle = count / 2;
-                out = count % 2 == 0 ? (corr[middle - 1] + corr[middle]) * 0.5f : corr[middle];
+                // 4: the upper middle element for even counts, lower for odd
+                out = RHDO_MUTANT(4) ? corr[count % 2 == 0 ? middle : middle - 1]
+                      : count % 2 == 0 ? (corr[middle - 1] + corr[middle]) * 0.5f : corr[middle];
                 out_radius = radius;
                 return true;
             }
@@ -306,7 +309,8 @@ struct Context {
         float corr;
         if (has_local && has_region) {
             float weight = (float)(LAST_RADIUS + 1 - radius) / (float)LAST_RADIUS;
-            corr = weight * local + (1.0f - weight) * region;
+            corr = RHDO_MUTANT(2) ? (1.0f - weight) * local + weight * region  // 2: swapped blend
+                                  : weight * local + (1.0f - weight) * region;
         } else if (has_local) {
             corr = local;
         } else if (has_region) {
@@ -336,6 +340,13 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                        const float* neutral_ceiling, Stats* stats_out, uint32_t* labels_out,
                        uint32_t* region_count_out) {
     Stats st{};
+    // lib.rs:108-119 (ChannelDetectionLevels::validate) and the same check on
+    // the neutral ceiling: finite and positive, before any sample is read.
+    for (int c = 0; c < 3; ++c) {
+        if (!(finite(levels[c]) && levels[c] > 0.0f)) return 8;
+        if (neutral_ceiling && !(finite(neutral_ceiling[c]) && neutral_ceiling[c] > 0.0f))
+            return 8;
+    }
     for (size_t y = 0; y < h; ++y)
         for (size_t x = 0; x < w; ++x) {
             float s = m[y * w + x];
@@ -409,7 +420,8 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                 const Cell& cell = cells[i];
                 for (int c = 0; c < 3; ++c) {
                     if (!(region_clipped[label] & (1u << c))) continue;
-                    if (cell.means[c] < 0.5f * levels[c]) continue;
+                    // 10: ring evidence admitted from a quarter of the level
+                    if (cell.means[c] < (RHDO_MUTANT(10) ? 0.25f : 0.5f) * levels[c]) continue;
                     unsigned pats[3];
                     support_patterns(c, pats);
                     for (int p = 0; p < 3; ++p) {
@@ -450,7 +462,7 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                 continue;
             }
             st.reconstructed++;
-            float bounded = num_min(estimate, 4.0f * level);
+            float bounded = num_min(estimate, (RHDO_MUTANT(11) ? 2.0f : 4.0f) * level);  // 11
             if (bounded > sample) {
                 sample = bounded;
                 st.changed++;

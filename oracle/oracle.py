@@ -5,6 +5,7 @@ this module, and only as the checker; the product path never does.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -125,6 +126,16 @@ def capture(rgb: np.ndarray, amount: float, sigma: float, noise_protection: floa
     if planes:
         return st, out, (guide, mask, est)
     return st, out
+
+
+def blur(plane: np.ndarray, sigma: float) -> np.ndarray:
+    """The capture blur (crates/core/src/sharpening.rs:328-384) of one plane."""
+    x = np.ascontiguousarray(plane, dtype=np.float32)
+    out = np.empty_like(x)
+    st = lib().rhdo_blur_f32(_p(x), _sz(x.shape[1]), _sz(x.shape[0]), ctypes.c_float(sigma),
+                             _p(out))
+    assert st == 0
+    return out
 
 
 def gaussian_kernel(sigma: float) -> np.ndarray:
@@ -400,3 +411,45 @@ def libm(fn: str, values: np.ndarray) -> np.ndarray:
     st = lib().rhdo_libm(ctypes.c_int({"cbrt": 0, "log2": 1}[fn]), _p(x), _p(out), _sz(x.size))
     assert st == 0
     return out
+
+
+MUTATIONS = {
+    1: "Opposed: 3-sigma trim window for the region corrections (2 sigma)",
+    2: "Opposed: local/region blend weights swapped",
+    3: "Opposed: 1.5 EV chromaticity gate (0.75 EV)",
+    4: "Opposed: local median takes one middle element",
+    5: "capture: highlight mask expanded with blur^2 (blur^4)",
+    6: "capture: upper highlight edge 0.95 (0.98)",
+    7: "capture: estimate clamped to [0.25, 4] x guide ([0.5, 2])",
+    8: "capture: lower highlight edge 0.80 (0.85)",
+    9: "Opposed: arithmetic mean as the opposing reference (cube-root mean)",
+    10: "Opposed: ring evidence from 0.25 x level (0.5 x level)",
+    11: "Opposed: estimates bounded at 2 x level (4 x level)",
+}
+
+
+_mutants_built = False
+
+
+def build_mutants() -> None:
+    """Builds every planted-mistake oracle (make mutants, in parallel)."""
+    jobs = str(min(8, os.cpu_count() or 1))
+    subprocess.run(["make", "-s", "-j", jobs, "-C", str(_HERE), "mutants"], check=True)
+
+
+@contextlib.contextmanager
+def mutant(k: int):
+    """Within the block every oracle function runs the build that carries
+    planted mistake `k` (MUTATIONS[k]); the checker is restored afterwards."""
+    global _lib, _mutants_built
+    path = _HERE / "_build" / f"librhd_oracle_m{k}.so"
+    if not _mutants_built:
+        build_mutants()
+        _mutants_built = True
+    saved = lib()
+    _lib = ctypes.CDLL(str(path))
+    _lib.rhdo_amaze_plane_names.restype = ctypes.c_char_p
+    try:
+        yield
+    finally:
+        _lib = saved

@@ -108,6 +108,15 @@ int rhdo_capture_f64(double* rgb, size_t width, size_t height, double amount, do
                                   iterations, nullptr, nullptr, nullptr, threads);
 }
 
+// sharpening.rs:328-384 with the kernel of gaussian.rs:9-17 for `sigma`.
+int rhdo_blur_f32(const float* in, size_t width, size_t height, float sigma, float* out) {
+    const size_t n = width * height;
+    std::vector<float> input(in, in + n), temporary(n), output(n);
+    capture::blur(input, temporary, output, width, height, capture::gaussian_kernel(sigma), 1);
+    for (size_t i = 0; i < n; ++i) out[i] = output[i];
+    return 0;
+}
+
 int rhdo_gaussian_kernel(float sigma, float* out, int capacity) {
     std::vector<float> k = capture::gaussian_kernel(sigma);
     if ((int)k.size() > capacity) return -(int)k.size();
