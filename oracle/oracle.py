@@ -404,11 +404,12 @@ def arw_probe(data: bytes):
 
 
 def libm(fn: str, values: np.ndarray) -> np.ndarray:
-    """The host libm's cbrtf ("cbrt") or log2f ("log2") over f32 values, as the
-    reference's f32::cbrt and f32::log2 compute them (support.rs:65,106)."""
+    """The host libm's cbrtf ("cbrt"), log2f ("log2") or cosf ("cos") over f32
+    values, as the reference's f32 methods compute them on Linux."""
     x = np.ascontiguousarray(values, dtype=np.float32)
     out = np.empty_like(x)
-    st = lib().rhdo_libm(ctypes.c_int({"cbrt": 0, "log2": 1}[fn]), _p(x), _p(out), _sz(x.size))
+    st = lib().rhdo_libm(ctypes.c_int({"cbrt": 0, "log2": 1, "cos": 2}[fn]), _p(x), _p(out),
+                         _sz(x.size))
     assert st == 0
     return out
 

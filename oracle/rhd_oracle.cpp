@@ -397,11 +397,13 @@ int rhdo_arw_probe(const uint8_t* file, size_t size, uint64_t out[12]) {
     return st;
 }
 
-// The platform libm's cbrtf (fn 0) or log2f (fn 1) over n floats: what the
-// reference's f32::cbrt and f32::log2 return on this host.
+// The platform libm's cbrtf (fn 0), log2f (fn 1) or cosf (fn 2) over n
+// floats: what the reference's f32::cbrt, f32::log2 and f32::cos return on
+// this host.
 int rhdo_libm(int fn, const float* in, float* out, size_t n) {
-    if (fn != 0 && fn != 1) return 1;
-    for (size_t i = 0; i < n; ++i) out[i] = fn == 0 ? std::cbrt(in[i]) : std::log2(in[i]);
+    if (fn < 0 || fn > 2) return 1;
+    for (size_t i = 0; i < n; ++i)
+        out[i] = fn == 0 ? std::cbrt(in[i]) : fn == 1 ? std::log2(in[i]) : std::cos(in[i]);
     return 0;
 }
 
