@@ -3,7 +3,8 @@
 // cells.rs:41-113, regions.rs:34-271, region_correction.rs:53-300 and
 // support.rs:26-125. f32 arithmetic with the reference's f64 accumulators;
 // cbrt/log2 come from the platform libm, which is what Rust's f32 methods
-// call on Linux.
+// call on Linux. Templated on the value type: T = double is the FP64 twin
+// (the same steps in double precision, with double libm calls).
 #pragma once
 
 #include <algorithm>
@@ -24,8 +25,9 @@ struct Stats {
         suspected_by_channel[3];
 };
 
+template <class T>
 struct Cell {
-    float means[3];
+    T means[3];
     uint8_t flags;
     bool usable(int c) const { return flags & (1 << c); }
     bool clean(int c) const { return !(flags & (1 << (c + 3))); }
@@ -34,12 +36,14 @@ struct Cell {
 inline int popcount(unsigned m) { return __builtin_popcount(m); }
 
 // support.rs:26-33
-inline bool bright_clean(const Cell& cell, int c, const float levels[3]) {
-    return cell.usable(c) && cell.clean(c) && cell.means[c] > 0.2f * levels[c];
+template <class T>
+inline bool bright_clean(const Cell<T>& cell, int c, const T levels[3]) {
+    return cell.usable(c) && cell.clean(c) && cell.means[c] > RHDO_LIT(T, 0.2) * levels[c];
 }
 
 // support.rs:37-46
-inline unsigned support_mask(const Cell& cell, int target, const float levels[3]) {
+template <class T>
+inline unsigned support_mask(const Cell<T>& cell, int target, const T levels[3]) {
     unsigned m = 0;
     for (int c = 0; c < 3; ++c)
         if (c != target && bright_clean(cell, c, levels)) m |= 1u << c;
@@ -47,59 +51,63 @@ inline unsigned support_mask(const Cell& cell, int target, const float levels[3]
 }
 
 // support.rs:51-72
-inline bool reference(const Cell& cell, unsigned support, float& out) {
+template <class T>
+inline bool reference(const Cell<T>& cell, unsigned support, T& out) {
     if (popcount(support) == 1) {
-        float v = cell.means[__builtin_ctz(support)];
-        if (finite(v) && v > 0.0f) {
+        T v = cell.means[__builtin_ctz(support)];
+        if (finite(v) && v > T(0)) {
             out = v;
             return true;
         }
         return false;
     }
-    float sum = 0.0f;
+    T sum = T(0);
     unsigned count = 0;
     for (int c = 0; c < 3; ++c) {
         if (!(support & (1u << c))) continue;
-        float v = cell.means[c];
-        if (!finite(v) || v <= 0.0f) return false;
+        T v = cell.means[c];
+        if (!finite(v) || v <= T(0)) return false;
         sum += RHDO_MUTANT(9) ? v : std::cbrt(v);  // 9: arithmetic instead of cube-root mean
         ++count;
     }
     if (count == 0) return false;
-    float root = sum / (float)count;
+    T root = sum / (T)count;
     out = RHDO_MUTANT(9) ? root : root * root * root;
     return true;
 }
 
 // support.rs:78-95
-inline bool correction(const Cell& cell, int target, unsigned support, const float levels[3],
-                       float& out) {
+template <class T>
+inline bool correction(const Cell<T>& cell, int target, unsigned support, const T levels[3],
+                       T& out) {
     if (!bright_clean(cell, target, levels)) return false;
     for (int c = 0; c < 3; ++c)
         if ((support & (1u << c)) && !bright_clean(cell, c, levels)) return false;
-    float ref;
+    T ref;
     if (!reference(cell, support, ref)) return false;
-    float v = cell.means[target] - ref;
+    T v = cell.means[target] - ref;
     if (!finite(v)) return false;
     out = v;
     return true;
 }
 
 // support.rs:101-110
-inline bool support_log_ratio(const Cell& cell, unsigned support, float& out) {
+template <class T>
+inline bool support_log_ratio(const Cell<T>& cell, unsigned support, T& out) {
     if (popcount(support) != 2) return false;
     int first = __builtin_ctz(support);
     int second = __builtin_ctz(support & ~(1u << first));
-    float ratio = cell.means[first] / cell.means[second];
-    if (!(finite(ratio) && ratio > 0.0f)) return false;
+    T ratio = cell.means[first] / cell.means[second];
+    if (!(finite(ratio) && ratio > T(0))) return false;
     out = std::log2(ratio);
     return true;
 }
 
 // support.rs:116-125
-inline bool chromaticity_agrees(unsigned support, bool has_t, float t, bool has_e, float e) {
+template <class T>
+inline bool chromaticity_agrees(unsigned support, bool has_t, T t, bool has_e, T e) {
     if (popcount(support) < 2) return true;
-    if (has_t && has_e) return std::fabs(t - e) <= (RHDO_MUTANT(3) ? 1.5f : 0.75f);  // 3: gate
+    if (has_t && has_e) return std::fabs(t - e) <= (RHDO_MUTANT(3) ? RHDO_LIT(T, 1.5) : RHDO_LIT(T, 0.75));  // 3: gate
     return false;
 }
 
@@ -121,32 +129,33 @@ inline int pattern_position(int channel, unsigned support) {
 }
 
 // cells.rs:41-113
-inline std::vector<Cell> build_cells(const float* m, size_t w, size_t h, int pattern,
-                                     const float levels[3], size_t& cw, size_t& ch) {
+template <class T>
+inline std::vector<Cell<T>> build_cells(const T* m, size_t w, size_t h, int pattern,
+                                     const T levels[3], size_t& cw, size_t& ch) {
     cw = w / 2 + w % 2;
     ch = h / 2 + h % 2;
-    std::vector<Cell> cells(cw * ch);
+    std::vector<Cell<T>> cells(cw * ch);
     for (size_t cy = 0; cy < ch; ++cy)
         for (size_t cx = 0; cx < cw; ++cx) {
-            float sums[3] = {0, 0, 0};
+            T sums[3] = {0, 0, 0};
             uint32_t counts[3] = {0, 0, 0};
             uint8_t flags = 0;
             for (size_t y = cy * 2; y < std::min(cy * 2 + 2, h); ++y)
                 for (size_t x = cx * 2; x < std::min(cx * 2 + 2, w); ++x) {
                     int c = color_at(pattern, x, y);
-                    float s = m[y * w + x];
+                    T s = m[y * w + x];
                     if (s >= levels[c]) {
                         flags |= 1 << (c + 3);
-                    } else if (s > 0.0f) {
+                    } else if (s > T(0)) {
                         sums[c] += s;
                         counts[c] += 1;
                     }
                 }
-            Cell& cell = cells[cy * cw + cx];
+            Cell<T>& cell = cells[cy * cw + cx];
             for (int c = 0; c < 3; ++c) {
-                cell.means[c] = 0.0f;
+                cell.means[c] = T(0);
                 if (counts[c] != 0) {
-                    cell.means[c] = sums[c] / (float)counts[c];
+                    cell.means[c] = sums[c] / (T)counts[c];
                     flags |= 1 << c;
                 }
             }
@@ -189,11 +198,12 @@ inline uint32_t label_components(uint32_t* labels, size_t w, size_t h) {
 }
 
 // region_correction.rs:53-110
+template <class T>
 struct Slot {
     uint32_t count = 0, kept = 0;
     double accum = 0, second = 0;
-    float mean = 0, tolerance = 0;
-    void add(bool trim, float value, bool has_ratio, float ratio) {
+    T mean = 0, tolerance = 0;
+    void add(bool trim, T value, bool has_ratio, T ratio) {
         double wide = (double)value;
         if (!trim) {
             count += 1;
@@ -202,7 +212,7 @@ struct Slot {
         } else if (std::fabs(value - mean) <= tolerance) {
             kept += 1;
             accum += wide;
-            second += (double)(has_ratio ? ratio : 0.0f);
+            second += (double)(has_ratio ? ratio : T(0));
         }
     }
     void finish_moments() {
@@ -210,44 +220,55 @@ struct Slot {
             double c = (double)count;
             double m = accum / c;
             double variance = num_max(second / c - m * m, 0.0);
-            mean = (float)m;
+            mean = (T)m;
             // 1: a 3-sigma trim window instead of 2 sigma
-            tolerance = (float)((RHDO_MUTANT(1) ? 3.0 : 2.0) * std::sqrt(variance)) + 1e-5f;
+            tolerance = (T)((RHDO_MUTANT(1) ? 3.0 : 2.0) * std::sqrt(variance)) + RHDO_LIT(T, 1e-5);
         }
         accum = 0;
         second = 0;
     }
-    bool value(bool has_ratio, float& offset, bool& ratio_present, float& ratio) const {
+    bool value(bool has_ratio, T& offset, bool& ratio_present, T& ratio) const {
         if (count < MIN_BORDER_CELLS || kept == 0) return false;
         double k = (double)kept;
-        offset = (float)(accum / k);
+        offset = (T)(accum / k);
         ratio_present = has_ratio;
-        ratio = has_ratio ? (float)(second / k) : 0.0f;
+        ratio = has_ratio ? (T)(second / k) : T(0);
         return true;
     }
 };
 
-// Rust's f32::total_cmp ordering.
-inline bool total_less(float a, float b) {
-    int32_t ia, ib;
-    std::memcpy(&ia, &a, 4);
-    std::memcpy(&ib, &b, 4);
-    ia ^= (int32_t)(((uint32_t)(ia >> 31)) >> 1);
-    ib ^= (int32_t)(((uint32_t)(ib >> 31)) >> 1);
-    return ia < ib;
+// Rust's f32::total_cmp ordering (f64::total_cmp for the f64 twin).
+template <class T>
+inline bool total_less(T a, T b) {
+    if constexpr (sizeof(T) == 4) {
+        int32_t ia, ib;
+        std::memcpy(&ia, &a, 4);
+        std::memcpy(&ib, &b, 4);
+        ia ^= (int32_t)(((uint32_t)(ia >> 31)) >> 1);
+        ib ^= (int32_t)(((uint32_t)(ib >> 31)) >> 1);
+        return ia < ib;
+    } else {
+        int64_t ia, ib;
+        std::memcpy(&ia, &a, 8);
+        std::memcpy(&ib, &b, 8);
+        ia ^= (int64_t)(((uint64_t)(ia >> 63)) >> 1);
+        ib ^= (int64_t)(((uint64_t)(ib >> 63)) >> 1);
+        return ia < ib;
+    }
 }
 
+template <class T>
 struct Context {
-    const float* levels;
-    const std::vector<Cell>* cells;
+    const T* levels;
+    const std::vector<Cell<T>>* cells;
     size_t cw, ch;
     const std::vector<uint32_t>* labels;
-    const std::vector<Slot>* slots;
+    const std::vector<Slot<T>>* slots;
 
     // opposed.rs:282-327
-    bool local_correction(int target, unsigned support, bool has_tr, float tr, size_t tx,
-                          size_t ty, float& out, int& out_radius) const {
-        float corr[MAX_CANDIDATES];
+    bool local_correction(int target, unsigned support, bool has_tr, T tr, size_t tx,
+                          size_t ty, T& out, int& out_radius) const {
+        T corr[MAX_CANDIDATES];
         int count = 0;
         for (int radius = FIRST_RADIUS; radius <= LAST_RADIUS; ++radius) {
             size_t y0 = ty >= (size_t)radius ? ty - radius : 0;
@@ -258,21 +279,21 @@ struct Context {
                 for (size_t cx = x0; cx <= x1; ++cx) {
                     size_t dx = cx > tx ? cx - tx : tx - cx, dy = cy > ty ? cy - ty : ty - cy;
                     if (std::max(dx, dy) != (size_t)radius) continue;
-                    const Cell& cand = (*cells)[cy * cw + cx];
+                    const Cell<T>& cand = (*cells)[cy * cw + cx];
                     if (!(cand.usable(target) && cand.clean(target))) continue;
-                    float value;
+                    T value;
                     if (!correction(cand, target, support, levels, value)) continue;
-                    float cr = 0;
+                    T cr = 0;
                     bool has_cr = support_log_ratio(cand, support, cr);
                     if (!chromaticity_agrees(support, has_tr, tr, has_cr, cr)) continue;
                     if (count < MAX_CANDIDATES) corr[count++] = value;
                 }
             if (count >= MIN_CANDIDATES) {
-                std::sort(corr, corr + count, total_less);
+                std::sort(corr, corr + count, total_less<T>);
                 int middle = count / 2;
                 // 4: the upper middle element for even counts, lower for odd
                 out = RHDO_MUTANT(4) ? corr[count % 2 == 0 ? middle : middle - 1]
-                      : count % 2 == 0 ? (corr[middle - 1] + corr[middle]) * 0.5f : corr[middle];
+                      : count % 2 == 0 ? (corr[middle - 1] + corr[middle]) * RHDO_LIT(T, 0.5) : corr[middle];
                 out_radius = radius;
                 return true;
             }
@@ -281,8 +302,8 @@ struct Context {
     }
 
     // region_correction.rs:270-282 (lookup)
-    bool lookup(uint32_t label, int target, unsigned support, float& offset, bool& has_ratio,
-                float& ratio) const {
+    bool lookup(uint32_t label, int target, unsigned support, T& offset, bool& has_ratio,
+                T& ratio) const {
         int p = pattern_position(target, support);
         if (p < 0) return false;
         int offs = target * 3 + p;
@@ -292,25 +313,25 @@ struct Context {
     }
 
     // opposed.rs:243-278
-    bool estimate_site(const Cell& target, int channel, size_t tx, size_t ty, float& out) const {
+    bool estimate_site(const Cell<T>& target, int channel, size_t tx, size_t ty, T& out) const {
         unsigned support = support_mask(target, channel, levels);
-        float target_reference;
+        T target_reference;
         if (!reference(target, support, target_reference)) return false;
-        float tr = 0;
+        T tr = 0;
         bool has_tr = support_log_ratio(target, support, tr);
-        float local = 0;
+        T local = 0;
         int radius = 0;
         bool has_local = local_correction(channel, support, has_tr, tr, tx, ty, local, radius);
         uint32_t label = (*labels)[ty * cw + tx];
-        float region = 0, border_ratio = 0;
+        T region = 0, border_ratio = 0;
         bool border_has_ratio = false;
         bool has_region = lookup(label, channel, support, region, border_has_ratio, border_ratio) &&
                           chromaticity_agrees(support, has_tr, tr, border_has_ratio, border_ratio);
-        float corr;
+        T corr;
         if (has_local && has_region) {
-            float weight = (float)(LAST_RADIUS + 1 - radius) / (float)LAST_RADIUS;
-            corr = RHDO_MUTANT(2) ? (1.0f - weight) * local + weight * region  // 2: swapped blend
-                                  : weight * local + (1.0f - weight) * region;
+            T weight = (T)(LAST_RADIUS + 1 - radius) / (T)LAST_RADIUS;
+            corr = RHDO_MUTANT(2) ? (T(1) - weight) * local + weight * region  // 2: swapped blend
+                                  : weight * local + (T(1) - weight) * region;
         } else if (has_local) {
             corr = local;
         } else if (has_region) {
@@ -318,16 +339,17 @@ struct Context {
         } else {
             return false;
         }
-        float estimate = target_reference + corr;
-        if (!(finite(estimate) && estimate > 0.0f)) return false;
+        T estimate = target_reference + corr;
+        if (!(finite(estimate) && estimate > T(0))) return false;
         out = estimate;
         return true;
     }
 };
 
-inline void neutralise(float& sample, const float* ceiling, int channel, uint64_t& neutralised) {
+template <class T>
+inline void neutralise(T& sample, const T* ceiling, int channel, uint64_t& neutralised) {
     if (!ceiling) return;
-    float limit = ceiling[channel];
+    T limit = ceiling[channel];
     if (sample > limit) {
         sample = limit;
         neutralised += 1;
@@ -336,20 +358,21 @@ inline void neutralise(float& sample, const float* ceiling, int channel, uint64_
 
 // opposed.rs:63-180. `labels_out` (cell grid, optional) receives the region
 // labels for parity checks of the segmentation.
-inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float levels[3],
-                       const float* neutral_ceiling, Stats* stats_out, uint32_t* labels_out,
+template <class T>
+inline int reconstruct(T* m, size_t w, size_t h, int pattern, const T levels[3],
+                       const T* neutral_ceiling, Stats* stats_out, uint32_t* labels_out,
                        uint32_t* region_count_out) {
     Stats st{};
     // lib.rs:108-119 (ChannelDetectionLevels::validate) and the same check on
     // the neutral ceiling: finite and positive, before any sample is read.
     for (int c = 0; c < 3; ++c) {
-        if (!(finite(levels[c]) && levels[c] > 0.0f)) return 8;
-        if (neutral_ceiling && !(finite(neutral_ceiling[c]) && neutral_ceiling[c] > 0.0f))
+        if (!(finite(levels[c]) && levels[c] > T(0))) return 8;
+        if (neutral_ceiling && !(finite(neutral_ceiling[c]) && neutral_ceiling[c] > T(0)))
             return 8;
     }
     for (size_t y = 0; y < h; ++y)
         for (size_t x = 0; x < w; ++x) {
-            float s = m[y * w + x];
+            T s = m[y * w + x];
             if (!finite(s)) return 8;
             int c = color_at(pattern, x, y);
             if (s >= levels[c]) {
@@ -364,7 +387,7 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
         if (stats_out) *stats_out = st;
         return 0;
     }
-    std::vector<Cell> cells = build_cells(m, w, h, pattern, levels, cw, chh);
+    std::vector<Cell<T>> cells = build_cells(m, w, h, pattern, levels, cw, chh);
     auto clipped_at = [&](size_t x, size_t y) { return (unsigned)(cells[y * cw + x].flags >> 3); };
     // regions.rs:126-183: Chebyshev radius-1 dilation of the clipped cells.
     std::vector<uint32_t> labels(cw * chh, 0);
@@ -392,7 +415,7 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
             uint32_t label = labels[i];
             unsigned clipped = clipped_at(x, y);
             if (label == 0 || clipped == 0) continue;
-            const Cell& cell = cells[i];
+            const Cell<T>& cell = cells[i];
             auto has_clean = [&](int c) {
                 for (int o = 0; o < 3; ++o)
                     if (o != c && cell.usable(o) && cell.clean(o)) return true;
@@ -410,18 +433,18 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
             }
         }
     // region_correction.rs:196-261: two raster-order passes over the rings.
-    std::vector<Slot> slots((size_t)(count + 1) * 9);
+    std::vector<Slot<T>> slots((size_t)(count + 1) * 9);
     for (int pass = 0; pass < 2; ++pass) {
         for (size_t y = 0; y < chh; ++y)
             for (size_t x = 0; x < cw; ++x) {
                 size_t i = y * cw + x;
                 uint32_t label = labels[i];
                 if (label == 0 || clipped_at(x, y) != 0) continue;
-                const Cell& cell = cells[i];
+                const Cell<T>& cell = cells[i];
                 for (int c = 0; c < 3; ++c) {
                     if (!(region_clipped[label] & (1u << c))) continue;
                     // 10: ring evidence admitted from a quarter of the level
-                    if (cell.means[c] < (RHDO_MUTANT(10) ? 0.25f : 0.5f) * levels[c]) continue;
+                    if (cell.means[c] < (RHDO_MUTANT(10) ? RHDO_LIT(T, 0.25) : RHDO_LIT(T, 0.5)) * levels[c]) continue;
                     unsigned pats[3];
                     support_patterns(c, pats);
                     for (int p = 0; p < 3; ++p) {
@@ -429,9 +452,9 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                         bool for_region = wanted[label] & (1u << offs);
                         bool for_pool = any_wanted & (1u << offs);
                         if (!for_region && !for_pool) continue;
-                        float value;
+                        T value;
                         if (!correction(cell, c, pats[p], levels, value)) continue;
-                        float ratio = 0;
+                        T ratio = 0;
                         bool has_ratio = support_log_ratio(cell, pats[p], ratio);
                         if (for_region) slots[label * 9 + offs].add(pass == 1, value, has_ratio, ratio);
                         if (for_pool) slots[offs].add(pass == 1, value, has_ratio, ratio);
@@ -439,21 +462,21 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                 }
             }
         if (pass == 0)
-            for (Slot& s : slots) s.finish_moments();
+            for (Slot<T>& s : slots) s.finish_moments();
     }
-    Context ctx{levels, &cells, cw, chh, &labels, &slots};
+    Context<T> ctx{levels, &cells, cw, chh, &labels, &slots};
     for (size_t y = 0; y < h; ++y)
         for (size_t x = 0; x < w; ++x) {
-            float& sample = m[y * w + x];
+            T& sample = m[y * w + x];
             int c = color_at(pattern, x, y);
-            float level = levels[c];
+            T level = levels[c];
             if (!(sample >= level)) continue;
             size_t tx = x / 2, ty = y / 2;
-            const Cell& target = cells[ty * cw + tx];
+            const Cell<T>& target = cells[ty * cw + tx];
             int support_count = 0;
             for (int o = 0; o < 3; ++o)
                 if (o != c && target.usable(o) && target.clean(o)) ++support_count;
-            float estimate = 0;
+            T estimate = 0;
             bool has = support_count != 0 && ctx.estimate_site(target, c, tx, ty, estimate);
             if (!has) {
                 st.fallback++;
@@ -462,7 +485,7 @@ inline int reconstruct(float* m, size_t w, size_t h, int pattern, const float le
                 continue;
             }
             st.reconstructed++;
-            float bounded = num_min(estimate, (RHDO_MUTANT(11) ? 2.0f : 4.0f) * level);  // 11
+            T bounded = num_min(estimate, (RHDO_MUTANT(11) ? RHDO_LIT(T, 2.0) : RHDO_LIT(T, 4.0)) * level);  // 11
             if (bounded > sample) {
                 sample = bounded;
                 st.changed++;

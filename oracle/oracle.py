@@ -59,6 +59,35 @@ def normalize(mosaic: np.ndarray, sensor_pattern: int, black: np.ndarray, repeat
     return out
 
 
+def normalize_f64(mosaic: np.ndarray, sensor_pattern: int, black: np.ndarray, repeat_w: int,
+                  repeat_h: int, white: np.ndarray, crop) -> np.ndarray:
+    """The FP64 twin of normalize."""
+    mosaic = np.ascontiguousarray(mosaic, dtype=np.uint16)
+    black = np.ascontiguousarray(black, dtype=np.float32)
+    white = np.ascontiguousarray(white, dtype=np.float32)
+    cx, cy, cw, ch = crop
+    out = np.empty((ch, cw), dtype=np.float64)
+    lib().rhdo_normalize_f64(_p(mosaic), _sz(mosaic.shape[1]), ctypes.c_int(sensor_pattern),
+                             _p(black), _sz(repeat_w), _sz(repeat_h), _p(white), _sz(white.size),
+                             _sz(cx), _sz(cy), _sz(cw), _sz(ch), _p(out))
+    return out
+
+
+def opposed_f64(mosaic: np.ndarray, pattern: int, detection, neutral_ceiling=None):
+    """The FP64 twin of opposed. Returns (status, out, stats[9])."""
+    m = np.ascontiguousarray(mosaic, dtype=np.float64).copy()
+    h, w = m.shape
+    det = np.asarray(detection, dtype=np.float64)
+    stats = np.zeros(9, dtype=np.uint64)
+    ceil_ptr = ctypes.c_void_p(0)
+    if neutral_ceiling is not None:
+        ceil = np.asarray(neutral_ceiling, dtype=np.float64)
+        ceil_ptr = _p(ceil)
+    st = lib().rhdo_opposed_f64(_p(m), _sz(w), _sz(h), ctypes.c_int(pattern), _p(det), ceil_ptr,
+                                _p(stats))
+    return st, m, stats
+
+
 def clip(mosaic: np.ndarray, pattern: int, limits) -> tuple[int, np.ndarray, np.ndarray]:
     """crates/highlight/src/clip.rs:24-72. Returns (status, clipped, stats[6])."""
     m = np.ascontiguousarray(mosaic, dtype=np.float32).copy()

@@ -31,6 +31,23 @@ int rhdo_normalize(const uint16_t* mosaic, size_t row_stride, int sensor_pattern
     return 0;
 }
 
+// The FP64 twin of rhdo_normalize: (u16 - black) / (white - black) in double.
+int rhdo_normalize_f64(const uint16_t* mosaic, size_t row_stride, int sensor_pattern,
+                       const float* black, size_t repeat_w, size_t repeat_h, const float* white,
+                       size_t white_count, size_t crop_x, size_t crop_y, size_t crop_w,
+                       size_t crop_h, double* out) {
+    LevelPattern lv{black, repeat_w, repeat_h, white, white_count};
+    for (size_t y = 0; y < crop_h; ++y)
+        for (size_t x = 0; x < crop_w; ++x) {
+            size_t sx = crop_x + x, sy = crop_y + y;
+            float b, w;
+            levels_at(lv, sensor_pattern, sx, sy, b, w);
+            out[y * crop_w + x] =
+                ((double)mosaic[sy * row_stride + sx] - (double)b) / ((double)w - (double)b);
+        }
+    return 0;
+}
+
 int rhdo_clip(float* mosaic, size_t width, size_t height, int pattern, const float limits[3],
               uint64_t stats_out[6]) {
     ClipStats st{};
@@ -138,6 +155,22 @@ int rhdo_opposed(float* mosaic, size_t width, size_t height, int pattern,
         stats_out[4] = st.fully_unsupported;
         stats_out[5] = st.neutralised;
         for (int c = 0; c < 3; ++c) stats_out[6 + c] = st.suspected_by_channel[c];
+    }
+    return status;
+}
+
+// The FP64 twin of rhdo_opposed: the same algorithm in double precision.
+int rhdo_opposed_f64(double* mosaic, size_t width, size_t height, int pattern,
+                     const double detection[3], const double* neutral_ceiling,
+                     uint64_t stats_out[9]) {
+    opposed::Stats st{};
+    int status = opposed::reconstruct(mosaic, width, height, pattern, detection, neutral_ceiling,
+                                      &st, nullptr, nullptr);
+    if (stats_out) {
+        const uint64_t v[9] = {st.suspected, st.reconstructed, st.changed, st.fallback,
+                               st.fully_unsupported, st.neutralised, st.suspected_by_channel[0],
+                               st.suspected_by_channel[1], st.suspected_by_channel[2]};
+        std::memcpy(stats_out, v, sizeof(v));
     }
     return status;
 }
