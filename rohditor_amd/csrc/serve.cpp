@@ -83,6 +83,7 @@ struct ResidentMosaic {
 struct BaseEntry {
     std::string key;  // frame id + sensor recipe
     uint64_t frame = 0;
+    uint64_t generation = 0;  // the frame's content generation it was developed from
     rhd_source* source = nullptr;
     int users = 0;
 };
@@ -139,6 +140,10 @@ struct rhd_server {
     std::unordered_map<uint64_t, Request*> requests;  // not yet collected by rhd_server_wait
     std::unordered_map<uint64_t, ResidentMosaic> mosaics;
     std::list<BaseEntry> bases;  // most recently used first
+    // Content generation per frame id, bumped whenever new samples are
+    // installed under the id (never reset, also across drops): a base is
+    // valid only for the generation it was developed from.
+    std::unordered_map<uint64_t, uint64_t> generations;
     uint64_t base_bytes = 0;
     uint64_t running_bytes = 0;
     uint64_t next_id = 1;
@@ -189,6 +194,7 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
     (void)hipSetDevice(ctx->device);
     rhd_mosaic* mosaic = nullptr;
     BaseEntry* base = nullptr;
+    uint64_t generation = 0;
     {
         std::lock_guard<std::mutex> g(s->mu);
         auto m = s->mosaics.find(r->frame);
@@ -196,9 +202,11 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
             mosaic = m->second.mosaic;
             m->second.users += 1;
         }
+        auto gen = s->generations.find(r->frame);
+        generation = gen == s->generations.end() ? 0 : gen->second;
         if (s->cfg.base_cache_bytes)
             for (auto it = s->bases.begin(); it != s->bases.end(); ++it)
-                if (it->key == r->base_key) {
+                if (it->key == r->base_key && it->generation == generation) {
                     s->bases.splice(s->bases.begin(), s->bases, it);
                     base = &s->bases.front();
                     base->users += 1;
@@ -240,7 +248,7 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
             std::lock_guard<std::mutex> g(s->mu);
             evict_bases_locked(s, fresh->bytes);
             if (s->base_bytes + fresh->bytes <= s->cfg.base_cache_bytes) {
-                s->bases.push_front(BaseEntry{r->base_key, r->frame, fresh, 0});
+                s->bases.push_front(BaseEntry{r->base_key, r->frame, generation, fresh, 0});
                 s->base_bytes += fresh->bytes;
                 fresh = nullptr;
             }
@@ -257,7 +265,21 @@ void run_request(rhd_server* s, Lane& L, Request* r) {
         r->metrics.total_ms += ms;
     }
     std::lock_guard<std::mutex> g(s->mu);
-    if (base) base->users -= 1;
+    if (base) {
+        base->users -= 1;
+        // A base whose frame got new samples while this request used it is
+        // released as soon as it is free.
+        auto gen = s->generations.find(base->frame);
+        if (base->users == 0 && gen != s->generations.end() && gen->second != base->generation) {
+            for (auto it = s->bases.begin(); it != s->bases.end(); ++it)
+                if (&*it == base) {
+                    s->base_bytes -= it->source->bytes;
+                    rhd_source_release(it->source);
+                    s->bases.erase(it);
+                    break;
+                }
+        }
+    }
     if (mosaic) {
         auto m = s->mosaics.find(r->frame);
         if (m != s->mosaics.end()) {
@@ -491,7 +513,10 @@ rhd_status install_mosaic(rhd_server* s, uint64_t frame_id, rhd_mosaic* m) {
         }
         slot.mosaic = m;
         slot.dropped = false;
-        // Bases of the replaced frame no longer match it.
+        // Bases of the earlier samples no longer match the frame: the new
+        // generation keeps them from being served; the unused ones go now,
+        // the ones still in use when their last request finishes.
+        s->generations[frame_id] += 1;
         for (auto it = s->bases.begin(); it != s->bases.end();)
             if (it->frame == frame_id && it->users == 0) {
                 s->base_bytes -= it->source->bytes;
