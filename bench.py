@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--batch-lanes", type=int, default=0, help="0: autotune")
     p.add_argument("--served-lanes", type=int, default=4)
     p.add_argument("--served-requests", type=int, default=60)
+    p.add_argument("--served-poisson", type=int, default=200,
+                   help="requests of the Poisson serving leg in the default line (0: skip)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain device copy of 1 GiB (read) into 1 GiB (write), the "
                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -183,13 +185,23 @@ def workload():
 
 
 def _cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with its family / model /
+    stepping (a virtualised host may report only a generic name)."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            key, _, value = line.partition(":")
+            key = key.strip()
+            if key in ("model name", "cpu family", "model", "stepping") and key not in fields:
+                fields[key] = value.strip()
+            if key == "flags" and "flags" not in fields:
+                flags = value.split()
+                fields["flags"] = " ".join(f for f in ("avx2", "fma", "avx512f") if f in flags)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{fields.get('model name', 'unknown')} (family {fields.get('cpu family', '?')}, "
+            f"model {fields.get('model', '?')}, stepping {fields.get('stepping', '?')}; "
+            f"{fields.get('flags', '')})")
 
 
 def _cpu_chain(frame, recipe, options, threads):
@@ -213,22 +225,30 @@ def _cpu_chain(frame, recipe, options, threads):
 
 
 def cpu_baseline(frame, recipe, options):
-    """The oracle's restatement of the same chain: the full frame on every
-    host core (1 warm-up, median of 5), and a 1536x1024 crop of it on one
-    core (median of 3)."""
+    """The oracle's restatement of the same chain, built -O3 -march=native
+    for this host (contraction off, so its results are the checker's): the
+    full frame on every host core (1 warm-up, median of 5), and a 1536x1024
+    crop of it on one core (median of 3)."""
     from oracle import oracle
     from rohditor_amd import synth
     threads = oracle.cpu_threads()
-    _cpu_chain(frame, recipe, options, threads)
-    runs = [_cpu_chain(frame, recipe, options, threads) for _ in range(5)]
-    dt = float(np.median([r[0] for r in runs]))
-    mp = runs[0][1]
+    build = "-O3 -march=native -ffp-contract=off"
+    try:
+        native = oracle.native_build()
+    except Exception:  # no compiler on this host: the -O2 checker build
+        native, build = None, "-O2 -ffp-contract=off (checker build)"
     crop = synth.frame_from_mosaic(np.ascontiguousarray(frame.mosaic[:1024, :1536]),
                                    frame.info.cfa, float(frame.info.black_levels.values[0]),
                                    float(frame.info.white_levels[0]))
-    single = [_cpu_chain(crop, recipe, options, 1) for _ in range(3)]
+    with (oracle.using(native) if native else _nullcontext()):
+        _cpu_chain(frame, recipe, options, threads)
+        runs = [_cpu_chain(frame, recipe, options, threads) for _ in range(5)]
+        single = [_cpu_chain(crop, recipe, options, 1) for _ in range(3)]
+    dt = float(np.median([r[0] for r in runs]))
+    mp = runs[0][1]
     dt1 = float(np.median([r[0] for r in single]))
     return {"value": round(mp / dt, 3), "unit": "MP/s", "cores": threads, "kind": "port",
+            "build": build,
             "sample": f"one full {frame.info.width}x{frame.info.height} frame, "
                       f"normalize+Opposed+AMaZE+RL({recipe.capture_iterations}), median of 5 "
                       f"after 1 warm-up: {dt:.2f} s",
@@ -236,6 +256,14 @@ def cpu_baseline(frame, recipe, options):
                             "sample": f"{crop.info.width}x{crop.info.height} crop of the frame, "
                                       f"median of 3: {dt1:.2f} s"},
             "cpu_model": _cpu_model()}
+
+
+class _nullcontext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
 
 
 def main():
@@ -331,7 +359,12 @@ def main():
     for name in (dominant, dominant + "<3>"):
         traffic = traffic if traffic is not None else measured_traffic(name)
         valu_busy = valu_busy if valu_busy is not None else measured_valu_busy(name)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+    # The roofline priced here is HBM's; the kernel's own limiter comes from
+    # its counters: vector-ALU issue when VALUBusy exceeds the HBM fraction.
+    frac = achieved / HBM_PEAK_GBPS
+    limiter = "valu" if valu_busy is not None and valu_busy / 100.0 > frac else "hbm"
+    roofline = {"bound": "hbm", "limiter": limiter,
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": None if traffic is None else int(traffic),
                 "traffic_source": None if traffic is None else str(TRAFFIC_FILE.relative_to(ROOT)),
@@ -347,6 +380,12 @@ def main():
              "stage_frac": round(mp * 1e6 * 14 / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
 
     served_line = None if args.no_served else served(frame, desc, detail, value, args)
+    poisson = None
+    if not args.no_served and args.served_poisson > 0:
+        sp, pp = poisson_legs(args, args.served_poisson, args.serve_rate)
+        poisson = {"served": sp, "plain_serial": pp, "requests": args.served_poisson,
+                   "rate_per_s": args.serve_rate, "lanes": args.served_lanes,
+                   "workload": poisson_workload(args.serve_rate)}
     cpu = None if args.no_cpu_baseline else cpu_baseline(frame, recipe, options)
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -365,6 +404,7 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(
             prof.items(), key=lambda kv: -kv[1][0])},
         "served": served_line,
+        "served_poisson": poisson,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -372,7 +412,8 @@ def main():
 
 def served(frame, desc, detail, plain_value, args):
     """The same develop through the serving loop (include/rhd.h rhd_server_*):
-    a closed loop keeping 2 x lanes requests in flight, each on its own
+    a closed loop keeping 2 x lanes requests in flight (its latency includes
+    the queueing behind them), each on its own
     resident frame id (so newest-wins never coalesces them), no base cache
     (every request runs the whole stage). Latency is submit -> RGB ready."""
     from rohditor_amd import sensor
@@ -495,9 +536,9 @@ def batch_bench(args):
 SERVE_SIZES = [(6048, 4024), (7008, 4672), (8256, 5504)]
 
 
-def serve_bench(args):
-    """The serving workload: 64 resident frames of 24, 33 and 45 MP; Poisson
-    arrivals (`--serve-rate` per second, seeded); 70 % re-develops of a frame
+def poisson_legs(args, n, rate):
+    """The serving workload: 64 resident frames of 24, 33 and 45 MP; `n`
+    Poisson arrivals (`rate` per second, seeded); 70 % re-develops of a frame
     already developed that change its highlight method or demosaic
     algorithm, 30 % first develops of a frame; RL (sigma 0.7, 10 iterations)
     throughout. The server runs with newest-wins, admission by device bytes
@@ -506,7 +547,7 @@ def serve_bench(args):
     import threading
     from rohditor_amd import sensor, synth
     rng = np.random.default_rng(0x52480444)
-    frames_n, n = 64, args.serve_requests
+    frames_n = 64
     scenes = [synth.mosaic(synth.SceneSpec(width=w, height=h, seed=0x52480400 + k,
                                            blob_fraction=0.05))
               for k, (w, h) in enumerate(SERVE_SIZES)]
@@ -515,7 +556,7 @@ def serve_bench(args):
     methods = [sensor.HighlightMethod.OPPOSED, sensor.HighlightMethod.CLIP]
     algos = [sensor.DemosaicAlgorithm.AMAZE, sensor.DemosaicAlgorithm.RCD,
              sensor.DemosaicAlgorithm.MHC]
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     developed, current, reqs = [], {}, []
     for i in range(n):
         if developed and rng.random() < 0.7:
@@ -608,6 +649,12 @@ def serve_bench(args):
                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
                   "busy_mp_per_s": round(total_mp / busy, 2), "completed": n,
                   "span_s": round(span, 3)}
+    return served_line, plain_line
+
+
+def serve_bench(args):
+    served_line, plain_line = poisson_legs(args, args.serve_requests, args.serve_rate)
+    n = args.serve_requests
     line = {
         "metric": "served sensor MP/s and p50/p99 develop latency vs plain serial GPU path",
         "value": served_line["value"], "unit": "MP/s", "n_gpus": 1, "higher_is_better": True,
@@ -615,13 +662,16 @@ def serve_bench(args):
         "served": served_line, "plain_serial": plain_line,
         "latency_p99_vs_plain": round(plain_line["p99_ms"] / max(served_line["p99_ms"], 1e-9), 3),
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"Poisson arrivals ({args.serve_rate}/s, seed 0x52480444) over 64 "
-                               "resident frames of 6048x4024, 7008x4672 and 8256x5504; 70 % "
-                               "re-develops changing highlight (Opposed/Clip) or demosaic "
-                               "(AMaZE/RCD/MHC), 30 % first develops; RL sigma 0.7, 10 "
-                               "iterations; newest-wins, 48 GiB admission, 16 GiB base cache"},
+        "config": {"workload": poisson_workload(args.serve_rate)},
     }
     print(json.dumps(line))
+
+
+def poisson_workload(rate):
+    return (f"Poisson arrivals ({rate}/s, seed 0x52480444) over 64 resident frames of "
+            "6048x4024, 7008x4672 and 8256x5504; 70 % re-develops changing highlight "
+            "(Opposed/Clip) or demosaic (AMaZE/RCD/MHC), 30 % first develops; RL sigma 0.7, "
+            "10 iterations; newest-wins, 48 GiB admission, 16 GiB base cache")
 
 
 C5_SIZES = [(11648, 8736), (6048, 4024), (6048, 4024), (6048, 4024)]

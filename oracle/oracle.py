@@ -468,14 +468,9 @@ def build_mutants() -> None:
 
 
 @contextlib.contextmanager
-def mutant(k: int):
-    """Within the block every oracle function runs the build that carries
-    planted mistake `k` (MUTATIONS[k]); the checker is restored afterwards."""
-    global _lib, _mutants_built
-    path = _HERE / "_build" / f"librhd_oracle_m{k}.so"
-    if not _mutants_built:
-        build_mutants()
-        _mutants_built = True
+def using(path: Path):
+    """Within the block every oracle function runs the build at `path`."""
+    global _lib
     saved = lib()
     _lib = ctypes.CDLL(str(path))
     _lib.rhdo_amaze_plane_names.restype = ctypes.c_char_p
@@ -483,3 +478,22 @@ def mutant(k: int):
         yield
     finally:
         _lib = saved
+
+
+@contextlib.contextmanager
+def mutant(k: int):
+    """Within the block every oracle function runs the build that carries
+    planted mistake `k` (MUTATIONS[k]); the checker is restored afterwards."""
+    global _mutants_built
+    if not _mutants_built:
+        build_mutants()
+        _mutants_built = True
+    with using(_HERE / "_build" / f"librhd_oracle_m{k}.so"):
+        yield
+
+
+def native_build() -> Path:
+    """The -O3 -march=native build of the same sources (make native), for the
+    CPU baseline on the host it runs on."""
+    subprocess.run(["make", "-s", "-C", str(_HERE), "native"], check=True)
+    return _HERE / "_build" / "librhd_oracle_native.so"
