@@ -54,15 +54,20 @@ __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rg
     mask[i] = 1.0f - smoothstep(0.85f, 0.98f, peak);
 }
 
+// The reference's sums start from 0.0 (interior, vertical) or -0.0 (the
+// mirrored horizontal border). Every plane blurred here is >= +0 (the mask
+// in [0, 1], the guide >= its floor, the estimate and the ratio positive),
+// so the first product p is >= +0 and 0.0 + p == -0.0 + p == p exactly: the
+// sums start from the first product instead, one add fewer per output.
+
 // Horizontal taps at image column x of an LDS row whose first element is
-// image column `col0`; the reference's mirrored border `.sum()` starts at -0.0.
+// image column `col0`.
 template <int R, bool FAST>
 __device__ __forceinline__ float hblur(const float* row, int col0, int x, int width,
                                        const Weights& k) {
-    const bool border = width <= 2 * R || x < R || x >= width - R;
-    float v = border ? -0.0f : 0.0f;
+    float v = row[(FAST ? x - R : mirror_half(x - R, width)) - col0] * k.w[0];
 #pragma unroll
-    for (int t = 0; t <= 2 * R; ++t) {
+    for (int t = 1; t <= 2 * R; ++t) {
         const int src = FAST ? x + t - R : mirror_half(x + t - R, width);
         v += row[src - col0] * k.w[t];
     }
@@ -74,9 +79,9 @@ __device__ __forceinline__ float hblur(const float* row, int col0, int x, int wi
 template <int R, bool FAST>
 __device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, int y, int c,
                                        int height, const Weights& k) {
-    float v = 0.0f;
+    float v = plane[((FAST ? y - R : mirror_half(y - R, height)) - row0) * pitch + c] * k.w[0];
 #pragma unroll
-    for (int t = 0; t <= 2 * R; ++t) {
+    for (int t = 1; t <= 2 * R; ++t) {
         const int src = FAST ? y + t - R : mirror_half(y + t - R, height);
         v += plane[(src - row0) * pitch + c] * k.w[t];
     }
@@ -285,9 +290,9 @@ __device__ __forceinline__ float4 hstrip(const float* src, const Weights& k) {
     float o[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float v = 0.0f;
+        float v = x[q] * k.w[0];
 #pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
+        for (int t = 1; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
         o[q] = v;
     }
     return make_float4(o[0], o[1], o[2], o[3]);
@@ -301,9 +306,9 @@ __device__ __forceinline__ void vstrip(const float* src, const Weights& k, float
     for (int i = 0; i < 4 + 2 * R; ++i) x[i] = src[i * PITCH];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float v = 0.0f;
+        float v = x[q] * k.w[0];
 #pragma unroll
-        for (int t = 0; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
+        for (int t = 1; t <= 2 * R; ++t) v += x[q + t] * k.w[t];
         o[q] = v;
     }
 }

@@ -584,9 +584,13 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
 
 constexpr int SUM_BATCH = 256;
 constexpr int SUM_PRODUCERS = SUM_BATCH / 64;
-constexpr int SUM_THREADS = 64 * (1 + SUM_PRODUCERS);
+// Two consumer wavefronts: wave 0 adds the first accumulator of every slot,
+// wave 1 the second. They run on different SIMDs, so each carries one
+// dependent f64 chain instead of two interleaved ones.
+constexpr int SUM_CONSUMERS = 2;
+constexpr int SUM_THREADS = 64 * (SUM_CONSUMERS + SUM_PRODUCERS);
 constexpr int SUM_UNROLL = 4;
-static_assert((SUM_BATCH / 2) % (2 * SUM_UNROLL) == 0, "whole register groups per batch");
+static_assert((SUM_BATCH / 2) % (4 * SUM_UNROLL) == 0, "whole register groups per batch");
 
 // One ring entry in a producer lane's registers: its slot flags (zero when it
 // belongs to another region), nine samples and three ratios.
@@ -621,25 +625,27 @@ struct SumStage {
     unsigned staged[SUM_PRODUCERS];
 };
 
-// One pass of every slot of one group over ring entries [begin, end). Waves
-// 1..SUM_PRODUCERS each load a quarter of batch i (their share of the next
-// batch's loads in flight behind it) and turn it into the terms each slot
-// adds: +0.0 for an entry that does not count, which leaves an accumulator
-// that starts at +0.0 unchanged, and ballot counts. Meanwhile lane s of wave 0
-// adds slot s's terms of batch i - 1 in raster order, one f64 add at a time,
-// as the reference's sequential loop does. The two stage buffers alternate.
+// One pass of every slot of one group over ring entries [begin, end). The
+// SUM_PRODUCERS producer waves each load a quarter of batch i (their share of
+// the next batch's loads in flight behind it) and turn it into the terms each
+// slot adds: +0.0 for an entry that does not count, which leaves an
+// accumulator that starts at +0.0 unchanged, and ballot counts. Meanwhile
+// lane s of consumer wave 0 (first accumulator) and of consumer wave 1
+// (second accumulator) add slot s's terms of batch i - 1 in raster order, one
+// f64 add at a time, as the reference's sequential loop does. The two stage
+// buffers alternate.
 template <int PASS>
 __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned begin,
                                          unsigned end, const float* means, const float* tols,
                                          SumStage* st, unsigned& count, double& accum,
                                          double& second) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int q = wave - 1, e = q * 64 + lane;
+    const int q = wave - SUM_CONSUMERS, e = q * 64 + lane;
     const unsigned batches = begin < end ? (end - begin + SUM_BATCH - 1) / SUM_BATCH : 0u;
     SumEntry b;
-    if (wave != 0 && batches != 0) sum_load(L, g, begin + e, end, b);
+    if (wave >= SUM_CONSUMERS && batches != 0) sum_load(L, g, begin + e, end, b);
     for (unsigned i = 0; i <= batches; ++i) {
-        if (wave != 0 && i < batches) {
+        if (wave >= SUM_CONSUMERS && i < batches) {
             SumStage& out = st[i & 1];
             const unsigned base = begin + i * SUM_BATCH;
             // Every producer writes its terms, so a batch that another
@@ -673,66 +679,62 @@ __device__ __forceinline__ void sum_pass(const Layout& L, unsigned g, unsigned b
             if (lane == 0) out.staged[q] = staged;
             if (base + SUM_BATCH < end) sum_load(L, g, base + SUM_BATCH + e, end, b);
         }
-        if (wave == 0 && i >= 1 && lane < SLOTS) {
+        if (wave < SUM_CONSUMERS && i >= 1 && lane < SLOTS) {
             const SumStage& in = st[(i - 1) & 1];
             unsigned staged = 0;
 #pragma unroll
             for (int p = 0; p < SUM_PRODUCERS; ++p) staged |= in.staged[p];
             if (staged) {
+                if (wave == 0) {
 #pragma unroll
-                for (int p = 0; p < SUM_PRODUCERS; ++p) count += in.count[p][lane];
-                const double2* pa = reinterpret_cast<const double2*>(in.a + lane * SUM_PITCH);
-                const double2* pb = reinterpret_cast<const double2*>(in.b + lane * SUM_PITCH);
+                    for (int p = 0; p < SUM_PRODUCERS; ++p) count += in.count[p][lane];
+                }
+                // Wave 0 adds slot `lane`'s first terms, wave 1 its second.
+                const double2* pa = reinterpret_cast<const double2*>(
+                    (wave == 0 ? in.a : in.b) + lane * SUM_PITCH);
+                double acc = wave == 0 ? accum : second;
                 // Two register groups alternate: the next group's LDS reads
                 // are in flight while the current group's terms are added.
-                double2 xa[SUM_UNROLL], xb[SUM_UNROLL], ya[SUM_UNROLL], yb[SUM_UNROLL];
+                double2 xa[2 * SUM_UNROLL], ya[2 * SUM_UNROLL];
 #pragma unroll
-                for (int u = 0; u < SUM_UNROLL; ++u) {
-                    xa[u] = pa[u];
-                    xb[u] = pb[u];
-                }
-                for (int j = 0; j < SUM_BATCH / 2; j += 2 * SUM_UNROLL) {
+                for (int u = 0; u < 2 * SUM_UNROLL; ++u) xa[u] = pa[u];
+                for (int j = 0; j < SUM_BATCH / 2; j += 4 * SUM_UNROLL) {
 #pragma unroll
-                    for (int u = 0; u < SUM_UNROLL; ++u) {
-                        ya[u] = pa[j + SUM_UNROLL + u];
-                        yb[u] = pb[j + SUM_UNROLL + u];
+                    for (int u = 0; u < 2 * SUM_UNROLL; ++u) ya[u] = pa[j + 2 * SUM_UNROLL + u];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 2 * SUM_UNROLL; ++u) {
+                        acc += xa[u].x;
+                        acc += xa[u].y;
+                    }
+                    if (j + 4 * SUM_UNROLL < SUM_BATCH / 2) {
+#pragma unroll
+                        for (int u = 0; u < 2 * SUM_UNROLL; ++u) xa[u] = pa[j + 4 * SUM_UNROLL + u];
                     }
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int u = 0; u < SUM_UNROLL; ++u) {
-                        accum += xa[u].x;
-                        second += xb[u].x;
-                        accum += xa[u].y;
-                        second += xb[u].y;
-                    }
-                    if (j + 2 * SUM_UNROLL < SUM_BATCH / 2) {
-#pragma unroll
-                        for (int u = 0; u < SUM_UNROLL; ++u) {
-                            xa[u] = pa[j + 2 * SUM_UNROLL + u];
-                            xb[u] = pb[j + 2 * SUM_UNROLL + u];
-                        }
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int u = 0; u < SUM_UNROLL; ++u) {
-                        accum += ya[u].x;
-                        second += yb[u].x;
-                        accum += ya[u].y;
-                        second += yb[u].y;
+                    for (int u = 0; u < 2 * SUM_UNROLL; ++u) {
+                        acc += ya[u].x;
+                        acc += ya[u].y;
                     }
                 }
+                if (wave == 0)
+                    accum = acc;
+                else
+                    second = acc;
             }
         }
         __syncthreads();
     }
 }
 
-// One workgroup (a consumer wavefront and SUM_PRODUCERS producers) per group
+// One workgroup (two consumer wavefronts and SUM_PRODUCERS producers) per group
 // (0 = the image-wide pool, r = region r); lane s of wave 0 owns slot s
 // (region_correction.rs:196-261).
 __global__ __launch_bounds__(SUM_THREADS) void opp_sums(Layout L) {
     __shared__ __align__(16) SumStage s_stage[2];
     __shared__ float s_mean[SLOTS], s_tol[SLOTS];
+    __shared__ double s_second[SLOTS];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned count = min(L.scalars[0], (unsigned)L.cap - 1);
     const unsigned ring = L.scalars[1];
@@ -752,7 +754,10 @@ __global__ __launch_bounds__(SUM_THREADS) void opp_sums(Layout L) {
 #pragma unroll
         for (int t = 0; t < SLOTS; ++t) means[t] = tols[t] = 0.0f;
         sum_pass<0>(L, g, begin, end, means, tols, s_stage, n, accum, second);
+        if (wave == 1 && lane < SLOTS) s_second[lane] = second;
+        __syncthreads();
         if (wave == 0 && lane < SLOTS) {
+            second = s_second[lane];
             float mean = 0.0f, tolerance = 0.0f;
             if (n != 0) {
                 const double cnt = (double)n;
@@ -773,7 +778,10 @@ __global__ __launch_bounds__(SUM_THREADS) void opp_sums(Layout L) {
         accum = 0.0;
         second = 0.0;
         sum_pass<1>(L, g, begin, end, means, tols, s_stage, kept, accum, second);
+        if (wave == 1 && lane < SLOTS) s_second[lane] = second;
+        __syncthreads();
         if (wave == 0 && lane < SLOTS) {
+            second = s_second[lane];
             // Slot::value: fewer than the minimum samples or nothing kept
             // leaves the slot empty.
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
