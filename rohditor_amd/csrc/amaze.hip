@@ -7,13 +7,16 @@
 // cores, 16-pixel mirrored halo, 256-pixel tile planes) is kept because the
 // Nyquist bounding box and the three in-place chains are per tile.
 //
-// Layout: every tile owns one scratch block in HBM (11 full planes of
-// 256x256 f32, 14 half planes of 256x128 f32 indexed by pixel pair, 2 byte
-// planes). Green is stored only at red/blue sites (the GRB half plane); at
-// green sites it equals the CFA sample. Stages run as grid kernels over
-// (tile, row, pixel or pair); the three sequential chains run as
+// Layout: every tile owns one scratch block in HBM (7 full planes of 256x256
+// f32, 10 half planes of 256x128 f32 indexed by pixel pair, 2 byte planes).
+// Green is stored only at red/blue sites (the GRB half plane); at green
+// sites it equals the CFA sample. Stencil quantities that are cheap to form
+// from the CFA samples (the directional and diagonal gradients, delhvsqsum,
+// cddiffsq) are never stored: the kernels that read them form them in LDS
+// or registers. Stages run as grid kernels over (tile, block of rows and
+// columns or pairs); the three sequential chains run as
 //   - bound_v: one thread per tile column walking down the rows (vcd),
-//   - bound_h: one thread per tile row walking right over LDS-staged chunks (hcd),
+//   - bound_h: one thread per tile row walking right in register chunks (hcd),
 //   - chains:  one workgroup per tile, one thread per pair, a barrier per row
 //              (hvwt in populate_green and pmwt in combine_diagonals).
 // The finish kernel writes planar RGB for the tile core, with the measured
@@ -28,9 +31,8 @@ namespace {
 constexpr int TS = 256, TSH = 128, BORDER = 16, VALID = TS - 2 * BORDER;
 constexpr int FULL = TS * TS, HALF = TS * TSH;
 
-enum FullPlane { CFA, DW0, DW1, DHV, VCD, HCD, VALT, HALT, DGV, DGH, CDS, NUM_FULL };
-enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, DELP, DELM, SQ1M, SQ1P, PMWT, RBM, RBP, RBINT,
-                 NUM_HALF };
+enum FullPlane { CFA, VCD, HCD, VALT, HALT, DGV, DGH, NUM_FULL };
+enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, PMWT, RBM, RBP, RBINT, NUM_HALF };
 // Per-row partial Nyquist boxes: one {min row, max row, min col, max col}
 // slot per (row, half-row) of the tile.
 constexpr int BOX_SLOTS = TS * 2;
@@ -76,146 +78,261 @@ __device__ __forceinline__ float intp(float a, float b, float c) { return a * (b
 __device__ __forceinline__ int fc(int pattern, int row, int col) { return color_at(pattern, col, row); }
 __device__ __forceinline__ int gae(int pattern, int row) { return fc(pattern, row, 2) & 1; }
 
-// ---------------------------------------------------------------- load
-// mod.rs:171-194 (populate) plus the zeroed state of scratch.rs:113-145.
-__global__ __launch_bounds__(256) void amz_load(AmazeArgs a) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
-    const int cc = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
-    const int sx = mirror_whole_near(tx * VALID - BORDER + cc, a.width);
-    const int sy = mirror_whole_near(ty * VALID - BORDER + rr, a.height);
-    const int c = fc(a.pattern, rr, cc);
-    const float v = a.mosaic[(size_t)sy * a.width + sx] * a.forward[c];
-    const int i = rr * TS + cc;
-    t.full(CFA)[i] = v;
-    if ((c & 1) == 0) {
-        const int p = i >> 1;
-        t.half(GRB)[p] = v;
-        t.half(D2H)[p] = 0.0f;
-        t.half(D2V)[p] = 0.0f;
-        t.nyq()[p] = 0;
-        t.nyq2()[p] = 0;
-    }
-    // vcd/hcd and their alternatives are read up to two rows and columns
-    // outside the region the stages write; those entries are zero.
-    if (rr < 4 || rr >= TS - 4 || cc < 4 || cc >= TS - 4) {
-        t.full(VCD)[i] = 0.0f;
-        t.full(HCD)[i] = 0.0f;
-        t.full(VALT)[i] = 0.0f;
-        t.full(HALT)[i] = 0.0f;
-    }
+// diagonal.rs:98-144 (one diagonal estimate) and diagonal.rs:79-94 and
+// :184-199 (bound_diagonal, bound_green).
+__device__ __forceinline__ float diagonal_estimate(float centre, float near_, float far_) {
+    float ratio = 2.0f * near_ / (EPS + centre + far_);
+    return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : near_ + 0.5f * (centre - far_);
 }
-
-// ---------------------------------------------------------------- gradients
-// green.rs:13-31 and diagonal.rs:15-42.
-__global__ __launch_bounds__(256) void amz_gradients(AmazeArgs a) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
-    const int cc = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const float* cfa = t.full(CFA);
-    if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
-        const int i = rr * TS + cc;
-        float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
-        float delv = fabsf(cfa[i + TS] - cfa[i - TS]);
-        t.full(DW0)[i] = EPS + fabsf(cfa[i + 2 * TS] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2 * TS]) + delv;
-        t.full(DW1)[i] = EPS + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
-        t.full(DHV)[i] = sqr(delh) + sqr(delv);
-    }
-    if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6 && (cc & 1) == 0) {
-        const float* up = cfa + (rr - 1) * TS;
-        const float* mid = cfa + rr * TS;
-        const float* down = cfa + (rr + 1) * TS;
-        const int p = (rr * TS + cc) >> 1;
-        float delp, delm, sqp, sqm;
-        if (gae(a.pattern, rr) == 0) {
-            delp = fabsf(up[cc + 1] - down[cc - 1]);
-            delm = fabsf(down[cc + 1] - up[cc - 1]);
-            float g = mid[cc + 1];
-            sqp = sqr(g - down[cc]) + sqr(g - up[cc + 2]);
-            sqm = sqr(g - up[cc]) + sqr(g - down[cc + 2]);
+__device__ __forceinline__ float bound_towards(float estimate, float centre, float a, float b,
+                                               float clip) {
+    float value = estimate;
+    if (value < centre) {
+        if (2.0f * value < centre) {
+            value = median3(value, a, b);
         } else {
-            float g = mid[cc];
-            sqp = sqr(g - down[cc - 1]) + sqr(g - up[cc + 1]);
-            sqm = sqr(g - up[cc - 1]) + sqr(g - down[cc + 1]);
-            delp = fabsf(up[cc + 2] - down[cc]);
-            delm = fabsf(down[cc + 2] - up[cc]);
+            float weight = 2.0f * (centre - value) / (EPS + value + centre);
+            value = weight * value + (1.0f - weight) * median3(value, a, b);
         }
-        t.half(DELP)[p] = delp;
-        t.half(DELM)[p] = delm;
-        t.half(SQ1P)[p] = sqp;
-        t.half(SQ1M)[p] = sqm;
     }
+    return value > clip ? median3(value, a, b) : value;
 }
 
-// ---------------------------------------------------------------- colour differences
+// ---------------------------------------------------------------- front
+// mod.rs:171-194 (populate), green.rs:13-31 and diagonal.rs:15-42
+// (gradients), green.rs:46-137 (colour differences) and diagonal.rs:98-144
+// (diagonal estimates) for one FW x FH block of a tile. The block's CFA
+// samples, 4 around, are staged in LDS from the mosaic; the directional
+// gradients (2 around) and the diagonal gradients and squares (2 rows and
+// one pair around) are formed in LDS from them. Only what later stages read
+// leaves for HBM: the CFA, the four colour differences and their gradients,
+// the diagonal estimates and weights, and the zeroed state of the planes
+// that later stages write only in part (scratch.rs:113-145).
+constexpr int FW = 64, FH = 32;
+constexpr int FC_W = FW + 8, FC_H = FH + 8;      // CFA: columns c0 - 4 .., rows r0 - 4 ..
+constexpr int FG_W = FW + 4, FG_H = FH + 4;      // gradients: columns c0 - 2 .., rows r0 - 2 ..
+constexpr int FD_W = FW / 2 + 2, FD_H = FH + 4;  // diagonal planes: pairs p0 - 1 .., rows r0 - 2 ..
+static_assert(TS % FW == 0 && TS % FH == 0 && FH % 4 == 0, "whole blocks");
+
+struct alignas(16) FrontLds {
+    float cfa[FC_H * FC_W];
+    float w0[FG_H * FG_W], w1[FG_H * FG_W];
+    float dp[FD_H * FD_W], dm[FD_H * FD_W], qp[FD_H * FD_W], qm[FD_H * FD_W];
+};
+
 __device__ __forceinline__ float adaptive_or(float ratio, float centre, float fallback) {
     return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : fallback;
 }
+// green.rs:13-31 at (r, c) of the CFA accessor f: the vertical (dirwts0)
+// and horizontal (dirwts1) gradients and delhvsqsum.
+template <class F>
+__device__ __forceinline__ float vgrad(const F& f, int r, int c) {
+    const float cur = f(r, c);
+    const float delv = fabsf(f(r + 1, c) - f(r - 1, c));
+    return EPS + fabsf(f(r + 2, c) - cur) + fabsf(cur - f(r - 2, c)) + delv;
+}
+template <class F>
+__device__ __forceinline__ float hgrad(const F& f, int r, int c) {
+    const float cur = f(r, c);
+    const float delh = fabsf(f(r, c + 1) - f(r, c - 1));
+    return EPS + fabsf(f(r, c + 2) - cur) + fabsf(cur - f(r, c - 2)) + delh;
+}
+template <class F>
+__device__ __forceinline__ float delhv(const F& f, int r, int c) {
+    const float delh = fabsf(f(r, c + 1) - f(r, c - 1));
+    const float delv = fabsf(f(r + 1, c) - f(r - 1, c));
+    return sqr(delh) + sqr(delv);
+}
 
-// green.rs:46-137
-__global__ __launch_bounds__(256) void amz_colour_differences(AmazeArgs a, float clip_pt8) {
+__global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, float clip_pt) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
-    const int cc = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    if (rr < 4 || rr >= TS - 4 || cc < 4 || cc >= TS - 4) return;
-    const float* cfa = t.full(CFA);
-    const float* w0 = t.full(DW0);
-    const float* w1 = t.full(DW1);
-    const int i = rr * TS + cc;
-    const bool fcswitch = (fc(a.pattern, rr, cc) & 1) == 1;  // green site
-    float cur = cfa[i];
-    float up = cfa[i - TS], up2 = cfa[i - 2 * TS], down = cfa[i + TS], down2 = cfa[i + 2 * TS];
-    float left = cfa[i - 1], left2 = cfa[i - 2], right = cfa[i + 1], right2 = cfa[i + 2];
-    float w0u2 = w0[i - 2 * TS], w0u1 = w0[i - TS], w0c = w0[i], w0d1 = w0[i + TS],
-          w0d2 = w0[i + 2 * TS];
-    float w1l2 = w1[i - 2], w1l1 = w1[i - 1], w1c = w1[i], w1r1 = w1[i + 1], w1r2 = w1[i + 2];
-    float cru = up * (w0u2 + w0c) / (w0u2 * (EPS + cur) + w0c * (EPS + up2));
-    float crd = down * (w0d2 + w0c) / (w0d2 * (EPS + cur) + w0c * (EPS + down2));
-    float crl = left * (w1l2 + w1c) / (w1l2 * (EPS + cur) + w1c * (EPS + left2));
-    float crr = right * (w1r2 + w1c) / (w1r2 * (EPS + cur) + w1c * (EPS + right2));
-    float guha = up + 0.5f * (cur - up2);
-    float gdha = down + 0.5f * (cur - down2);
-    float glha = left + 0.5f * (cur - left2);
-    float grha = right + 0.5f * (cur - right2);
-    float guar = adaptive_or(cru, cur, guha);
-    float gdar = adaptive_or(crd, cur, gdha);
-    float glar = adaptive_or(crl, cur, glha);
-    float grar = adaptive_or(crr, cur, grha);
-    float hwt = w1l1 / (w1l1 + w1r1);
-    float vwt = w0u1 / (w0d1 + w0u1);
-    float gintvha = vwt * gdha + (1.0f - vwt) * guha;
-    float ginthha = hwt * grha + (1.0f - hwt) * glha;
-    float vcd, hcd, vcdalt, hcdalt;
-    if (fcswitch) {
-        vcd = cur - (vwt * gdar + (1.0f - vwt) * guar);
-        hcd = cur - (hwt * grar + (1.0f - hwt) * glar);
-        vcdalt = cur - gintvha;
-        hcdalt = cur - ginthha;
-    } else {
-        vcd = (vwt * gdar + (1.0f - vwt) * guar) - cur;
-        hcd = (hwt * grar + (1.0f - hwt) * glar) - cur;
-        vcdalt = gintvha - cur;
-        hcdalt = ginthha - cur;
+    const int c0 = blk.x * FW, r0 = blk.y * FH, p0 = c0 / 2;
+    __shared__ FrontLds s;
+    auto C = [&](int rr, int cc) { return s.cfa[(rr - r0 + 4) * FC_W + (cc - c0 + 4)]; };
+    auto W0 = [&](int rr, int cc) { return s.w0[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
+    auto W1 = [&](int rr, int cc) { return s.w1[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
+    auto D = [&](const float* pl, int rr, int p) { return pl[(rr - r0 + 2) * FD_W + (p - p0 + 1)]; };
+
+    // CFA samples of the block and its halo (zero outside the tile, where
+    // no stencil of a written site reaches).
+    {
+        const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
+        const int ox = tx * VALID - BORDER, oy = ty * VALID - BORDER;
+        for (int k = threadIdx.x; k < FC_H * FC_W; k += 256) {
+            const int ly = k / FC_W, lx = k - ly * FC_W;
+            const int rr = r0 - 4 + ly, cc = c0 - 4 + lx;
+            float v = 0.0f;
+            if (rr >= 0 && rr < TS && cc >= 0 && cc < TS) {
+                const int sx = mirror_whole_near(ox + cc, a.width);
+                const int sy = mirror_whole_near(oy + rr, a.height);
+                const int ch = fc(a.pattern, rr, cc);
+                const float gain = ch == 0 ? a.forward[0] : (ch == 1 ? a.forward[1] : a.forward[2]);
+                v = a.mosaic[(size_t)sy * a.width + sx] * gain;
+            }
+            s.cfa[k] = v;
+        }
     }
-    bool clipped = (cur > clip_pt8) | (gintvha > clip_pt8) | (ginthha > clip_pt8);
-    if (clipped) {
-        guar = guha;
-        gdar = gdha;
-        glar = glha;
-        grar = grha;
-        vcd = vcdalt;
-        hcd = hcdalt;
+    __syncthreads();
+    // Gradients over [2, TS - 2) and the diagonal planes over [6, TS - 6)
+    // (even columns, one entry per pair); zero elsewhere, unread.
+    for (int k = threadIdx.x; k < FG_H * FG_W; k += 256) {
+        const int ly = k / FG_W, lx = k - ly * FG_W;
+        const int rr = r0 - 2 + ly, cc = c0 - 2 + lx;
+        float w0 = 0.0f, w1 = 0.0f;
+        if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
+            w0 = vgrad(C, rr, cc);
+            w1 = hgrad(C, rr, cc);
+        }
+        s.w0[k] = w0;
+        s.w1[k] = w1;
     }
-    t.full(DGV)[i] = sel_min(sqr(guha - gdha), sqr(guar - gdar));
-    t.full(DGH)[i] = sel_min(sqr(glha - grha), sqr(glar - grar));
-    t.full(VCD)[i] = vcd;
-    t.full(HCD)[i] = hcd;
-    t.full(VALT)[i] = vcdalt;
-    t.full(HALT)[i] = hcdalt;
+    for (int k = threadIdx.x; k < FD_H * FD_W; k += 256) {
+        const int ly = k / FD_W, lx = k - ly * FD_W;
+        const int rr = r0 - 2 + ly, cc = c0 - 2 + 2 * lx;
+        float delp = 0.0f, delm = 0.0f, sqp = 0.0f, sqm = 0.0f;
+        if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
+            if (gae(a.pattern, rr) == 0) {
+                delp = fabsf(C(rr - 1, cc + 1) - C(rr + 1, cc - 1));
+                delm = fabsf(C(rr + 1, cc + 1) - C(rr - 1, cc - 1));
+                const float g = C(rr, cc + 1);
+                sqp = sqr(g - C(rr + 1, cc)) + sqr(g - C(rr - 1, cc + 2));
+                sqm = sqr(g - C(rr - 1, cc)) + sqr(g - C(rr + 1, cc + 2));
+            } else {
+                const float g = C(rr, cc);
+                sqp = sqr(g - C(rr + 1, cc - 1)) + sqr(g - C(rr - 1, cc + 1));
+                sqm = sqr(g - C(rr - 1, cc - 1)) + sqr(g - C(rr + 1, cc + 1));
+                delp = fabsf(C(rr - 1, cc + 2) - C(rr + 1, cc));
+                delm = fabsf(C(rr + 1, cc + 2) - C(rr - 1, cc));
+            }
+        }
+        s.dp[k] = delp;
+        s.dm[k] = delm;
+        s.qp[k] = sqp;
+        s.qm[k] = sqm;
+    }
+    __syncthreads();
+
+    // Per site: the CFA plane and the colour differences (green.rs:46-137);
+    // vcd, hcd and their alternatives are zero in the 4-pixel band, where
+    // later stages read them but nothing writes them.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cc = c0 + lane;
+    for (int j = 0; j < FH / 4; ++j) {
+        const int rr = r0 + wave * (FH / 4) + j;
+        const int i = rr * TS + cc;
+        const float cur = C(rr, cc);
+        t.full(CFA)[i] = cur;
+        if (!(rr >= 4 && rr < TS - 4 && cc >= 4 && cc < TS - 4)) {
+            t.full(VCD)[i] = 0.0f;
+            t.full(HCD)[i] = 0.0f;
+            t.full(VALT)[i] = 0.0f;
+            t.full(HALT)[i] = 0.0f;
+            continue;
+        }
+        const bool fcswitch = (fc(a.pattern, rr, cc) & 1) == 1;  // green site
+        const float up = C(rr - 1, cc), up2 = C(rr - 2, cc), down = C(rr + 1, cc),
+                    down2 = C(rr + 2, cc);
+        const float left = C(rr, cc - 1), left2 = C(rr, cc - 2), right = C(rr, cc + 1),
+                    right2 = C(rr, cc + 2);
+        const float w0u2 = W0(rr - 2, cc), w0u1 = W0(rr - 1, cc), w0c = W0(rr, cc),
+                    w0d1 = W0(rr + 1, cc), w0d2 = W0(rr + 2, cc);
+        const float w1l2 = W1(rr, cc - 2), w1l1 = W1(rr, cc - 1), w1c = W1(rr, cc),
+                    w1r1 = W1(rr, cc + 1), w1r2 = W1(rr, cc + 2);
+        float cru = up * (w0u2 + w0c) / (w0u2 * (EPS + cur) + w0c * (EPS + up2));
+        float crd = down * (w0d2 + w0c) / (w0d2 * (EPS + cur) + w0c * (EPS + down2));
+        float crl = left * (w1l2 + w1c) / (w1l2 * (EPS + cur) + w1c * (EPS + left2));
+        float crr = right * (w1r2 + w1c) / (w1r2 * (EPS + cur) + w1c * (EPS + right2));
+        float guha = up + 0.5f * (cur - up2);
+        float gdha = down + 0.5f * (cur - down2);
+        float glha = left + 0.5f * (cur - left2);
+        float grha = right + 0.5f * (cur - right2);
+        float guar = adaptive_or(cru, cur, guha);
+        float gdar = adaptive_or(crd, cur, gdha);
+        float glar = adaptive_or(crl, cur, glha);
+        float grar = adaptive_or(crr, cur, grha);
+        float hwt = w1l1 / (w1l1 + w1r1);
+        float vwt = w0u1 / (w0d1 + w0u1);
+        float gintvha = vwt * gdha + (1.0f - vwt) * guha;
+        float ginthha = hwt * grha + (1.0f - hwt) * glha;
+        float vcd, hcd, vcdalt, hcdalt;
+        if (fcswitch) {
+            vcd = cur - (vwt * gdar + (1.0f - vwt) * guar);
+            hcd = cur - (hwt * grar + (1.0f - hwt) * glar);
+            vcdalt = cur - gintvha;
+            hcdalt = cur - ginthha;
+        } else {
+            vcd = (vwt * gdar + (1.0f - vwt) * guar) - cur;
+            hcd = (hwt * grar + (1.0f - hwt) * glar) - cur;
+            vcdalt = gintvha - cur;
+            hcdalt = ginthha - cur;
+        }
+        bool clipped = (cur > clip_pt8) | (gintvha > clip_pt8) | (ginthha > clip_pt8);
+        if (clipped) {
+            guar = guha;
+            gdar = gdha;
+            glar = glha;
+            grar = grha;
+            vcd = vcdalt;
+            hcd = hcdalt;
+        }
+        t.full(DGV)[i] = sel_min(sqr(guha - gdha), sqr(guar - gdar));
+        t.full(DGH)[i] = sel_min(sqr(glha - grha), sqr(glar - grar));
+        t.full(VCD)[i] = vcd;
+        t.full(HCD)[i] = hcd;
+        t.full(VALT)[i] = vcdalt;
+        t.full(HALT)[i] = hcdalt;
+    }
+
+    // Per red/blue site (one per pair): the zeroed Nyquist flags, d2h / d2v
+    // zeroed outside [8, TS - 8) where the row chain does not write them,
+    // and the diagonal estimates over [8, TS - 8) (diagonal.rs:98-144).
+    for (int k = threadIdx.x; k < FH * (FW / 2); k += 256) {
+        const int rr = r0 + k / (FW / 2), p = p0 + k % (FW / 2);
+        const int par = gae(a.pattern, rr);
+        const int cc = 2 * p + par, hp = rr * TSH + p;
+        t.nyq()[hp] = 0;
+        t.nyq2()[hp] = 0;
+        if (rr < 8 || rr >= TS - 8 || cc < 8 || cc >= TS - 8) {
+            t.half(D2H)[hp] = 0.0f;
+            t.half(D2V)[hp] = 0.0f;
+            continue;
+        }
+        const float centre = C(rr, cc);
+        const float se1 = C(rr + 1, cc + 1), se2 = C(rr + 2, cc + 2);
+        const float nw1 = C(rr - 1, cc - 1), nw2 = C(rr - 2, cc - 2);
+        const float ne1 = C(rr - 1, cc + 1), ne2 = C(rr - 2, cc + 2);
+        const float sw1 = C(rr + 1, cc - 1), sw2 = C(rr + 2, cc - 2);
+        const float dm_c = D(s.dm, rr, p), dp_c = D(s.dp, rr, p);
+        const float dm_se1 = D(s.dm, rr + 1, p + par), dm_se2 = D(s.dm, rr + 2, p + 1);
+        const float dm_nw1 = D(s.dm, rr - 1, p + par - 1), dm_nw2 = D(s.dm, rr - 2, p - 1);
+        const float dp_ne1 = D(s.dp, rr - 1, p + par), dp_ne2 = D(s.dp, rr - 2, p + 1);
+        const float dp_sw1 = D(s.dp, rr + 1, p + par - 1), dp_sw2 = D(s.dp, rr + 2, p - 1);
+        // even_kernel (diagonal.rs:46-61) over rows rr - 2 .. rr + 2.
+        auto even = [&](const float* q) {
+            return GAUSSEVEN[0] * (D(q, rr - 1, p) + D(q, rr, p + par - 1) + D(q, rr, p + par) +
+                                   D(q, rr + 1, p)) +
+                   GAUSSEVEN[1] * (D(q, rr - 2, p + par - 1) + D(q, rr - 2, p + par) +
+                                   D(q, rr - 1, p - 1) + D(q, rr - 1, p + 1) +
+                                   D(q, rr + 1, p - 1) + D(q, rr + 1, p + 1) +
+                                   D(q, rr + 2, p + par - 1) + D(q, rr + 2, p + par));
+        };
+        const float rbvarm_even = even(s.qm), rbvarp_even = even(s.qp);
+        float rbse = diagonal_estimate(centre, se1, se2);
+        float rbnw = diagonal_estimate(centre, nw1, nw2);
+        float rbne = diagonal_estimate(centre, ne1, ne2);
+        float rbsw = diagonal_estimate(centre, sw1, sw2);
+        float wtse = EPS + dm_c + dm_se1 + dm_se2;
+        float wtnw = EPS + dm_c + dm_nw1 + dm_nw2;
+        float wtne = EPS + dp_c + dp_ne1 + dp_ne2;
+        float wtsw = EPS + dp_c + dp_sw1 + dp_sw2;
+        float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
+        float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
+        float rbvarm = EPSSQ + rbvarm_even;
+        t.half(PMWT)[hp] = rbvarm / ((EPSSQ + rbvarp_even) + rbvarm);
+        t.half(RBP)[hp] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
+        t.half(RBM)[hp] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
+    }
 }
 
 // ---------------------------------------------------------------- bound differences
@@ -304,14 +421,13 @@ __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
     }
 }
 
-// green.rs:184-226, horizontal half: hcd is bounded in place along each row,
-// and cddiffsq is formed at red/blue sites from the final vcd and hcd. One
-// thread per tile row walks it in 32-column chunks: each chunk's hcd, hcdalt
-// and CFA columns (with the 4-column halo on the left, 8 on the right) and
-// its vcd arrive as aligned float4 loads straight into registers, the serial
-// recurrence runs fully unrolled over them, and hcd and cddiffsq leave as
-// float4 stores. cddiffsq is written as 0 at green sites, which nothing
-// reads (the Nyquist stencil touches red/blue sites only).
+// green.rs:184-226, horizontal half: hcd is bounded in place along each row.
+// One thread per tile row walks it in 32-column chunks: each chunk's hcd,
+// hcdalt and CFA columns (with the 4-column halo on the left, 8 on the right)
+// arrive as aligned float4 loads straight into registers, the serial
+// recurrence runs fully unrolled over them, and hcd leaves as float4 stores.
+// cddiffsq (green.rs:219-223) is formed where it is read, by the Nyquist
+// test, from the bounded vcd and hcd.
 constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_IN = BH_CHUNK + 8;
 
 __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
@@ -322,14 +438,12 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     float* __restrict__ hcd_row = t.full(HCD) + rr * TS;
     const float* __restrict__ alt_row = t.full(HALT) + rr * TS;
     const float* __restrict__ cfa_row = t.full(CFA) + rr * TS;
-    const float* __restrict__ vcd_row = t.full(VCD) + rr * TS;
-    float* __restrict__ cds_row = t.full(CDS) + rr * TS;
     const bool green4 = (fc(a.pattern, rr, 4) & 1) == 1;  // colour at column 4 (and c0)
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded hcd two and one columns left (cols 2, 3 are zero)
     for (int c0 = 4; c0 < TS - 4; c0 += BH_CHUNK) {
         // Register k holds column c0 - 4 + k. hcd and hcdalt read as zero
         // outside [4, TS - 4), the CFA as zero beyond the tile.
-        float h[BH_IN], al[BH_IN], cf[BH_IN], vc[BH_CHUNK];
+        float h[BH_IN], al[BH_IN], cf[BH_IN];
 #pragma unroll
         for (int q = 0; q < BH_IN / 4; ++q) {
             const int col = c0 - 4 + 4 * q;
@@ -349,16 +463,7 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
                 cf[4 * q + e] = col + e < TS ? cs[e] : 0.0f;
             }
         }
-#pragma unroll
-        for (int q = 0; q < BH_CHUNK / 4; ++q) {
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (c0 + 4 * q < TS - 4) v = *reinterpret_cast<const float4*>(vcd_row + c0 + 4 * q);
-            vc[4 * q] = v.x;
-            vc[4 * q + 1] = v.y;
-            vc[4 * q + 2] = v.z;
-            vc[4 * q + 3] = v.w;
-        }
-        float out[BH_CHUNK], cd[BH_CHUNK];
+        float out[BH_CHUNK];
 #pragma unroll
         for (int k = 0; k < BH_CHUNK; ++k) {
             const bool green = green4 != ((k & 1) != 0);
@@ -368,7 +473,6 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
             const float bounded = green ? bound_at_green(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt)
                                         : bound_at_red_blue(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt);
             out[k] = bounded;
-            cd[k] = green ? 0.0f : sqr(vc[k] - bounded);
             b_m2 = b_m1;
             b_m1 = bounded;
         }
@@ -378,81 +482,9 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
             if (c0 + 4 * q < TS - 4) {
                 *reinterpret_cast<float4*>(hcd_row + c0 + 4 * q) =
                     make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
-                *reinterpret_cast<float4*>(cds_row + c0 + 4 * q) =
-                    make_float4(cd[4 * q], cd[4 * q + 1], cd[4 * q + 2], cd[4 * q + 3]);
             }
         }
     }
-}
-
-// ---------------------------------------------------------------- weights, Nyquist test, diagonals
-// green.rs:229-292 (direction_weights), nyquist.rs:25-89 (test and bounding
-// box), diagonal.rs:98-144 (diagonal_estimates).
-__device__ __forceinline__ float diagonal_estimate(float centre, float near_, float far_) {
-    float ratio = 2.0f * near_ / (EPS + centre + far_);
-    return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : near_ + 0.5f * (centre - far_);
-}
-// diagonal.rs:79-94 and :184-199 (bound_diagonal, bound_green).
-__device__ __forceinline__ float bound_towards(float estimate, float centre, float a, float b,
-                                               float clip) {
-    float value = estimate;
-    if (value < centre) {
-        if (2.0f * value < centre) {
-            value = median3(value, a, b);
-        } else {
-            float weight = 2.0f * (centre - value) / (EPS + value + centre);
-            value = weight * value + (1.0f - weight) * median3(value, a, b);
-        }
-    }
-    return value > clip ? median3(value, a, b) : value;
-}
-
-// One thread per red/blue site: cc = 2 * pair + green_at_even(row). A
-// wavefront holds 64 consecutive pairs of one row, so the horizontal taps of
-// a plane row come from one float2 load per lane (columns 2p, 2p + 1) and
-// the neighbouring lanes; only lanes at the wavefront's ends load theirs.
-// The arithmetic of each site is the reference's, term for term.
-struct Lanes {
-    int lane, base, par;  // base = 2 * pair
-    // Column cc + dx of a plane row of which this lane holds columns
-    // (base, base + 1) in v; `fetch` says whether this lane may read the row
-    // itself when the neighbour lies outside the wavefront.
-    __device__ __forceinline__ float tap(float2 v, const float* row, int dx, bool fetch) const {
-        const int o = par + dx;
-        const int po = o >> 1;  // pair offset (floor)
-        const float mine = (o & 1) ? v.y : v.x;
-        if (po == 0) return mine;
-        const int src = lane + po;
-        float r = __shfl(mine, src, 64);
-        if ((src < 0 || src > 63) && fetch) r = row[base + o];
-        return r;
-    }
-    // Half column pair + s of a half-plane row of which this lane holds
-    // half column `pair` in h.
-    __device__ __forceinline__ float half_tap(float h, const float* row, int s, bool fetch) const {
-        if (s == 0) return h;
-        const int src = lane + s;
-        float r = __shfl(h, src, 64);
-        if ((src < 0 || src > 63) && fetch) r = row[(base >> 1) + s];
-        return r;
-    }
-};
-
-__device__ __forceinline__ float2 pair_load(const float* plane, int row, int base) {
-    return *reinterpret_cast<const float2*>(plane + row * TS + base);
-}
-
-// even_kernel over half-plane rows rr - 2 .. rr + 2 held one half column per
-// lane (e[0] .. e[4]); (cc >> 1) is the lane's own half column, ((cc + 1) >> 1)
-// is + par, ((cc - 1) >> 1) is + par - 1, ((cc + 2) >> 1) is + 1 and
-// ((cc - 2) >> 1) is - 1.
-__device__ __forceinline__ float even_lanes(const Lanes& L, const float* plane, int rr,
-                                            const float* e, bool fetch) {
-    auto at = [&](int m, int s) { return L.half_tap(e[m], plane + (rr + m - 2) * TSH, s, fetch); };
-    const int par = L.par;
-    return GAUSSEVEN[0] * (e[1] + at(2, par - 1) + at(2, par) + e[3]) +
-           GAUSSEVEN[1] * (at(0, par - 1) + at(0, par) + at(1, -1) + at(1, 1) + at(3, -1) +
-                           at(3, 1) + at(4, par - 1) + at(4, par));
 }
 
 // Reduces a tile's slots into its Nyquist box (min/max over flagged sites).
@@ -487,87 +519,118 @@ __global__ __launch_bounds__(256) void amz_box(AmazeArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
+// ---------------------------------------------------------------- weights, Nyquist test
+// green.rs:229-292 (direction_weights) and nyquist.rs:25-89 (test and
+// bounding box) for a WH x WW block of a tile, one thread per red/blue site
+// (cc = 2 * pair + green_at_even(row); a wavefront holds 64 consecutive pairs
+// of a row). The CFA, the bounded colour differences and their gradients
+// are staged in LDS with the halo the stencils need (float4 loads of whole
+// plane rows); the directional gradients and delhvsqsum (dhv) are formed
+// from the CFA samples there, cddiffsq from the bounded colour differences.
+// The arithmetic of each site is the reference's, term for term.
+constexpr int WW = 128, WH = 16;
+constexpr int WS_W = WW + 8;                 // staged columns c0 - 4 .. c0 + WW + 4
+constexpr int WC_H = WH + 6;                 // CFA and vcd rows r0 - 3 .. r0 + WH + 3
+constexpr int WK_H = WH + 4;                 // hcd and dgv rows r0 - 2 .. r0 + WH + 2
+constexpr int WD_W = WW + 4, WD_H = WH + 4;  // dhv: columns c0 - 2 .., rows r0 - 2 ..
+static_assert(TS % WW == 0 && TS % WH == 0 && WH % 4 == 0 && WW == 128, "whole blocks");
+
+struct alignas(16) WeightsLds {
+    float cfa[WC_H * WS_W];
+    float vcd[WC_H * WS_W];
+    float hcd[WK_H * WS_W];
+    float dgv[WK_H * WW];
+    float dgh[WH * WS_W];
+    float dhv[WD_H * WD_W];
+};
+
+// Rows [row0, row0 + rows) of a tile plane, columns [col0, col0 + 4 * quads),
+// into LDS rows of `stride` floats; zero outside the tile.
+__device__ __forceinline__ void stage_rows(float* dst, int stride, const float* plane, int row0,
+                                           int rows, int col0, int quads) {
+    for (int k = threadIdx.x; k < rows * quads; k += 256) {
+        const int ly = k / quads, q = k - ly * quads;
+        const int rr = row0 + ly, cc = col0 + 4 * q;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (rr >= 0 && rr < TS && cc >= 0 && cc < TS)
+            v = *reinterpret_cast<const float4*>(plane + rr * TS + cc);
+        *reinterpret_cast<float4*>(dst + ly * stride + 4 * q) = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
-    const int lane = threadIdx.x & 63;
+    const int c0 = blk.x * WW, r0 = blk.y * WH;
+    __shared__ WeightsLds s;
+    stage_rows(s.cfa, WS_W, t.full(CFA), r0 - 3, WC_H, c0 - 4, WS_W / 4);
+    stage_rows(s.vcd, WS_W, t.full(VCD), r0 - 3, WC_H, c0 - 4, WS_W / 4);
+    stage_rows(s.hcd, WS_W, t.full(HCD), r0 - 2, WK_H, c0 - 4, WS_W / 4);
+    stage_rows(s.dgv, WW, t.full(DGV), r0 - 2, WK_H, c0, WW / 4);
+    stage_rows(s.dgh, WS_W, t.full(DGH), r0, WH, c0 - 4, WS_W / 4);
+    __syncthreads();
+    auto C = [&](int rr, int cc) { return s.cfa[(rr - r0 + 3) * WS_W + (cc - c0 + 4)]; };
+    auto V = [&](int rr, int cc) { return s.vcd[(rr - r0 + 3) * WS_W + (cc - c0 + 4)]; };
+    auto H = [&](int rr, int cc) { return s.hcd[(rr - r0 + 2) * WS_W + (cc - c0 + 4)]; };
+    for (int k = threadIdx.x; k < WD_H * WD_W; k += 256) {
+        const int ly = k / WD_W, lx = k - ly * WD_W;
+        const int rr = r0 - 2 + ly, cc = c0 - 2 + lx;
+        s.dhv[k] = rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2 ? delhv(C, rr, cc) : 0.0f;
+    }
+    __syncthreads();
+    auto DHV = [&](int rr, int cc) { return s.dhv[(rr - r0 + 2) * WD_W + (cc - c0 + 2)]; };
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blk.x * 64 + lane;
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int par = gae(a.pattern, rr);
-    const int cc = 2 * p + par, i = rr * TS + cc;
-    const Lanes L{lane, 2 * p, par};
-    bool flagged = false;
-    // Whole wavefronts skip the rows outside [6, TS - 6); inside them every
-    // lane takes part in the exchanges and only valid sites fetch or store.
-    if (rr >= 6 && rr < TS - 6) {
-        const bool valid = cc >= 6 && cc < TS - 6;
-        // HV weights (green.rs:229-292).
-        {
-            const float* vcd = t.full(VCD);
-            const float* dgv = t.full(DGV);
-            const float* w0 = t.full(DW0);
-            const float* hcd_row = t.full(HCD) + rr * TS;
-            const float* w1_row = t.full(DW1) + rr * TS;
-            const float* dgh_row = t.full(DGH) + rr * TS;
-            const float2 HC = pair_load(t.full(HCD), rr, L.base);
-            const float2 W1 = pair_load(t.full(DW1), rr, L.base);
-            const float2 DH = pair_load(t.full(DGH), rr, L.base);
-            float v[7], dv[5];
+    for (int j = 0; j < WH / 4; ++j) {
+        const int rr = r0 + wave * (WH / 4) + j;
+        const int cc = 2 * p + gae(a.pattern, rr);
+        bool flagged = false;
+        if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
+            // HV weights (green.rs:229-292).
+            float v[7], dv[5], hcd[7], dgh[5];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) v[k] = vcd[i + (k - 3) * TS];
+            for (int k = 0; k < 7; ++k) v[k] = V(rr + k - 3, cc);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) dv[k] = dgv[i + (k - 2) * TS];
-            const float w0u = w0[i - TS], w0d = w0[i + TS];
-            float hcd[7], dgh[5];
+            for (int k = 0; k < 5; ++k) dv[k] = s.dgv[(rr + k - 2 - r0 + 2) * WW + (cc - c0)];
 #pragma unroll
-            for (int d = -3; d <= 3; ++d) hcd[d + 3] = L.tap(HC, hcd_row, d, valid);
+            for (int d = -3; d <= 3; ++d) hcd[d + 3] = H(rr, cc + d);
 #pragma unroll
-            for (int d = -2; d <= 2; ++d) dgh[d + 2] = L.tap(DH, dgh_row, d, valid);
-            const float w1l = L.tap(W1, w1_row, -1, valid), w1r = L.tap(W1, w1_row, 1, valid);
-            if (valid) {
-                float uave = v[3] + v[2] + v[1] + v[0];
-                float dave = v[3] + v[4] + v[5] + v[6];
-                float lave = hcd[3] + hcd[2] + hcd[1] + hcd[0];
-                float rave = hcd[3] + hcd[4] + hcd[5] + hcd[6];
-                float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
-                float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
-                float varl = sqr(hcd[3] - lave) + sqr(hcd[2] - lave) + sqr(hcd[1] - lave) +
-                             sqr(hcd[0] - lave);
-                float varr = sqr(hcd[3] - rave) + sqr(hcd[4] - rave) + sqr(hcd[5] - rave) +
-                             sqr(hcd[6] - rave);
-                float hwt = w1l / (w1l + w1r);
-                float vwt = w0u / (w0d + w0u);
-                float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
-                float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
-                float fluct_u = dv[2] + dv[1] + dv[0];
-                float fluct_d = dv[2] + dv[3] + dv[4];
-                float fluct_l = dgh[2] + dgh[1] + dgh[0];
-                float fluct_r = dgh[2] + dgh[3] + dgh[4];
-                float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
-                float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
-                float varwt = hcdvar / (vcdvar + hcdvar);
-                float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
-                t.half(HVWT)[i >> 1] =
-                    ((0.5f - varwt) * (0.5f - diffwt) > 0.0f &&
-                     fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
-                        ? varwt
-                        : diffwt;
-            }
-        }
-        // Nyquist test statistic (nyquist.rs:28-57).
-        {
-            float2 C[5], D[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                C[m] = pair_load(t.full(CDS), rr + m - 2, L.base);
-                D[m] = pair_load(t.full(DHV), rr + m - 2, L.base);
-            }
-            auto c = [&](int dy, int dx) {
-                return L.tap(C[dy + 2], t.full(CDS) + (rr + dy) * TS, dx, valid);
-            };
-            auto d = [&](int dy, int dx) {
-                return L.tap(D[dy + 2], t.full(DHV) + (rr + dy) * TS, dx, valid);
-            };
+            for (int d = -2; d <= 2; ++d) dgh[d + 2] = s.dgh[(rr - r0) * WS_W + (cc + d - c0 + 4)];
+            const float w0u = vgrad(C, rr - 1, cc), w0d = vgrad(C, rr + 1, cc);
+            const float w1l = hgrad(C, rr, cc - 1), w1r = hgrad(C, rr, cc + 1);
+            float uave = v[3] + v[2] + v[1] + v[0];
+            float dave = v[3] + v[4] + v[5] + v[6];
+            float lave = hcd[3] + hcd[2] + hcd[1] + hcd[0];
+            float rave = hcd[3] + hcd[4] + hcd[5] + hcd[6];
+            float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
+            float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
+            float varl = sqr(hcd[3] - lave) + sqr(hcd[2] - lave) + sqr(hcd[1] - lave) +
+                         sqr(hcd[0] - lave);
+            float varr = sqr(hcd[3] - rave) + sqr(hcd[4] - rave) + sqr(hcd[5] - rave) +
+                         sqr(hcd[6] - rave);
+            float hwt = w1l / (w1l + w1r);
+            float vwt = w0u / (w0d + w0u);
+            float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
+            float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
+            float fluct_u = dv[2] + dv[1] + dv[0];
+            float fluct_d = dv[2] + dv[3] + dv[4];
+            float fluct_l = dgh[2] + dgh[1] + dgh[0];
+            float fluct_r = dgh[2] + dgh[3] + dgh[4];
+            float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
+            float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
+            float varwt = hcdvar / (vcdvar + hcdvar);
+            float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
+            t.half(HVWT)[rr * TSH + p] =
+                ((0.5f - varwt) * (0.5f - diffwt) > 0.0f &&
+                 fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
+                    ? varwt
+                    : diffwt;
+            // Nyquist test statistic (nyquist.rs:28-57); cddiffsq at the
+            // red/blue sites the stencil reads is (vcd - hcd)^2 of the
+            // bounded differences (green.rs:219-223).
+            auto c = [&](int dy, int dx) { return sqr(V(rr + dy, cc + dx) - H(rr + dy, cc + dx)); };
+            auto d = [&](int dy, int dx) { return DHV(rr + dy, cc + dx); };
             float test =
                 (GAUSSODD[0] * c(0, 0) +
                  GAUSSODD[1] * (c(-1, -1) + c(-1, 1) + c(1, -1) + c(1, 1)) +
@@ -579,78 +642,24 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a, float clip_pt) {
                  GAUSSGRAD[4] * (d(-2, -1) + d(-2, 1) + d(-1, -2) + d(-1, 2) + d(1, -2) +
                                  d(1, 2) + d(2, -1) + d(2, 1)) +
                  GAUSSGRAD[5] * (d(-2, -2) + d(-2, 2) + d(2, -2) + d(2, 2)));
-            if (valid && test > 0.0f) {
-                t.nyq()[i >> 1] = 1;
+            if (test > 0.0f) {
+                t.nyq()[rr * TSH + p] = 1;
                 flagged = true;
             }
         }
-        // Diagonal estimates (diagonal.rs:98-144).
-        if (rr >= 8 && rr < TS - 8) {
-            const bool valid8 = cc >= 8 && cc < TS - 8;
-            float2 F[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) F[m] = pair_load(t.full(CFA), rr + m - 2, L.base);
-            auto f = [&](int dy, int dx) {
-                return L.tap(F[dy + 2], t.full(CFA) + (rr + dy) * TS, dx, valid8);
-            };
-            const float centre = f(0, 0);
-            const float se1 = f(1, 1), se2 = f(2, 2);
-            const float nw1 = f(-1, -1), nw2 = f(-2, -2);
-            const float ne1 = f(-1, 1), ne2 = f(-2, 2);
-            const float sw1 = f(1, -1), sw2 = f(2, -2);
-            const int hp = rr * TSH + p;
-            float dm[5], dp[5], sm[5], sp[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                dm[m] = t.half(DELM)[hp + (m - 2) * TSH];
-                dp[m] = t.half(DELP)[hp + (m - 2) * TSH];
-                sm[m] = t.half(SQ1M)[hp + (m - 2) * TSH];
-                sp[m] = t.half(SQ1P)[hp + (m - 2) * TSH];
-            }
-            // ((cc + 1) >> 1) is half column p + par, ((cc - 1) >> 1) is
-            // p + par - 1, ((cc + 2) >> 1) is p + 1, ((cc - 2) >> 1) is p - 1.
-            auto hm = [&](int m, int s) {
-                return L.half_tap(dm[m], t.half(DELM) + (rr + m - 2) * TSH, s, valid8);
-            };
-            auto hq = [&](int m, int s) {
-                return L.half_tap(dp[m], t.half(DELP) + (rr + m - 2) * TSH, s, valid8);
-            };
-            const float dm_se1 = hm(3, par), dm_se2 = hm(4, 1);
-            const float dm_nw1 = hm(1, par - 1), dm_nw2 = hm(0, -1);
-            const float dp_ne1 = hq(1, par), dp_ne2 = hq(0, 1);
-            const float dp_sw1 = hq(3, par - 1), dp_sw2 = hq(4, -1);
-            const float rbvarm_even = even_lanes(L, t.half(SQ1M), rr, sm, valid8);
-            const float rbvarp_even = even_lanes(L, t.half(SQ1P), rr, sp, valid8);
-            if (valid8) {
-                float rbse = diagonal_estimate(centre, se1, se2);
-                float rbnw = diagonal_estimate(centre, nw1, nw2);
-                float rbne = diagonal_estimate(centre, ne1, ne2);
-                float rbsw = diagonal_estimate(centre, sw1, sw2);
-                float wtse = EPS + dm[2] + dm_se1 + dm_se2;
-                float wtnw = EPS + dm[2] + dm_nw1 + dm_nw2;
-                float wtne = EPS + dp[2] + dp_ne1 + dp_ne2;
-                float wtsw = EPS + dp[2] + dp_sw1 + dp_sw2;
-                float rbm_est = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
-                float rbp_est = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
-                float rbvarm = EPSSQ + rbvarm_even;
-                t.half(PMWT)[hp] = rbvarm / ((EPSSQ + rbvarp_even) + rbvarm);
-                t.half(RBP)[hp] = bound_towards(rbp_est, centre, sw1, ne1, clip_pt);
-                t.half(RBM)[hp] = bound_towards(rbm_est, centre, nw1, se1, clip_pt);
-            }
+        // The row's bounding box of flagged sites (nyquist.rs:59-89): cc
+        // increases with the lane, so the flagged sites span [cc of the first
+        // flagged lane, cc of the last]. Every wavefront row writes its slot
+        // (empty: INT_MAX, -1, INT_MAX, -1); amz_box reduces the slots, so no
+        // two rows touch the same address.
+        const unsigned long long m = __ballot(flagged);
+        int4 box = make_int4(INT_MAX, -1, INT_MAX, -1);
+        if (m) {
+            const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
+            box = make_int4(rr, rr, __shfl(cc, first, 64), __shfl(cc, last, 64));
         }
+        if (lane == 0) reinterpret_cast<int4*>(t.boxes())[rr * 2 + blk.x] = box;
     }
-    // The tile's bounding box of flagged sites (nyquist.rs:59-89). A
-    // wavefront is one half row of the tile, with cc increasing by lane, so
-    // its flagged sites span [cc of the first flagged lane, cc of the last].
-    // Every wavefront writes its slot (empty: INT_MAX, -1, INT_MAX, -1);
-    // amz_box reduces the slots, so no two waves touch the same address.
-    const unsigned long long m = __ballot(flagged);
-    int4 box = make_int4(INT_MAX, -1, INT_MAX, -1);
-    if (m) {
-        const int first = __ffsll((long long)m) - 1, last = 63 - __clzll((long long)m);
-        box = make_int4(rr, rr, __shfl(cc, first, 64), __shfl(cc, last, 64));
-    }
-    if (lane == 0) reinterpret_cast<int4*>(t.boxes())[rr * 2 + blk.x] = box;
 }
 
 
@@ -923,15 +932,16 @@ __global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
         const float* hv = t.half(HVWT);
         if (!(fabsf(0.5f - pm[h]) < fabsf(0.5f - hv[h]))) {
             const float* rb = t.half(RBINT);
-            const float* w0 = t.full(DW0);
-            const float* w1 = t.full(DW1);
+            auto F = [&](int r, int c) { return cfa[r * TS + c]; };
+            const float w0u = vgrad(F, rr - 1, cc), w0d = vgrad(F, rr + 1, cc);
+            const float w1l = hgrad(F, rr, cc - 1), w1r = hgrad(F, rr, cc + 1);
             float centre = rb[h];
             float gu = green_from_rb(cfa[i - TS], centre, rb[h - 2 * TSH]);
             float gd = green_from_rb(cfa[i + TS], centre, rb[h + 2 * TSH]);
             float gl = green_from_rb(cfa[i - 1], centre, rb[h - 1]);
             float gr = green_from_rb(cfa[i + 1], centre, rb[h + 1]);
-            float gintv = (w0[i - TS] * gd + w0[i + TS] * gu) / (w0[i + TS] + w0[i - TS]);
-            float ginth = (w1[i - 1] * gr + w1[i + 1] * gl) / (w1[i - 1] + w1[i + 1]);
+            float gintv = (w0u * gd + w0d * gu) / (w0d + w0u);
+            float ginth = (w1l * gr + w1r * gl) / (w1l + w1r);
             gintv = bound_towards(gintv, centre, cfa[i - TS], cfa[i + TS], clip_pt);
             ginth = bound_towards(ginth, centre, cfa[i - 1], cfa[i + 1], clip_pt);
             float green = ginth * (1.0f - hv[h]) + gintv * hv[h];
@@ -1042,30 +1052,28 @@ size_t amaze_scratch_bytes_per_tile() { return TILE_FLOATS * sizeof(float) + 4 *
 // a.tile_base; a.scratch and a.areas hold one slot per tile of this launch.
 hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* prof) {
     const float clip_pt8 = 0.8f * a.clip_point;
-    const dim3 px_grid(TS / 64, TS / 4, tiles), pair_grid(TSH / 64, TS / 4, tiles), blk(256);
+    const dim3 pair_grid(TSH / 64, TS / 4, tiles), blk(256);
     // Algorithmic bytes: the planes each stage must read and write once, per
     // tile (f = full-plane floats per tile, h = half-plane floats per tile).
     const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
-    RHD_TIMED(prof, s, "amz_load", f + f + h, hipLaunchKernelGGL(amz_load, px_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_gradients", f + 3 * f + 4 * h,
-              hipLaunchKernelGGL(amz_gradients, px_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_colour_differences", 3 * f + 6 * f,
-              hipLaunchKernelGGL(amz_colour_differences, px_grid, blk, 0, s, a, clip_pt8));
+    RHD_TIMED(prof, s, "amz_front", f + 7 * f + 3 * h + h / 2,
+              hipLaunchKernelGGL(amz_front, dim3(TS / FW, TS / FH, tiles), blk, 0, s, a, clip_pt8,
+                                 a.clip_point));
     RHD_TIMED(prof, s, "amz_bound_v", 3 * f + f,
               hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a,
                                  a.clip_point));
-    RHD_TIMED(prof, s, "amz_bound_h", 4 * f + f + f / 2,
+    RHD_TIMED(prof, s, "amz_bound_h", 3 * f + f,
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
                                  dim3(BH_ROWS), 0, s, a, a.clip_point));
-    RHD_TIMED(prof, s, "amz_weights", 9 * f / 2 + 7 * h + h / 2 + h / 8,
-              hipLaunchKernelGGL(amz_weights, pair_grid, blk, 0, s, a, a.clip_point));
+    RHD_TIMED(prof, s, "amz_weights", 5 * f + h,
+              hipLaunchKernelGGL(amz_weights, dim3(TS / WW, TS / WH, tiles), blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_box", 16ull * BOX_SLOTS * tiles,
               hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_chains", 3 * f / 2 + 6 * h + 6 * h,
               hipLaunchKernelGGL(amz_chains, dim3(tiles), dim3(TSH), 0, s, a));
-    RHD_TIMED(prof, s, "amz_refine", f / 2 + 2 * f / 2 + 5 * h + 3 * h,
+    RHD_TIMED(prof, s, "amz_refine", f / 2 + 5 * h + 3 * h,
               hipLaunchKernelGGL(amz_refine, pair_grid, blk, 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_chroma", 2 * h + h, hipLaunchKernelGGL(amz_chroma, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_finish", 12ull * a.width * a.height * tiles / (a.tiles_x * a.tiles_y) + f / 2 + 4 * h,
