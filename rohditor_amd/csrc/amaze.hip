@@ -7,12 +7,12 @@
 // cores, 16-pixel mirrored halo, 256-pixel tile planes) is kept because the
 // Nyquist bounding box and the three in-place chains are per tile.
 //
-// Layout: every tile owns one scratch block in HBM (7 full planes of 256x256
+// Layout: every tile owns one scratch block in HBM (5 full planes of 256x256
 // f32, 10 half planes of 256x128 f32 indexed by pixel pair, 2 byte planes).
 // Green is stored only at red/blue sites (the GRB half plane); at green
 // sites it equals the CFA sample. Stencil quantities that are cheap to form
 // from the CFA samples (the directional and diagonal gradients, delhvsqsum,
-// cddiffsq) are never stored: the kernels that read them form them in LDS
+// cddiffsq, the alternative colour differences) are never stored: the kernels that read them form them in LDS
 // or registers. Stages run as grid kernels over (tile, block of rows and
 // columns or pairs); the three sequential chains run as
 //   - bound_v: one thread per tile column walking down the rows (vcd),
@@ -31,7 +31,7 @@ namespace {
 constexpr int TS = 256, TSH = 128, BORDER = 16, VALID = TS - 2 * BORDER;
 constexpr int FULL = TS * TS, HALF = TS * TSH;
 
-enum FullPlane { CFA, VCD, HCD, VALT, HALT, DGV, DGH, NUM_FULL };
+enum FullPlane { CFA, VCD, HCD, DGV, DGH, NUM_FULL };
 enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, PMWT, RBM, RBP, RBINT, NUM_HALF };
 // Per-row partial Nyquist boxes: one {min row, max row, min col, max col}
 // slot per (row, half-row) of the tile.
@@ -105,7 +105,7 @@ __device__ __forceinline__ float bound_towards(float estimate, float centre, flo
 // samples, 4 around, are staged in LDS from the mosaic; the directional
 // gradients (2 around) and the diagonal gradients and squares (2 rows and
 // one pair around) are formed in LDS from them. Only what later stages read
-// leaves for HBM: the CFA, the four colour differences and their gradients,
+// leaves for HBM: the CFA, the two colour differences and their gradients,
 // the diagonal estimates and weights, and the zeroed state of the planes
 // that later stages write only in part (scratch.rs:113-145).
 constexpr int FW = 64, FH = 32;
@@ -154,42 +154,66 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     auto W1 = [&](int rr, int cc) { return s.w1[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
     auto D = [&](const float* pl, int rr, int p) { return pl[(rr - r0 + 2) * FD_W + (p - p0 + 1)]; };
 
+    // Lanes take columns, wavefronts rows: a lane's source column, mirror
+    // and channel gains are formed once, a row's source row once per wave.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+
     // CFA samples of the block and its halo (zero outside the tile, where
-    // no stencil of a written site reaches).
+    // no stencil of a written site reaches). Columns c0 - 4 + lane and, for
+    // the first FC_W - 64 lanes, c0 + 60 + lane.
     {
+        static_assert(FC_W > 64 && FC_W <= 128, "two column slots per lane");
         const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
         const int ox = tx * VALID - BORDER, oy = ty * VALID - BORDER;
-        for (int k = threadIdx.x; k < FC_H * FC_W; k += 256) {
-            const int ly = k / FC_W, lx = k - ly * FC_W;
-            const int rr = r0 - 4 + ly, cc = c0 - 4 + lx;
-            float v = 0.0f;
-            if (rr >= 0 && rr < TS && cc >= 0 && cc < TS) {
-                const int sx = mirror_whole_near(ox + cc, a.width);
-                const int sy = mirror_whole_near(oy + rr, a.height);
-                const int ch = fc(a.pattern, rr, cc);
-                const float gain = ch == 0 ? a.forward[0] : (ch == 1 ? a.forward[1] : a.forward[2]);
-                v = a.mosaic[(size_t)sy * a.width + sx] * gain;
+        int sx[2];
+        bool in[2];
+        float gain[2][2];  // [column slot][row parity]
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int cc = c0 - 4 + lane + 64 * m;
+            in[m] = (m == 0 || lane < FC_W - 64) && cc >= 0 && cc < TS;
+            sx[m] = in[m] ? mirror_whole_near(ox + cc, a.width) : 0;
+#pragma unroll
+            for (int rp = 0; rp < 2; ++rp) {
+                const int ch = fc(a.pattern, rp, cc & 1);
+                gain[m][rp] = ch == 0 ? a.forward[0] : (ch == 1 ? a.forward[1] : a.forward[2]);
             }
-            s.cfa[k] = v;
+        }
+        for (int ly = wave; ly < FC_H; ly += 4) {
+            const int rr = r0 - 4 + ly;
+            const bool row_in = rr >= 0 && rr < TS;
+            const float* src = a.mosaic + (size_t)(row_in ? mirror_whole_near(oy + rr, a.height) : 0) * a.width;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                if (m == 1 && lane >= FC_W - 64) break;
+                s.cfa[ly * FC_W + lane + 64 * m] = row_in && in[m] ? src[sx[m]] * gain[m][rr & 1] : 0.0f;
+            }
         }
     }
     __syncthreads();
     // Gradients over [2, TS - 2) and the diagonal planes over [6, TS - 6)
     // (even columns, one entry per pair); zero elsewhere, unread.
-    for (int k = threadIdx.x; k < FG_H * FG_W; k += 256) {
-        const int ly = k / FG_W, lx = k - ly * FG_W;
-        const int rr = r0 - 2 + ly, cc = c0 - 2 + lx;
-        float w0 = 0.0f, w1 = 0.0f;
-        if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
-            w0 = vgrad(C, rr, cc);
-            w1 = hgrad(C, rr, cc);
+    static_assert(FG_W > 64 && FG_W <= 128 && FD_W <= 64, "column slots per lane");
+    for (int ly = wave; ly < FG_H; ly += 4) {
+        const int rr = r0 - 2 + ly;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            if (m == 1 && lane >= FG_W - 64) break;
+            const int lx = lane + 64 * m, cc = c0 - 2 + lx;
+            float w0 = 0.0f, w1 = 0.0f;
+            if (rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2) {
+                w0 = vgrad(C, rr, cc);
+                w1 = hgrad(C, rr, cc);
+            }
+            s.w0[ly * FG_W + lx] = w0;
+            s.w1[ly * FG_W + lx] = w1;
         }
-        s.w0[k] = w0;
-        s.w1[k] = w1;
     }
-    for (int k = threadIdx.x; k < FD_H * FD_W; k += 256) {
-        const int ly = k / FD_W, lx = k - ly * FD_W;
-        const int rr = r0 - 2 + ly, cc = c0 - 2 + 2 * lx;
+    for (int ly = wave; ly < FD_H; ly += 4) {
+        const int rr = r0 - 2 + ly;
+        if (lane >= FD_W) continue;
+        const int cc = c0 - 2 + 2 * lane;
         float delp = 0.0f, delm = 0.0f, sqp = 0.0f, sqm = 0.0f;
         if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
             if (gae(a.pattern, rr) == 0) {
@@ -206,6 +230,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
                 delm = fabsf(C(rr + 1, cc + 2) - C(rr - 1, cc));
             }
         }
+        const int k = ly * FD_W + lane;
         s.dp[k] = delp;
         s.dm[k] = delm;
         s.qp[k] = sqp;
@@ -214,9 +239,8 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     __syncthreads();
 
     // Per site: the CFA plane and the colour differences (green.rs:46-137);
-    // vcd, hcd and their alternatives are zero in the 4-pixel band, where
-    // later stages read them but nothing writes them.
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // vcd and hcd are zero in the 4-pixel band, where later stages read them
+    // but nothing writes them.
     const int cc = c0 + lane;
     for (int j = 0; j < FH / 4; ++j) {
         const int rr = r0 + wave * (FH / 4) + j;
@@ -226,8 +250,6 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         if (!(rr >= 4 && rr < TS - 4 && cc >= 4 && cc < TS - 4)) {
             t.full(VCD)[i] = 0.0f;
             t.full(HCD)[i] = 0.0f;
-            t.full(VALT)[i] = 0.0f;
-            t.full(HALT)[i] = 0.0f;
             continue;
         }
         const bool fcswitch = (fc(a.pattern, rr, cc) & 1) == 1;  // green site
@@ -280,8 +302,6 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         t.full(DGH)[i] = sel_min(sqr(glha - grha), sqr(glar - grar));
         t.full(VCD)[i] = vcd;
         t.full(HCD)[i] = hcd;
-        t.full(VALT)[i] = vcdalt;
-        t.full(HALT)[i] = hcdalt;
     }
 
     // Per red/blue site (one per pair): the zeroed Nyquist flags, d2h / d2v
@@ -370,36 +390,72 @@ __device__ __forceinline__ float spread(float a, float b, float c) {
     return 3.0f * (sqr(a) + sqr(b) + sqr(c)) - sqr(a + b + c);
 }
 
+// The alternative colour differences (vcdalt, hcdalt of green.rs:46-137):
+// the Hamilton-Adams estimates along one axis, weighted by that axis's
+// gradients. Each depends on the CFA samples of its own column (row) only,
+// three either side, so the bound kernels form them from the CFA samples
+// they walk instead of reading a stored plane. f(d) is the sample d away
+// along the axis.
+template <class F>
+__device__ __forceinline__ float alt_difference(const F& f, bool green) {
+    const float cur = f(0);
+    const float w_m1 = EPS + fabsf(f(1) - f(-1)) + fabsf(f(-1) - f(-3)) + fabsf(f(0) - f(-2));
+    const float w_p1 = EPS + fabsf(f(3) - f(1)) + fabsf(f(1) - f(-1)) + fabsf(f(2) - f(0));
+    const float wt = w_m1 / (w_p1 + w_m1);
+    const float ha_m = f(-1) + 0.5f * (cur - f(-2));
+    const float ha_p = f(1) + 0.5f * (cur - f(2));
+    const float gint = wt * ha_p + (1.0f - wt) * ha_m;
+    return green ? cur - gint : gint - cur;
+}
+
 // green.rs:184-226, vertical half: vcd is bounded in place going down each
 // column; the value two rows up is already bounded when read. Rows are
-// processed four at a time with the next four rows' inputs loaded up front,
-// so the loop waits for memory once per four rows.
+// processed four at a time with the next four rows' inputs (vcd, and the CFA
+// samples that give vcdalt) loaded up front, so the loop waits for memory
+// once per four rows.
 __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
     const int cc = blk.x * 64 + threadIdx.x;
     if (cc < 4 || cc >= TS - 4) return;
     float* __restrict__ vcd = t.full(VCD);
-    const float* __restrict__ alt = t.full(VALT);
     const float* __restrict__ cfa = t.full(CFA);
+    // vcdalt at row r (zero outside [4, TS - 4)) from the CFA samples of rows
+    // r - 3 .. r + 3 held in w[0 .. 6].
+    auto alt_at = [&](const float* w, int r, bool green) {
+        return r < TS - 4 ? alt_difference([&](int d) { return w[3 + d]; }, green) : 0.0f;
+    };
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded vcd two and one rows up (rows 2, 3 are zero)
     float a_m2 = 0.0f, a_m1 = 0.0f;  // vcdalt two and one rows up
-    float u0 = vcd[4 * TS + cc], u1 = vcd[5 * TS + cc];  // unbounded vcd at rr, rr + 1
-    float a0 = alt[4 * TS + cc], a1 = alt[5 * TS + cc];
-    float c_m1 = cfa[3 * TS + cc], c0 = cfa[4 * TS + cc];
     bool green = (fc(a.pattern, 4, cc) & 1) == 1;
+    float u0 = vcd[4 * TS + cc], u1 = vcd[5 * TS + cc];  // unbounded vcd at rr, rr + 1
+    float a0, a1;
+    {
+        float w[9];  // CFA rows 1 .. 9
+#pragma unroll
+        for (int j = 0; j < 9; ++j) w[j] = cfa[(1 + j) * TS + cc];
+        a0 = alt_at(w, 4, green);
+        a1 = alt_at(w + 1, 5, !green);
+    }
+    float c_m1 = cfa[3 * TS + cc], c0 = cfa[4 * TS + cc];
     for (int rr = 4; rr < TS - 4; rr += 4) {
-        float U[4], A[4], C[4];
+        // Rows rr + 2 .. rr + 5: unbounded vcd, and vcdalt from CFA rows
+        // rr - 1 .. rr + 8 (w[j] is row rr - 1 + j; rows past the tile read
+        // as zero and only feed vcdalt rows outside [4, TS - 4)).
+        float U[4], w[10];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int r2 = rr + 2 + k;
             U[k] = r2 < TS - 4 ? vcd[r2 * TS + cc] : 0.0f;
-            A[k] = r2 < TS - 4 ? alt[r2 * TS + cc] : 0.0f;
-            C[k] = cfa[(rr + 1 + k) * TS + cc];
         }
 #pragma unroll
+        for (int j = 0; j < 10; ++j) w[j] = rr - 1 + j < TS ? cfa[(rr - 1 + j) * TS + cc] : 0.0f;
+        float A[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) A[k] = alt_at(w + k, rr + 2 + k, green != ((k & 1) != 0));
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float u2 = U[k], a2 = A[k], c1 = C[k];
+            const float u2 = U[k], a2 = A[k], c1 = w[k + 2];
             float vcdvar = spread(b_m2, u0, u2);
             float vcdaltvar = spread(a_m2, a0, a2);
             float v = vcdaltvar < vcdvar ? a0 : u0;
@@ -422,13 +478,14 @@ __global__ __launch_bounds__(64) void amz_bound_v(AmazeArgs a, float clip_pt) {
 }
 
 // green.rs:184-226, horizontal half: hcd is bounded in place along each row.
-// One thread per tile row walks it in 32-column chunks: each chunk's hcd,
-// hcdalt and CFA columns (with the 4-column halo on the left, 8 on the right)
-// arrive as aligned float4 loads straight into registers, the serial
-// recurrence runs fully unrolled over them, and hcd leaves as float4 stores.
-// cddiffsq (green.rs:219-223) is formed where it is read, by the Nyquist
-// test, from the bounded vcd and hcd.
-constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_IN = BH_CHUNK + 8;
+// One thread per tile row walks it in 32-column chunks: each chunk's hcd
+// (with the 4-column halo on the left, 8 on the right) and CFA samples (8
+// and 8) arrive as aligned float4 loads straight into registers, hcdalt is
+// formed from the CFA samples, the serial recurrence runs fully unrolled
+// over them, and hcd leaves as float4 stores. cddiffsq (green.rs:219-223)
+// is formed where it is read, by the Nyquist test, from the bounded vcd and
+// hcd.
+constexpr int BH_ROWS = 64, BH_CHUNK = 32, BH_IN = BH_CHUNK + 8, BH_CF = BH_CHUNK + 16;
 
 __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     const uint3 blk = xcd_block();
@@ -436,32 +493,40 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
     const int rr = 4 + blk.x * BH_ROWS + (int)threadIdx.x;
     if (rr >= TS - 4) return;
     float* __restrict__ hcd_row = t.full(HCD) + rr * TS;
-    const float* __restrict__ alt_row = t.full(HALT) + rr * TS;
     const float* __restrict__ cfa_row = t.full(CFA) + rr * TS;
     const bool green4 = (fc(a.pattern, rr, 4) & 1) == 1;  // colour at column 4 (and c0)
     float b_m2 = 0.0f, b_m1 = 0.0f;  // bounded hcd two and one columns left (cols 2, 3 are zero)
     for (int c0 = 4; c0 < TS - 4; c0 += BH_CHUNK) {
-        // Register k holds column c0 - 4 + k. hcd and hcdalt read as zero
-        // outside [4, TS - 4), the CFA as zero beyond the tile.
-        float h[BH_IN], al[BH_IN], cf[BH_IN];
+        // h[k] and al[k] hold column c0 - 4 + k, cf[k] column c0 - 8 + k.
+        // hcd and hcdalt read as zero outside [4, TS - 4), the CFA as zero
+        // outside the tile.
+        float h[BH_IN], al[BH_IN], cf[BH_CF];
 #pragma unroll
         for (int q = 0; q < BH_IN / 4; ++q) {
             const int col = c0 - 4 + 4 * q;
-            float4 hv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), av = hv, cv = hv;
-            if (col < TS) {
-                hv = *reinterpret_cast<const float4*>(hcd_row + col);
-                av = *reinterpret_cast<const float4*>(alt_row + col);
-                cv = *reinterpret_cast<const float4*>(cfa_row + col);
-            }
-            const float hs[4] = {hv.x, hv.y, hv.z, hv.w}, as[4] = {av.x, av.y, av.z, av.w};
-            const float cs[4] = {cv.x, cv.y, cv.z, cv.w};
+            float4 hv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (col < TS) hv = *reinterpret_cast<const float4*>(hcd_row + col);
+            const float hs[4] = {hv.x, hv.y, hv.z, hv.w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const bool inside = col + e >= 4 && col + e < TS - 4;
-                h[4 * q + e] = inside ? hs[e] : 0.0f;
-                al[4 * q + e] = inside ? as[e] : 0.0f;
-                cf[4 * q + e] = col + e < TS ? cs[e] : 0.0f;
-            }
+            for (int e = 0; e < 4; ++e) h[4 * q + e] = col + e >= 4 && col + e < TS - 4 ? hs[e] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < BH_CF / 4; ++q) {
+            const int col = c0 - 8 + 4 * q;
+            float4 cv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (col >= 0 && col < TS) cv = *reinterpret_cast<const float4*>(cfa_row + col);
+            cf[4 * q] = cv.x;
+            cf[4 * q + 1] = cv.y;
+            cf[4 * q + 2] = cv.z;
+            cf[4 * q + 3] = cv.w;
+        }
+        // hcdalt at columns c0 - 2 .. c0 + 33, the ones the chunk reads.
+#pragma unroll
+        for (int j = 2; j < BH_CHUNK + 6; ++j) {
+            const int col = c0 - 4 + j;
+            al[j] = col >= 4 && col < TS - 4
+                        ? alt_difference([&](int d) { return cf[j + 4 + d]; }, green4 != ((j & 1) != 0))
+                        : 0.0f;
         }
         float out[BH_CHUNK];
 #pragma unroll
@@ -470,8 +535,8 @@ __global__ __launch_bounds__(64) void amz_bound_h(AmazeArgs a, float clip_pt) {
             const float hcdvar = spread(b_m2, h[k + 4], h[k + 6]);
             const float hcdaltvar = spread(al[k + 2], al[k + 4], al[k + 6]);
             const float hv = hcdaltvar < hcdvar ? al[k + 4] : h[k + 4];
-            const float bounded = green ? bound_at_green(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt)
-                                        : bound_at_red_blue(hv, cf[k + 4], cf[k + 3], cf[k + 5], clip_pt);
+            const float bounded = green ? bound_at_green(hv, cf[k + 8], cf[k + 7], cf[k + 9], clip_pt)
+                                        : bound_at_red_blue(hv, cf[k + 8], cf[k + 7], cf[k + 9], clip_pt);
             out[k] = bounded;
             b_m2 = b_m1;
             b_m1 = bounded;
@@ -527,12 +592,16 @@ __global__ __launch_bounds__(256) void amz_box(AmazeArgs a) {
 // are staged in LDS with the halo the stencils need (float4 loads of whole
 // plane rows); the directional gradients and delhvsqsum (dhv) are formed
 // from the CFA samples there, cddiffsq from the bounded colour differences.
-// The arithmetic of each site is the reference's, term for term.
-constexpr int WW = 128, WH = 16;
+// LDS rows are stored split by column parity (even columns, then odd), so
+// the lanes of a wavefront, whose sites are two columns apart, read
+// consecutive words. The arithmetic of each site is the reference's, term
+// for term.
+constexpr int WW = 128, WH = 8;
 constexpr int WS_W = WW + 8;                 // staged columns c0 - 4 .. c0 + WW + 4
 constexpr int WC_H = WH + 6;                 // CFA and vcd rows r0 - 3 .. r0 + WH + 3
 constexpr int WK_H = WH + 4;                 // hcd and dgv rows r0 - 2 .. r0 + WH + 2
 constexpr int WD_W = WW + 4, WD_H = WH + 4;  // dhv: columns c0 - 2 .., rows r0 - 2 ..
+constexpr int WQ_W = WD_W / 2;               // cddiffsq: the red/blue sites of those columns
 static_assert(TS % WW == 0 && TS % WH == 0 && WH % 4 == 0 && WW == 128, "whole blocks");
 
 struct alignas(16) WeightsLds {
@@ -542,20 +611,97 @@ struct alignas(16) WeightsLds {
     float dgv[WK_H * WW];
     float dgh[WH * WS_W];
     float dhv[WD_H * WD_W];
+    float cds[WD_H * WQ_W];
 };
 
-// Rows [row0, row0 + rows) of a tile plane, columns [col0, col0 + 4 * quads),
-// into LDS rows of `stride` floats; zero outside the tile.
-__device__ __forceinline__ void stage_rows(float* dst, int stride, const float* plane, int row0,
-                                           int rows, int col0, int quads) {
+// Word of column lc (counted from the row's first staged column, which is
+// even) in an LDS row of `width` words split by parity.
+__device__ __forceinline__ int split_at(int width, int lc) { return (lc & 1) * (width / 2) + (lc >> 1); }
+
+// Rows [row0, row0 + rows) of a tile plane, columns [col0, col0 + 4 * quads)
+// (col0 a multiple of 4), into parity-split LDS rows of 4 * quads words;
+// zero outside the tile.
+__device__ __forceinline__ void stage_rows(float* dst, const float* plane, int row0, int rows,
+                                           int col0, int quads) {
     for (int k = threadIdx.x; k < rows * quads; k += 256) {
         const int ly = k / quads, q = k - ly * quads;
         const int rr = row0 + ly, cc = col0 + 4 * q;
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (rr >= 0 && rr < TS && cc >= 0 && cc < TS)
             v = *reinterpret_cast<const float4*>(plane + rr * TS + cc);
-        *reinterpret_cast<float4*>(dst + ly * stride + 4 * q) = v;
+        float* row = dst + ly * 4 * quads;
+        *reinterpret_cast<float2*>(row + 2 * q) = make_float2(v.x, v.z);
+        *reinterpret_cast<float2*>(row + 2 * quads + 2 * q) = make_float2(v.y, v.w);
     }
+}
+
+// One red/blue site of a weights block: row ly of the block, column
+// c0 + 2 * lane + PAR. Every LDS word it reads sits at a fixed offset from
+// the lane's own, so the offsets are immediates. Writes hvwt and returns
+// the Nyquist flag.
+template <int PAR>
+__device__ __forceinline__ bool weights_site(const WeightsLds& s, float* hvwt, int ly, int lane) {
+    constexpr int LS = 4 + PAR;  // staged column (from c0 - 4) of the site, less 2 * lane
+    constexpr int LD = 2 + PAR;  // the same from c0 - 2
+    auto at = [&](int dx, int half) { return ((LS + dx) & 1) * half + ((LS + dx) >> 1) + lane; };
+    auto C = [&](int dy, int dx) { return s.cfa[(ly + 3 + dy) * WS_W + at(dx, WS_W / 2)]; };
+    auto V = [&](int dy) { return s.vcd[(ly + 3 + dy) * WS_W + at(0, WS_W / 2)]; };
+    auto H = [&](int dx) { return s.hcd[(ly + 2) * WS_W + at(dx, WS_W / 2)]; };
+    auto G = [&](int dx) { return s.dgh[ly * WS_W + at(dx, WS_W / 2)]; };
+    auto DV = [&](int dy) { return s.dgv[(ly + 2 + dy) * WW + PAR * (WW / 2) + lane]; };
+    auto d = [&](int dy, int dx) {
+        return s.dhv[(ly + 2 + dy) * WD_W + ((LD + dx) & 1) * (WD_W / 2) + ((LD + dx) >> 1) + lane];
+    };
+    auto c = [&](int dy, int dx) { return s.cds[(ly + 2 + dy) * WQ_W + ((LD + dx) >> 1) + lane]; };
+    // HV weights (green.rs:229-292).
+    float v[7], dv[5], hcd[7], dgh[5];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = V(k - 3);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dv[k] = DV(k - 2);
+#pragma unroll
+    for (int k = -3; k <= 3; ++k) hcd[k + 3] = H(k);
+#pragma unroll
+    for (int k = -2; k <= 2; ++k) dgh[k + 2] = G(k);
+    const float w0u = vgrad(C, -1, 0), w0d = vgrad(C, 1, 0);
+    const float w1l = hgrad(C, 0, -1), w1r = hgrad(C, 0, 1);
+    float uave = v[3] + v[2] + v[1] + v[0];
+    float dave = v[3] + v[4] + v[5] + v[6];
+    float lave = hcd[3] + hcd[2] + hcd[1] + hcd[0];
+    float rave = hcd[3] + hcd[4] + hcd[5] + hcd[6];
+    float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
+    float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
+    float varl = sqr(hcd[3] - lave) + sqr(hcd[2] - lave) + sqr(hcd[1] - lave) + sqr(hcd[0] - lave);
+    float varr = sqr(hcd[3] - rave) + sqr(hcd[4] - rave) + sqr(hcd[5] - rave) + sqr(hcd[6] - rave);
+    float hwt = w1l / (w1l + w1r);
+    float vwt = w0u / (w0d + w0u);
+    float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
+    float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
+    float fluct_u = dv[2] + dv[1] + dv[0];
+    float fluct_d = dv[2] + dv[3] + dv[4];
+    float fluct_l = dgh[2] + dgh[1] + dgh[0];
+    float fluct_r = dgh[2] + dgh[3] + dgh[4];
+    float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
+    float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
+    float varwt = hcdvar / (vcdvar + hcdvar);
+    float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
+    *hvwt = ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
+                ? varwt
+                : diffwt;
+    // Nyquist test statistic (nyquist.rs:28-57); cddiffsq at the red/blue
+    // sites the stencil reads is (vcd - hcd)^2 of the bounded differences
+    // (green.rs:219-223).
+    float test =
+        (GAUSSODD[0] * c(0, 0) + GAUSSODD[1] * (c(-1, -1) + c(-1, 1) + c(1, -1) + c(1, 1)) +
+         GAUSSODD[2] * (c(-2, 0) + c(0, -2) + c(0, 2) + c(2, 0)) +
+         GAUSSODD[3] * (c(-2, -2) + c(-2, 2) + c(2, -2) + c(2, 2))) -
+        (GAUSSGRAD[0] * d(0, 0) + GAUSSGRAD[1] * (d(-1, 0) + d(0, 1) + d(0, -1) + d(1, 0)) +
+         GAUSSGRAD[2] * (d(-1, -1) + d(-1, 1) + d(1, -1) + d(1, 1)) +
+         GAUSSGRAD[3] * (d(-2, 0) + d(0, -2) + d(0, 2) + d(2, 0)) +
+         GAUSSGRAD[4] * (d(-2, -1) + d(-2, 1) + d(-1, -2) + d(-1, 2) + d(1, -2) + d(1, 2) +
+                         d(2, -1) + d(2, 1)) +
+         GAUSSGRAD[5] * (d(-2, -2) + d(-2, 2) + d(2, -2) + d(2, 2)));
+    return test > 0.0f;
 }
 
 __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
@@ -563,89 +709,45 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
     const Tile t = tile_of(a, blk.z);
     const int c0 = blk.x * WW, r0 = blk.y * WH;
     __shared__ WeightsLds s;
-    stage_rows(s.cfa, WS_W, t.full(CFA), r0 - 3, WC_H, c0 - 4, WS_W / 4);
-    stage_rows(s.vcd, WS_W, t.full(VCD), r0 - 3, WC_H, c0 - 4, WS_W / 4);
-    stage_rows(s.hcd, WS_W, t.full(HCD), r0 - 2, WK_H, c0 - 4, WS_W / 4);
-    stage_rows(s.dgv, WW, t.full(DGV), r0 - 2, WK_H, c0, WW / 4);
-    stage_rows(s.dgh, WS_W, t.full(DGH), r0, WH, c0 - 4, WS_W / 4);
+    stage_rows(s.cfa, t.full(CFA), r0 - 3, WC_H, c0 - 4, WS_W / 4);
+    stage_rows(s.vcd, t.full(VCD), r0 - 3, WC_H, c0 - 4, WS_W / 4);
+    stage_rows(s.hcd, t.full(HCD), r0 - 2, WK_H, c0 - 4, WS_W / 4);
+    stage_rows(s.dgv, t.full(DGV), r0 - 2, WK_H, c0, WW / 4);
+    stage_rows(s.dgh, t.full(DGH), r0, WH, c0 - 4, WS_W / 4);
     __syncthreads();
-    auto C = [&](int rr, int cc) { return s.cfa[(rr - r0 + 3) * WS_W + (cc - c0 + 4)]; };
-    auto V = [&](int rr, int cc) { return s.vcd[(rr - r0 + 3) * WS_W + (cc - c0 + 4)]; };
-    auto H = [&](int rr, int cc) { return s.hcd[(rr - r0 + 2) * WS_W + (cc - c0 + 4)]; };
-    for (int k = threadIdx.x; k < WD_H * WD_W; k += 256) {
-        const int ly = k / WD_W, lx = k - ly * WD_W;
-        const int rr = r0 - 2 + ly, cc = c0 - 2 + lx;
-        s.dhv[k] = rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2 ? delhv(C, rr, cc) : 0.0f;
+    auto C = [&](int rr, int cc) { return s.cfa[(rr - r0 + 3) * WS_W + split_at(WS_W, cc - c0 + 4)]; };
+    auto V = [&](int rr, int cc) { return s.vcd[(rr - r0 + 3) * WS_W + split_at(WS_W, cc - c0 + 4)]; };
+    auto H = [&](int rr, int cc) { return s.hcd[(rr - r0 + 2) * WS_W + split_at(WS_W, cc - c0 + 4)]; };
+    // dhv over [2, TS - 2) and cddiffsq at the red/blue sites; zero
+    // elsewhere, where no valid site's stencil reaches. Lanes take columns
+    // c0 - 2 + lane + 64 m, wavefronts rows.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    static_assert(WD_W > 128 && WD_W <= 192, "three column slots per lane");
+    for (int ly = wave; ly < WD_H; ly += 4) {
+        const int rr = r0 - 2 + ly;
+        const int par = gae(a.pattern, rr);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            if (m == 2 && lane >= WD_W - 128) break;
+            const int lc = lane + 64 * m, cc = c0 - 2 + lc;
+            const bool inside = rr >= 2 && rr < TS - 2 && cc >= 2 && cc < TS - 2;
+            s.dhv[ly * WD_W + split_at(WD_W, lc)] = inside ? delhv(C, rr, cc) : 0.0f;
+            if ((cc & 1) == par)
+                s.cds[ly * WQ_W + (lc >> 1)] = inside ? sqr(V(rr, cc) - H(rr, cc)) : 0.0f;
+        }
     }
     __syncthreads();
-    auto DHV = [&](int rr, int cc) { return s.dhv[(rr - r0 + 2) * WD_W + (cc - c0 + 2)]; };
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blk.x * 64 + lane;
     for (int j = 0; j < WH / 4; ++j) {
-        const int rr = r0 + wave * (WH / 4) + j;
-        const int cc = 2 * p + gae(a.pattern, rr);
+        const int ly = wave * (WH / 4) + j, rr = r0 + ly;
+        const int par = gae(a.pattern, rr);
+        const int cc = 2 * p + par;
         bool flagged = false;
         if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
-            // HV weights (green.rs:229-292).
-            float v[7], dv[5], hcd[7], dgh[5];
-#pragma unroll
-            for (int k = 0; k < 7; ++k) v[k] = V(rr + k - 3, cc);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) dv[k] = s.dgv[(rr + k - 2 - r0 + 2) * WW + (cc - c0)];
-#pragma unroll
-            for (int d = -3; d <= 3; ++d) hcd[d + 3] = H(rr, cc + d);
-#pragma unroll
-            for (int d = -2; d <= 2; ++d) dgh[d + 2] = s.dgh[(rr - r0) * WS_W + (cc + d - c0 + 4)];
-            const float w0u = vgrad(C, rr - 1, cc), w0d = vgrad(C, rr + 1, cc);
-            const float w1l = hgrad(C, rr, cc - 1), w1r = hgrad(C, rr, cc + 1);
-            float uave = v[3] + v[2] + v[1] + v[0];
-            float dave = v[3] + v[4] + v[5] + v[6];
-            float lave = hcd[3] + hcd[2] + hcd[1] + hcd[0];
-            float rave = hcd[3] + hcd[4] + hcd[5] + hcd[6];
-            float varu = sqr(v[3] - uave) + sqr(v[2] - uave) + sqr(v[1] - uave) + sqr(v[0] - uave);
-            float vard = sqr(v[3] - dave) + sqr(v[4] - dave) + sqr(v[5] - dave) + sqr(v[6] - dave);
-            float varl = sqr(hcd[3] - lave) + sqr(hcd[2] - lave) + sqr(hcd[1] - lave) +
-                         sqr(hcd[0] - lave);
-            float varr = sqr(hcd[3] - rave) + sqr(hcd[4] - rave) + sqr(hcd[5] - rave) +
-                         sqr(hcd[6] - rave);
-            float hwt = w1l / (w1l + w1r);
-            float vwt = w0u / (w0d + w0u);
-            float vcdvar = EPSSQ + vwt * vard + (1.0f - vwt) * varu;
-            float hcdvar = EPSSQ + hwt * varr + (1.0f - hwt) * varl;
-            float fluct_u = dv[2] + dv[1] + dv[0];
-            float fluct_d = dv[2] + dv[3] + dv[4];
-            float fluct_l = dgh[2] + dgh[1] + dgh[0];
-            float fluct_r = dgh[2] + dgh[3] + dgh[4];
-            float vcdvar1 = EPSSQ + vwt * fluct_d + (1.0f - vwt) * fluct_u;
-            float hcdvar1 = EPSSQ + hwt * fluct_r + (1.0f - hwt) * fluct_l;
-            float varwt = hcdvar / (vcdvar + hcdvar);
-            float diffwt = hcdvar1 / (vcdvar1 + hcdvar1);
-            t.half(HVWT)[rr * TSH + p] =
-                ((0.5f - varwt) * (0.5f - diffwt) > 0.0f &&
-                 fabsf(0.5f - diffwt) < fabsf(0.5f - varwt))
-                    ? varwt
-                    : diffwt;
-            // Nyquist test statistic (nyquist.rs:28-57); cddiffsq at the
-            // red/blue sites the stencil reads is (vcd - hcd)^2 of the
-            // bounded differences (green.rs:219-223).
-            auto c = [&](int dy, int dx) { return sqr(V(rr + dy, cc + dx) - H(rr + dy, cc + dx)); };
-            auto d = [&](int dy, int dx) { return DHV(rr + dy, cc + dx); };
-            float test =
-                (GAUSSODD[0] * c(0, 0) +
-                 GAUSSODD[1] * (c(-1, -1) + c(-1, 1) + c(1, -1) + c(1, 1)) +
-                 GAUSSODD[2] * (c(-2, 0) + c(0, -2) + c(0, 2) + c(2, 0)) +
-                 GAUSSODD[3] * (c(-2, -2) + c(-2, 2) + c(2, -2) + c(2, 2))) -
-                (GAUSSGRAD[0] * d(0, 0) + GAUSSGRAD[1] * (d(-1, 0) + d(0, 1) + d(0, -1) + d(1, 0)) +
-                 GAUSSGRAD[2] * (d(-1, -1) + d(-1, 1) + d(1, -1) + d(1, 1)) +
-                 GAUSSGRAD[3] * (d(-2, 0) + d(0, -2) + d(0, 2) + d(2, 0)) +
-                 GAUSSGRAD[4] * (d(-2, -1) + d(-2, 1) + d(-1, -2) + d(-1, 2) + d(1, -2) +
-                                 d(1, 2) + d(2, -1) + d(2, 1)) +
-                 GAUSSGRAD[5] * (d(-2, -2) + d(-2, 2) + d(2, -2) + d(2, 2)));
-            if (test > 0.0f) {
-                t.nyq()[rr * TSH + p] = 1;
-                flagged = true;
-            }
+            float* hv = t.half(HVWT) + rr * TSH + p;
+            flagged = par ? weights_site<1>(s, hv, ly, lane) : weights_site<0>(s, hv, ly, lane);
+            if (flagged) t.nyq()[rr * TSH + p] = 1;
         }
         // The row's bounding box of flagged sites (nyquist.rs:59-89): cc
         // increases with the lane, so the flagged sites span [cc of the first
@@ -1056,13 +1158,13 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
     // Algorithmic bytes: the planes each stage must read and write once, per
     // tile (f = full-plane floats per tile, h = half-plane floats per tile).
     const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
-    RHD_TIMED(prof, s, "amz_front", f + 7 * f + 3 * h + h / 2,
+    RHD_TIMED(prof, s, "amz_front", f + 5 * f + 3 * h + h / 2,
               hipLaunchKernelGGL(amz_front, dim3(TS / FW, TS / FH, tiles), blk, 0, s, a, clip_pt8,
                                  a.clip_point));
-    RHD_TIMED(prof, s, "amz_bound_v", 3 * f + f,
+    RHD_TIMED(prof, s, "amz_bound_v", 2 * f + f,
               hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a,
                                  a.clip_point));
-    RHD_TIMED(prof, s, "amz_bound_h", 3 * f + f,
+    RHD_TIMED(prof, s, "amz_bound_h", 2 * f + f,
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
                                  dim3(BH_ROWS), 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_weights", 5 * f + h,
