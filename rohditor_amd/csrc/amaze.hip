@@ -32,7 +32,7 @@ constexpr int TS = 256, TSH = 128, BORDER = 16, VALID = TS - 2 * BORDER;
 constexpr int FULL = TS * TS, HALF = TS * TSH;
 
 enum FullPlane { CFA, VCD, HCD, DGV, DGH, NUM_FULL };
-enum HalfPlane { GRB, HVWT, DGRB0, DGRB1, D2H, D2V, PMWT, RBM, RBP, RBINT, NUM_HALF };
+enum HalfPlane { HVWT, PMWT, RBM, RBP, NUM_HALF };
 // Per-row partial Nyquist boxes: one {min row, max row, min col, max col}
 // slot per (row, half-row) of the tile.
 constexpr int BOX_SLOTS = TS * 2;
@@ -304,20 +304,15 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         t.full(HCD)[i] = hcd;
     }
 
-    // Per red/blue site (one per pair): the zeroed Nyquist flags, d2h / d2v
-    // zeroed outside [8, TS - 8) where the row chain does not write them,
-    // and the diagonal estimates over [8, TS - 8) (diagonal.rs:98-144).
+    // Per red/blue site (one per pair): the zeroed Nyquist flags and the
+    // diagonal estimates over [8, TS - 8) (diagonal.rs:98-144).
     for (int k = threadIdx.x; k < FH * (FW / 2); k += 256) {
         const int rr = r0 + k / (FW / 2), p = p0 + k % (FW / 2);
         const int par = gae(a.pattern, rr);
         const int cc = 2 * p + par, hp = rr * TSH + p;
         t.nyq()[hp] = 0;
         t.nyq2()[hp] = 0;
-        if (rr < 8 || rr >= TS - 8 || cc < 8 || cc >= TS - 8) {
-            t.half(D2H)[hp] = 0.0f;
-            t.half(D2V)[hp] = 0.0f;
-            continue;
-        }
+        if (rr < 8 || rr >= TS - 8 || cc < 8 || cc >= TS - 8) continue;
         const float centre = C(rr, cc);
         const float se1 = C(rr + 1, cc + 1), se2 = C(rr + 2, cc + 2);
         const float nw1 = C(rr - 1, cc - 1), nw2 = C(rr - 2, cc - 2);
@@ -874,121 +869,55 @@ __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
     }
 }
 
-// ---------------------------------------------------------------- row chains
-// nyquist.rs:159-201 (populate_green: hvwt) and diagonal.rs:149-168
-// (combine_diagonals: pmwt), both updated in place row after row. One
-// workgroup per tile; thread p owns pair p; the previous row's final weights
-// live in LDS.
-__global__ __launch_bounds__(128) void amz_chains(AmazeArgs a) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.x);
-    const int p = threadIdx.x;
-    __shared__ float hv_prev[TSH];
-    __shared__ float pm_prev[TSH];
-    const float* cfa = t.full(CFA);
-    const float* vcd = t.full(VCD);
-    const float* hcd = t.full(HCD);
-    const uint8_t* nyq2 = t.nyq2();
-    float* hvwt = t.half(HVWT);
-    float* pmwt = t.half(PMWT);
-    float* dgrb0 = t.half(DGRB0);
-    float* grb = t.half(GRB);
-    float* d2h = t.half(D2H);
-    float* d2v = t.half(D2V);
-    const float* rbm = t.half(RBM);
-    const float* rbp = t.half(RBP);
-    float* rbint = t.half(RBINT);
-    hv_prev[p] = hvwt[7 * TSH + p];
-    pm_prev[p] = pmwt[9 * TSH + p];
-    // Inputs of one row. Every value is final before the kernel starts except
-    // the current row's weights, which only this row's pass writes, so later
-    // rows' inputs are loaded while the current row is processed.
-    struct RowIn {
-        float hv, hv_dl, hv_dr, vcd, hcd, cfa, cfa_l, cfa_r, cfa_u, cfa_d;
-        float pm, pm_dl, pm_dr, rbm, rbp;
-        uint8_t nyq;
-    };
-    auto load = [&](int rr) {
-        RowIn in{};
-        const int cc = 2 * p + gae(a.pattern, rr);
-        const int h = rr * TSH + p, down = (rr + 1) * TSH;
-        const int i = rr * TS + min(max(cc, 1), TS - 2);
-        in.hv = hvwt[h];
-        in.hv_dl = hvwt[down + ((cc - 1) >> 1)];
-        in.hv_dr = hvwt[down + min((cc + 1) >> 1, TSH - 1)];
-        in.vcd = vcd[i];
-        in.hcd = hcd[i];
-        in.cfa = cfa[i];
-        in.cfa_l = cfa[i - 1];
-        in.cfa_r = cfa[i + 1];
-        in.cfa_u = cfa[i - TS];
-        in.cfa_d = cfa[i + TS];
-        in.nyq = nyq2[h];
-        in.pm = pmwt[h];
-        in.pm_dl = pmwt[down + ((cc - 1) >> 1)];
-        in.pm_dr = pmwt[down + min((cc + 1) >> 1, TSH - 1)];
-        in.rbm = rbm[h];
-        in.rbp = rbp[h];
-        return in;
-    };
-    // CHAIN_AHEAD rows of inputs are in flight: row rr + CHAIN_AHEAD is
-    // requested as row rr is processed (re-reading row rr past the end).
-    constexpr int CHAIN_AHEAD = 8;
-    static_assert((TS - 16) % CHAIN_AHEAD == 0, "whole groups of rows");
-    RowIn buf[CHAIN_AHEAD];
-#pragma unroll
-    for (int k = 0; k < CHAIN_AHEAD; ++k) buf[k] = load(8 + k);
-    __syncthreads();
-    for (int r0 = 8; r0 < TS - 8; r0 += CHAIN_AHEAD)
-#pragma unroll
-    for (int k = 0; k < CHAIN_AHEAD; ++k) {
-        const int rr = r0 + k;
-        const RowIn cur = buf[k];
-        buf[k] = load(rr + CHAIN_AHEAD < TS - 8 ? rr + CHAIN_AHEAD : rr);
-        const int cc = 2 * p + gae(a.pattern, rr);
-        const int h = rr * TSH + p;
-        float hv = cur.hv;
-        if (cc >= 8 && cc < TS - 8) {
-            float alt = (hv_prev[(cc - 1) >> 1] + hv_prev[(cc + 1) >> 1] + cur.hv_dl + cur.hv_dr) /
-                        4.0f;
-            if (fabsf(0.5f - hv) < fabsf(0.5f - alt)) {
-                hv = alt;
-                hvwt[h] = hv;
-            }
-            float dg = intp(hv, cur.vcd, cur.hcd);
-            float green = cur.cfa + dg;
-            dgrb0[h] = dg;
-            grb[h] = green;
-            bool flagged = cur.nyq != 0;
-            d2h[h] = flagged ? sqr(green - 0.5f * (cur.cfa_l + cur.cfa_r)) : 0.0f;
-            d2v[h] = flagged ? sqr(green - 0.5f * (cur.cfa_u + cur.cfa_d)) : 0.0f;
-        }
-        float pm = 0.0f;
-        const bool pm_row = rr >= 10 && rr < TS - 10;
-        if (pm_row) {
-            pm = cur.pm;
-            if (cc >= 10 && cc < TS - 10) {
-                float alt = (pm_prev[(cc - 1) >> 1] + pm_prev[(cc + 1) >> 1] + cur.pm_dl +
-                             cur.pm_dr) /
-                            4.0f;
-                if (fabsf(0.5f - pm) < fabsf(0.5f - alt)) {
-                    pm = alt;
-                    pmwt[h] = pm;
-                }
-                rbint[h] = 0.5f * (cur.cfa + cur.rbm * (1.0f - pm) + cur.rbp * pm);
-            }
-        }
-        __syncthreads();
-        hv_prev[p] = hv;
-        if (pm_row) pm_prev[p] = pm;
-        __syncthreads();
-    }
-}
+// ---------------------------------------------------------------- back end
+// nyquist.rs:159-201 and diagonal.rs:149-168 (the row chains: hvwt in
+// populate_green, pmwt in combine_diagonals), nyquist.rs:204-236 and
+// diagonal.rs:193-247 (refine_green, refine_green_with_rb), chroma.rs:26-96
+// (the G-B split and the diagonal chroma interpolation) and chroma.rs:103-147
+// with mod.rs:196-273 (finish_planes and write_tile), for one tile in one
+// workgroup as a row pipeline. Four groups of two wavefronts (thread p of a
+// group owns pair p) work on different rows at step s:
+//   phase 1: group 0 runs the row chains on row s, group 1 refines row
+//            s - 3, group 2 interpolates chroma on row s - 7, group 3
+//            finishes and writes row s - 9,
+//   barrier,
+//   phase 2: group 0 passes row s's weights on; groups 0 and 1 store their
+//            share of the inputs of row s + 1 (loaded eight steps earlier),
+//   barrier.
+// Each stage reads only rows that the stages before it completed in earlier
+// steps, so every value equals the one the stage sequence produces. The
+// rows still in use live in LDS rings; nothing but the RGB output goes back
+// to HBM.
+constexpr int BACK_THREADS = 512, BACK_AHEAD = 8;
+constexpr int LAG_REFINE = 3, LAG_CHROMA = 7, LAG_FINISH = 9;
+// Ring depths: the span of rows alive at once, from the lags above.
+constexpr int R_LONG = 12;  // cfa, final hvwt, dgrb0, dgrb1, green
+constexpr int R_MID = 6;    // d2h, d2v, rbint
+constexpr int R_SHORT = 4;  // original hvwt / pmwt, final pmwt, vcd, hcd, Nyquist flags
+constexpr int R_RB = 2;     // rbm, rbp
+static_assert((TS - 16) % BACK_AHEAD == 0, "whole groups of steps");
 
-// ---------------------------------------------------------------- refinements
-// nyquist.rs:204-236 (refine_green), diagonal.rs:193-247
-// (refine_green_with_rb) and the G-B split of chroma.rs:26-32, in that order,
-// each on the thread's own red/blue site.
+struct BackLds {
+    float cfa[R_LONG][TS];
+    float hv[R_LONG][TSH], g0[R_LONG][TSH], g1[R_LONG][TSH], grb[R_LONG][TSH];
+    float d2h[R_MID][TSH], d2v[R_MID][TSH], rbint[R_MID][TSH];
+    float hvo[R_SHORT][TSH], pmo[R_SHORT][TSH], pm[R_SHORT][TSH];
+    float vcd[R_SHORT][TSH], hcd[R_SHORT][TSH];
+    float rbm[R_RB][TSH], rbp[R_RB][TSH];
+    uint32_t nyq[R_SHORT][TSH / 4];
+    float hv_prev[TSH], pm_prev[TSH];
+};
+
+// The inputs of step X that a thread loads ahead: its share of CFA row X + 1,
+// of original hvwt / pmwt row X + 1, of rbm / rbp row X, of vcd / hcd at the
+// red/blue sites of row X and of the Nyquist flags of row X.
+struct BackIn {
+    float cfa, w, rb, d;
+    uint32_t nyq;
+};
+
+__device__ __forceinline__ int ring(int row, int depth) { return row % depth; }
+
 __device__ __forceinline__ float green_from_rb(float sample, float centre, float other) {
     float ratio = sample * 2.0f / (EPS + centre + other);
     return fabsf(1.0f - ratio) < ARTHRESH ? centre * ratio : sample + 0.5f * (centre - other);
@@ -1001,80 +930,139 @@ __device__ __forceinline__ void red_offset(int pattern, int& ey, int& ex) {
     else { ey = 1; ex = 1; }
 }
 
-__global__ __launch_bounds__(256) void amz_refine(AmazeArgs a, float clip_pt) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
-    const int p = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int cc = 2 * p + gae(a.pattern, rr);
-    const int i = rr * TS + cc;
-    const int h = i >> 1;
-    const float* cfa = t.full(CFA);
-    float* dgrb0 = t.half(DGRB0);
-    float* grb = t.half(GRB);
-    Area ar;
-    if (tile_area(a, blk.z, ar) && in_area(ar, a.pattern, rr, cc) && t.nyq2()[h] != 0) {
-        auto average = [&](const float* pl) {
-            return EPSSQ + (GQUINC[0] * pl[h] +
-                            GQUINC[1] * (pl[(i - TS - 1) >> 1] + pl[(i - TS + 1) >> 1] +
-                                         pl[(i + TS - 1) >> 1] + pl[(i + TS + 1) >> 1]) +
-                            GQUINC[2] * (pl[(i - 2 * TS) >> 1] + pl[(i - 2) >> 1] +
-                                         pl[(i + 2) >> 1] + pl[(i + 2 * TS) >> 1]) +
-                            GQUINC[3] * (pl[(i - 2 * TS - 2) >> 1] + pl[(i - 2 * TS + 2) >> 1] +
-                                         pl[(i + 2 * TS - 2) >> 1] + pl[(i + 2 * TS + 2) >> 1]));
+__device__ __forceinline__ void back_load(const Tile& t, int pattern, int x, BackIn& in) {
+    const int tid = threadIdx.x, p = tid & (TSH - 1);
+    const bool lo = tid < TSH;
+    in.cfa = t.full(CFA)[(x + 1) * TS + tid];
+    in.w = (lo ? t.half(HVWT) : t.half(PMWT))[(x + 1) * TSH + p];
+    in.rb = (lo ? t.half(RBM) : t.half(RBP))[x * TSH + p];
+    in.d = t.full(lo ? VCD : HCD)[x * TS + 2 * p + gae(pattern, x)];
+    in.nyq = tid < TSH / 4 ? reinterpret_cast<const uint32_t*>(t.nyq2() + x * TSH)[tid] : 0u;
+}
+__device__ __forceinline__ void back_store(BackLds& s, int x, const BackIn& in) {
+    const int tid = threadIdx.x, p = tid & (TSH - 1);
+    const bool lo = tid < TSH;
+    s.cfa[ring(x + 1, R_LONG)][tid] = in.cfa;
+    (lo ? s.hvo : s.pmo)[ring(x + 1, R_SHORT)][p] = in.w;
+    (lo ? s.rbm : s.rbp)[ring(x, R_RB)][p] = in.rb;
+    (lo ? s.vcd : s.hcd)[ring(x, R_SHORT)][p] = in.d;
+    if (tid < TSH / 4) s.nyq[ring(x, R_SHORT)][tid] = in.nyq;
+}
+__device__ __forceinline__ bool nyq_at(const BackLds& s, int row, int p) {
+    return ((s.nyq[ring(row, R_SHORT)][p >> 2] >> (8 * (p & 3))) & 0xffu) != 0;
+}
+
+// nyquist.rs:159-201 and diagonal.rs:149-168 for pair p of row rr.
+__device__ __forceinline__ void back_chains(BackLds& s, int pattern, int rr, int p, float& hv_out,
+                                            float& pm_out) {
+    const int cc = 2 * p + gae(pattern, rr);
+    const int rc = ring(rr, R_LONG), rs = ring(rr, R_SHORT), rn = ring(rr + 1, R_SHORT);
+    const int dl = max((cc - 1) >> 1, 0), dr = min((cc + 1) >> 1, TSH - 1);
+    const int cl = min(max(cc, 1), TS - 2);
+    float hv = s.hvo[rs][p];
+    float d2h = 0.0f, d2v = 0.0f;
+    if (cc >= 8 && cc < TS - 8) {
+        float alt = (s.hv_prev[(cc - 1) >> 1] + s.hv_prev[(cc + 1) >> 1] + s.hvo[rn][dl] +
+                     s.hvo[rn][dr]) /
+                    4.0f;
+        if (fabsf(0.5f - hv) < fabsf(0.5f - alt)) hv = alt;
+        const float cfa = s.cfa[rc][cl];
+        float dg = intp(hv, s.vcd[rs][p], s.hcd[rs][p]);
+        float green = cfa + dg;
+        s.g0[rc][p] = dg;
+        s.grb[rc][p] = green;
+        if (nyq_at(s, rr, p)) {
+            d2h = sqr(green - 0.5f * (s.cfa[rc][cl - 1] + s.cfa[rc][cl + 1]));
+            d2v = sqr(green - 0.5f * (s.cfa[ring(rr - 1, R_LONG)][cl] + s.cfa[ring(rr + 1, R_LONG)][cl]));
+        }
+    }
+    const int rm = ring(rr, R_MID);
+    s.d2h[rm][p] = d2h;
+    s.d2v[rm][p] = d2v;
+    s.hv[rc][p] = hv;
+    hv_out = hv;
+    float pm = 0.0f;
+    if (rr >= 10 && rr < TS - 10) {
+        pm = s.pmo[rs][p];
+        if (cc >= 10 && cc < TS - 10) {
+            float alt = (s.pm_prev[(cc - 1) >> 1] + s.pm_prev[(cc + 1) >> 1] + s.pmo[rn][dl] +
+                         s.pmo[rn][dr]) /
+                        4.0f;
+            if (fabsf(0.5f - pm) < fabsf(0.5f - alt)) pm = alt;
+            s.rbint[rm][p] = 0.5f * (s.cfa[rc][cc] + s.rbm[ring(rr, R_RB)][p] * (1.0f - pm) +
+                                     s.rbp[ring(rr, R_RB)][p] * pm);
+        }
+        s.pm[rs][p] = pm;
+    }
+    pm_out = pm;
+}
+
+// nyquist.rs:204-236, diagonal.rs:193-247 and the G-B split of
+// chroma.rs:26-32, in that order, on pair p of row rr.
+__device__ __forceinline__ void back_refine(BackLds& s, int pattern, int rr, int p, bool has_area,
+                                            const Area& ar, float clip_pt) {
+    const int cc = 2 * p + gae(pattern, rr);
+    const int rc = ring(rr, R_LONG);
+    auto F = [&](int r, int c) { return s.cfa[ring(r, R_LONG)][c]; };
+    if (has_area && in_area(ar, pattern, rr, cc) && nyq_at(s, rr, p)) {
+        auto pl = [&](const float (*plane)[TSH], int dr, int dc) {
+            return plane[ring(rr + dr, R_MID)][(cc + dc) >> 1];
         };
-        float gvarh = average(t.half(D2H));
-        float gvarv = average(t.half(D2V));
-        float dg = (t.full(HCD)[i] * gvarv + t.full(VCD)[i] * gvarh) / (gvarv + gvarh);
-        dgrb0[h] = dg;
-        grb[h] = cfa[i] + dg;
+        auto average = [&](const float (*plane)[TSH]) {
+            return EPSSQ + (GQUINC[0] * pl(plane, 0, 0) +
+                            GQUINC[1] * (pl(plane, -1, -1) + pl(plane, -1, 1) + pl(plane, 1, -1) +
+                                         pl(plane, 1, 1)) +
+                            GQUINC[2] * (pl(plane, -2, 0) + pl(plane, 0, -2) + pl(plane, 0, 2) +
+                                         pl(plane, 2, 0)) +
+                            GQUINC[3] * (pl(plane, -2, -2) + pl(plane, -2, 2) + pl(plane, 2, -2) +
+                                         pl(plane, 2, 2)));
+        };
+        float gvarh = average(s.d2h);
+        float gvarv = average(s.d2v);
+        const int rs = ring(rr, R_SHORT);
+        float dg = (s.hcd[rs][p] * gvarv + s.vcd[rs][p] * gvarh) / (gvarv + gvarh);
+        s.g0[rc][p] = dg;
+        s.grb[rc][p] = F(rr, cc) + dg;
     }
     if (rr >= 12 && rr < TS - 12 && cc >= 12 && cc < TS - 12) {
-        const float* pm = t.half(PMWT);
-        const float* hv = t.half(HVWT);
-        if (!(fabsf(0.5f - pm[h]) < fabsf(0.5f - hv[h]))) {
-            const float* rb = t.half(RBINT);
-            auto F = [&](int r, int c) { return cfa[r * TS + c]; };
+        const float pm = s.pm[ring(rr, R_SHORT)][p], hv = s.hv[rc][p];
+        if (!(fabsf(0.5f - pm) < fabsf(0.5f - hv))) {
+            const int rm = ring(rr, R_MID);
+            const float centre = s.rbint[rm][p];
+            float gu = green_from_rb(F(rr - 1, cc), centre, s.rbint[ring(rr - 2, R_MID)][p]);
+            float gd = green_from_rb(F(rr + 1, cc), centre, s.rbint[ring(rr + 2, R_MID)][p]);
+            float gl = green_from_rb(F(rr, cc - 1), centre, s.rbint[rm][p - 1]);
+            float gr = green_from_rb(F(rr, cc + 1), centre, s.rbint[rm][p + 1]);
             const float w0u = vgrad(F, rr - 1, cc), w0d = vgrad(F, rr + 1, cc);
             const float w1l = hgrad(F, rr, cc - 1), w1r = hgrad(F, rr, cc + 1);
-            float centre = rb[h];
-            float gu = green_from_rb(cfa[i - TS], centre, rb[h - 2 * TSH]);
-            float gd = green_from_rb(cfa[i + TS], centre, rb[h + 2 * TSH]);
-            float gl = green_from_rb(cfa[i - 1], centre, rb[h - 1]);
-            float gr = green_from_rb(cfa[i + 1], centre, rb[h + 1]);
             float gintv = (w0u * gd + w0d * gu) / (w0d + w0u);
             float ginth = (w1l * gr + w1r * gl) / (w1l + w1r);
-            gintv = bound_towards(gintv, centre, cfa[i - TS], cfa[i + TS], clip_pt);
-            ginth = bound_towards(ginth, centre, cfa[i - 1], cfa[i + 1], clip_pt);
-            float green = ginth * (1.0f - hv[h]) + gintv * hv[h];
-            grb[h] = green;
-            dgrb0[h] = green - cfa[i];
+            gintv = bound_towards(gintv, centre, F(rr - 1, cc), F(rr + 1, cc), clip_pt);
+            ginth = bound_towards(ginth, centre, F(rr, cc - 1), F(rr, cc + 1), clip_pt);
+            float green = ginth * (1.0f - hv) + gintv * hv;
+            s.grb[rc][p] = green;
+            s.g0[rc][p] = green - F(rr, cc);
         }
     }
     // Split G - B out of the combined colour difference on blue rows.
     int ey, ex;
-    red_offset(a.pattern, ey, ex);
+    red_offset(pattern, ey, ex);
     if (rr >= 13 - ey && rr < TS - 12 && ((rr - (13 - ey)) & 1) == 0 && p >= ((13 - ex) >> 1) &&
         p < ((TS - 12) >> 1)) {
-        t.half(DGRB1)[h] = dgrb0[h];
-        dgrb0[h] = 0.0f;
+        s.g1[rc][p] = s.g0[rc][p];
+        s.g0[rc][p] = 0.0f;
     }
 }
 
-// ---------------------------------------------------------------- chroma
-// chroma.rs:34-96
-__global__ __launch_bounds__(256) void amz_chroma(AmazeArgs a) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
-    const int p = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
+// chroma.rs:34-96 on pair p of row rr.
+__device__ __forceinline__ void back_chroma(BackLds& s, int pattern, int rr, int p) {
     if (rr < 14 || rr >= TS - 14) return;
-    const int start = 14 + gae(a.pattern, rr);
-    const int cc = 2 * p + gae(a.pattern, rr);
+    const int start = 14 + gae(pattern, rr);
+    const int cc = 2 * p + gae(pattern, rr);
     if (cc < start || cc >= TS - 14) return;
-    const int plane = 1 - fc(a.pattern, rr, start) / 2;
-    float* pl = t.half(plane ? DGRB1 : DGRB0);
-    auto d = [&](int dr, int dc) { return pl[(rr + dr) * TSH + ((cc + dc) >> 1)]; };
+    const int plane = 1 - fc(pattern, rr, start) / 2;
+    float (*pl)[TSH] = plane ? s.g1 : s.g0;
+    auto d = [&](int dr, int dc) { return pl[ring(rr + dr, R_LONG)][(cc + dc) >> 1]; };
     float wtnw = 1.0f / (EPS + fabsf(d(-1, -1) - d(1, 1)) + fabsf(d(-1, -1) - d(-3, -3)) +
                          fabsf(d(1, 1) - d(-3, -3)));
     float wtne = 1.0f / (EPS + fabsf(d(-1, 1) - d(1, -1)) + fabsf(d(-1, 1) - d(-3, 3)) +
@@ -1092,58 +1080,125 @@ __global__ __launch_bounds__(256) void amz_chroma(AmazeArgs a) {
                    wtse * (1.325f * d(1, 1) - 0.175f * d(3, 3) - 0.075f * d(1, 3) -
                            0.075f * d(3, 1))) /
                   (wtnw + wtne + wtsw + wtse);
-    pl[rr * TSH + (cc >> 1)] = value;
+    pl[ring(rr, R_LONG)][cc >> 1] = value;
 }
 
-// ---------------------------------------------------------------- finish + write
-// chroma.rs:103-147 (finish_planes) fused with mod.rs:196-273 (write_tile).
-__global__ __launch_bounds__(256) void amz_finish(AmazeArgs a) {
-    const uint3 blk = xcd_block();
-    const Tile t = tile_of(a, blk.z);
-    const int cc = BORDER + blk.x * 64 + (threadIdx.x & 63);
-    const int rr = BORDER + blk.y * 4 + (threadIdx.x >> 6);
-    if (cc >= TS - BORDER || rr >= TS - BORDER) return;
+// chroma.rs:103-147 (finish_planes) with mod.rs:196-273 (write_tile) on the
+// two pixels of pair p of row rr, inside the tile core.
+__device__ __forceinline__ void back_finish(const BackLds& s, const AmazeArgs& a, const Tile& t,
+                                            int rr, int p) {
+    if (rr < BORDER || rr >= TS - BORDER) return;
     const int tx = t.index % a.tiles_x, ty = t.index / a.tiles_x;
-    const int gx = tx * VALID + cc - BORDER, gy = ty * VALID + rr - BORDER;
-    if (gx >= a.width || gy >= a.height) return;
-    const float* cfa = t.full(CFA);
-    const float* hv = t.half(HVWT);
-    const float* g_r = t.half(DGRB0);
-    const float* g_b = t.half(DGRB1);
-    const int i = rr * TS + cc;
-    const int ch = fc(a.pattern, rr, cc);
-    const float sample = cfa[i];
-    float green, red, blue;
-    if (ch == 1) {
-        green = sample;
-        const int r0 = rr * TSH;
-        float up = hv[r0 - TSH + (cc >> 1)], down = hv[r0 + TSH + (cc >> 1)];
-        float left = hv[r0 + ((cc - 1) >> 1)], right = hv[r0 + ((cc + 1) >> 1)];
-        float temp = 1.0f / (up + 2.0f - right - left + down);
-        float br = (up * g_r[r0 - TSH + (cc >> 1)] + (1.0f - right) * g_r[r0 + ((cc + 1) >> 1)] +
-                    (1.0f - left) * g_r[r0 + ((cc - 1) >> 1)] + down * g_r[r0 + TSH + (cc >> 1)]) *
-                   temp;
-        float bb = (up * g_b[r0 - TSH + (cc >> 1)] + (1.0f - right) * g_b[r0 + ((cc + 1) >> 1)] +
-                    (1.0f - left) * g_b[r0 + ((cc - 1) >> 1)] + down * g_b[r0 + TSH + (cc >> 1)]) *
-                   temp;
-        red = sel_max(green - br, 0.0f);
-        blue = sel_max(green - bb, 0.0f);
-    } else {
-        green = t.half(GRB)[i >> 1];
-        red = sel_max(green - g_r[i >> 1], 0.0f);
-        blue = sel_max(green - g_b[i >> 1], 0.0f);
+    const int gy = ty * VALID + rr - BORDER;
+    if (gy >= a.height) return;
+    const int ru = ring(rr - 1, R_LONG), rc = ring(rr, R_LONG), rd = ring(rr + 1, R_LONG);
+    const size_t n = (size_t)a.width * a.height;
+    bool finite = true;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int cc = 2 * p + e;
+        const int gx = tx * VALID + cc - BORDER;
+        if (cc < BORDER || cc >= TS - BORDER || gx >= a.width) continue;
+        const int ch = fc(a.pattern, rr, cc);
+        const float sample = s.cfa[rc][cc];
+        float green, red, blue;
+        if (ch == 1) {
+            green = sample;
+            const int h = cc >> 1, hl = (cc - 1) >> 1, hr = (cc + 1) >> 1;
+            float up = s.hv[ru][h], down = s.hv[rd][h];
+            float left = s.hv[rc][hl], right = s.hv[rc][hr];
+            float temp = 1.0f / (up + 2.0f - right - left + down);
+            float br = (up * s.g0[ru][h] + (1.0f - right) * s.g0[rc][hr] +
+                        (1.0f - left) * s.g0[rc][hl] + down * s.g0[rd][h]) *
+                       temp;
+            float bb = (up * s.g1[ru][h] + (1.0f - right) * s.g1[rc][hr] +
+                        (1.0f - left) * s.g1[rc][hl] + down * s.g1[rd][h]) *
+                       temp;
+            red = sel_max(green - br, 0.0f);
+            blue = sel_max(green - bb, 0.0f);
+        } else {
+            green = s.grb[rc][cc >> 1];
+            red = sel_max(green - s.g0[rc][cc >> 1], 0.0f);
+            blue = sel_max(green - s.g1[rc][cc >> 1], 0.0f);
+        }
+        float rgb[3] = {ch == 0 ? sample : red, ch == 1 ? sample : sel_max(green, 0.0f),
+                        ch == 2 ? sample : blue};
+        rgb[0] *= a.inverse[0];
+        rgb[1] *= a.inverse[1];
+        rgb[2] *= a.inverse[2];
+        const size_t o = (size_t)gy * a.width + gx;
+        a.out[o] = rgb[0];
+        a.out[n + o] = rgb[1];
+        a.out[2 * n + o] = rgb[2];
+        finite = finite && is_finite(rgb[0]) && is_finite(rgb[1]) && is_finite(rgb[2]);
     }
-    float rgb[3] = {ch == 0 ? sample : red, ch == 1 ? sample : sel_max(green, 0.0f),
-                    ch == 2 ? sample : blue};
-    rgb[0] *= a.inverse[0];
-    rgb[1] *= a.inverse[1];
-    rgb[2] *= a.inverse[2];
-    const size_t n = (size_t)a.width * a.height, o = (size_t)gy * a.width + gx;
-    a.out[o] = rgb[0];
-    a.out[n + o] = rgb[1];
-    a.out[2 * n + o] = rgb[2];
-    if (!(is_finite(rgb[0]) && is_finite(rgb[1]) && is_finite(rgb[2])))
-        atomicOr(&a.stats->nonfinite, 2u);
+    if (!finite) atomicOr(&a.stats->nonfinite, 2u);
+}
+
+__global__ __launch_bounds__(BACK_THREADS) void amz_back(AmazeArgs a, float clip_pt) {
+    static_assert(BACK_THREADS == 4 * TSH && TS == 2 * TSH, "four groups of a thread per pair");
+    const uint3 blk = xcd_block();
+    const Tile t = tile_of(a, blk.x);
+    __shared__ BackLds s;
+    const int tid = threadIdx.x, p = tid & (TSH - 1);
+    const int group = __builtin_amdgcn_readfirstlane(tid / TSH);
+    const bool loader = tid < TS;  // groups 0 and 1 bring the inputs in
+    Area ar;
+    const bool has_area = tile_area(a, blk.x, ar);
+    // Rows before the pipeline: CFA rows 0 .. 8, original hvwt / pmwt row 8,
+    // step 8's inputs; d2h and d2v read as zero until the chains write them.
+    if (loader) {
+        for (int r = 0; r <= 8; ++r) s.cfa[ring(r, R_LONG)][tid] = t.full(CFA)[r * TS + tid];
+        (group == 0 ? s.hvo : s.pmo)[ring(8, R_SHORT)][p] =
+            (group == 0 ? t.half(HVWT) : t.half(PMWT))[8 * TSH + p];
+        for (int r = 0; r < R_MID; ++r) (group == 0 ? s.d2h : s.d2v)[r][p] = 0.0f;
+        BackIn in;
+        back_load(t, a.pattern, 8, in);
+        back_store(s, 8, in);
+    }
+    if (group == 0) {
+        s.hv_prev[p] = t.half(HVWT)[7 * TSH + p];
+        s.pm_prev[p] = t.half(PMWT)[9 * TSH + p];
+    }
+    BackIn buf[BACK_AHEAD];
+    if (loader) {
+#pragma unroll
+        for (int k = 1; k < BACK_AHEAD; ++k) back_load(t, a.pattern, 8 + k, buf[k]);
+    }
+    __syncthreads();
+    // Phase 1 of step st (the chains only while st is a chain row).
+    auto phase1 = [&](int st, float& hv, float& pm) {
+        if (group == 0) {
+            if (st < TS - 8) back_chains(s, a.pattern, st, p, hv, pm);
+        } else if (group == 1) {
+            back_refine(s, a.pattern, st - LAG_REFINE, p, has_area, ar, clip_pt);
+        } else if (group == 2) {
+            back_chroma(s, a.pattern, st - LAG_CHROMA, p);
+        } else {
+            back_finish(s, a, t, st - LAG_FINISH, p);
+        }
+    };
+    for (int s0 = 8; s0 < TS - 8; s0 += BACK_AHEAD) {
+#pragma unroll
+        for (int k = 0; k < BACK_AHEAD; ++k) {
+            const int st = s0 + k;
+            // Inputs of step st + BACK_AHEAD (the last step's, re-read, past
+            // the end) into the register slot step st's inputs left.
+            if (loader) back_load(t, a.pattern, min(st + BACK_AHEAD, TS - 9), buf[k]);
+            float hv = 0.0f, pm = 0.0f;
+            phase1(st, hv, pm);
+            __syncthreads();
+            if (group == 0) {
+                s.hv_prev[p] = hv;
+                if (st >= 10 && st < TS - 10) s.pm_prev[p] = pm;
+            }
+            if (loader && st + 1 < TS - 8) back_store(s, st + 1, buf[(k + 1) % BACK_AHEAD]);
+            __syncthreads();
+        }
+    }
+    // One more step drains the later stages (the last chroma and finish rows).
+    float hv = 0.0f, pm = 0.0f;
+    phase1(TS - 8, hv, pm);
 }
 
 }  // namespace
@@ -1173,13 +1228,11 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
               hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_chains", 3 * f / 2 + 6 * h + 6 * h,
-              hipLaunchKernelGGL(amz_chains, dim3(tiles), dim3(TSH), 0, s, a));
-    RHD_TIMED(prof, s, "amz_refine", f / 2 + 5 * h + 3 * h,
-              hipLaunchKernelGGL(amz_refine, pair_grid, blk, 0, s, a, a.clip_point));
-    RHD_TIMED(prof, s, "amz_chroma", 2 * h + h, hipLaunchKernelGGL(amz_chroma, pair_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_finish", 12ull * a.width * a.height * tiles / (a.tiles_x * a.tiles_y) + f / 2 + 4 * h,
-              hipLaunchKernelGGL(amz_finish, dim3(VALID / 64 + 1, VALID / 4, tiles), blk, 0, s, a));
+    // The back end reads CFA, vcd, hcd (full planes), hvwt, pmwt, rbm, rbp
+    // and the Nyquist flags, and writes the RGB core of the tile.
+    RHD_TIMED(prof, s, "amz_back",
+              3 * f + 4 * h + h / 4 + 12ull * a.width * a.height * tiles / (a.tiles_x * a.tiles_y),
+              hipLaunchKernelGGL(amz_back, dim3(tiles), dim3(BACK_THREADS), 0, s, a, a.clip_point));
     return hipGetLastError();
 }
 

@@ -65,8 +65,10 @@ struct Layout {
     unsigned* region_first;
     unsigned* region_last;
     float4* corrections;           // per (group, slot): offset, ratio, valid, has_ratio
-    unsigned* site_counts;         // per opp_sites block: its five statistics
-    unsigned* scalars;             // [0] region count, [1] ring count, [2] any wanted
+    unsigned* site_counts;         // per opp_sites workgroup: its five statistics
+    unsigned* site_list;           // (sites block, channel) pairs with a suspected cell
+    unsigned* scalars;             // [0] region count, [1] ring count, [2] any wanted,
+                                   // [3] site_list length, [4] next site_list item
     const uint32_t* log2_bitmap;   // libm_f32.hpp exception bitmap
 };
 
@@ -472,6 +474,8 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L, DeviceStats
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         L.scalars[2] = 0;
+        L.scalars[3] = 0;
+        L.scalars[4] = 0;
         stats->regions = L.scalars[0];
     }
 }
@@ -931,10 +935,47 @@ struct Site {
     }
 };
 
+// The (sites block, channel) pairs with a suspected cell of the channel,
+// appended to site_list in any order. A wavefront looks at LIST_PER_WAVE
+// sites blocks in turn (SX == 64: a lane per cell column); the workgroup
+// gathers its pairs in LDS and reserves their places in the list with one
+// atomic. The site estimates are independent of one another, so opp_sites
+// may take the pairs in any order.
+constexpr int LIST_PER_WAVE = 4, LIST_PER_GROUP = LIST_PER_WAVE * (THREADS / 64);
+__global__ __launch_bounds__(THREADS) void opp_site_list(Layout L) {
+    static_assert(SX == 64, "a lane per cell column");
+    __shared__ unsigned s_ids[3 * LIST_PER_GROUP];
+    __shared__ unsigned s_n, s_base;
+    const int gx = (L.cw + SX - 1) / SX, gy = (L.ch + SY - 1) / SY;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int i = 0; i < LIST_PER_WAVE; ++i) {
+        const int b = blockIdx.x * LIST_PER_GROUP + wave * LIST_PER_WAVE + i;
+        if (b >= gx * gy) break;
+        const int cx = (b % gx) * SX + lane, by0 = (b / gx) * SY;
+        unsigned suspected = 0;
+#pragma unroll
+        for (int r = 0; r < SY; ++r)
+            if (cx < L.cw && by0 + r < L.ch)
+                suspected |= flags_of(L.cell[(size_t)(by0 + r) * L.cw + cx]) >> 3;
+        for (int c = 0; c < 3; ++c)
+            if (__ballot((suspected >> c) & 1u) && lane == 0)
+                s_ids[atomicAdd(&s_n, 1u)] = (unsigned)(b + c * gx * gy);
+    }
+    __syncthreads();
+    const unsigned n = s_n;
+    if (threadIdx.x == 0 && n) s_base = atomicAdd(L.scalars + 3, n);
+    __syncthreads();
+    for (unsigned k = threadIdx.x; k < n; k += THREADS) L.site_list[s_base + k] = s_ids[k];
+}
+
 // opposed.rs:97-162: one (cell, channel) with a suspected site of that
 // channel at a time; the estimate is computed once and applied to each such
-// site. Blocks without a suspected cell of their channel
-// leave before staging anything.
+// site. A resident grid of workgroups takes the listed (sites block,
+// channel) pairs from a shared counter until none is left, so the blocks
+// without a suspected cell cost nothing and the long searches of large
+// clipped areas spread over every workgroup.
 __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
                                                      float l0, float l1, float l2, int has_ceiling,
                                                      float c0, float c1, float c2,
@@ -945,26 +986,25 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     __shared__ int s_keys[MAX_LOCAL * THREADS];
     __shared__ unsigned short s_task[SX * SY];
     __shared__ int s_ntask;
+    __shared__ unsigned s_item;
     const float lv[3] = {l0, l1, l2};
     const float ceiling[3] = {c0, c1, c2};
-    const uint3 blk = xcd_block();
-    const int channel = blk.z;
-    const int bx0 = blk.x * SX, by0 = blk.y * SY;
+    const int gx = (L.cw + SX - 1) / SX, gy = (L.ch + SY - 1) / SY;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     constexpr int ROWS_PER_THREAD = SY / (THREADS / 64);
-    bool mine = false;
-#pragma unroll
-    for (int q = 0; q < ROWS_PER_THREAD; ++q) {
-        const int cx = bx0 + tx, cy = by0 + ty + 4 * q;
-        if (cx < L.cw && cy < L.ch)
-            mine |= (flags_of(L.cell[(size_t)cy * L.cw + cx]) >> (channel + 3)) & 1u;
-    }
-    unsigned* counts = L.site_counts +
-                       5 * (size_t)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-    if (!__syncthreads_or(mine)) {
-        if (threadIdx.x < 5) counts[threadIdx.x] = 0;
-        return;
-    }
+    const unsigned listed = L.scalars[3];
+    unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
+    for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(L.scalars + 4, 1u);
+    __syncthreads();
+    const unsigned work = s_item;
+    // Every thread reads the same item, so the whole workgroup leaves
+    // together once the list is exhausted.
+    if (work >= listed) break;
+    const unsigned id = L.site_list[work];
+    const int channel = (int)(id / (unsigned)(gx * gy));
+    const int rest = (int)(id % (unsigned)(gx * gy));
+    const int bx0 = (rest % gx) * SX, by0 = (rest / gx) * SY;
     const size_t plane = (size_t)channel * L.cells;
     for (int k = threadIdx.x; k < WY * WX; k += THREADS) {
         const int x = bx0 - SR + k % WX, y = by0 - SR + k / WX;
@@ -985,7 +1025,6 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     // Cells whose suspected sites need an estimate (a supporting channel)
     // are listed in LDS and taken densely below, so the ring searches run in
     // full wavefronts; the others fall back at once.
-    unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
     for (int pass = 0; pass < 2; ++pass) {
     const int items = pass == 0 ? ROWS_PER_THREAD : (s_ntask + THREADS - 1 - (int)threadIdx.x) / THREADS;
     for (int item = 0; item < items; ++item) {
@@ -1051,8 +1090,9 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     }
     __syncthreads();
     }
-    // Per-block totals go to the block's own slot (no shared counters);
-    // opp_site_stats adds them up.
+    }
+    // Per-workgroup totals go to the workgroup's own slot (no shared
+    // counters); opp_site_stats adds them up.
     __shared__ unsigned s_part[THREADS / 64][5];
     unsigned v[5] = {reconstructed, changed, fallback, unsupported, neutralised};
     for (int k = 0; k < 5; ++k) {
@@ -1063,7 +1103,7 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     if (threadIdx.x < 5) {
         unsigned t = 0;
         for (int w = 0; w < THREADS / 64; ++w) t += s_part[w][threadIdx.x];
-        counts[threadIdx.x] = t;
+        L.site_counts[5 * (size_t)blockIdx.x + threadIdx.x] = t;
     }
 }
 
@@ -1082,6 +1122,10 @@ __global__ __launch_bounds__(THREADS) void opp_site_stats(Layout L, unsigned blo
 size_t site_blocks(const Layout& L) {
     return (size_t)((L.cw + SX - 1) / SX) * ((L.ch + SY - 1) / SY) * 3;
 }
+// Resident opp_sites workgroups: three per CU of the MI355X (LDS-bound),
+// never more than there are (sites block, channel) pairs.
+constexpr int SITE_WORKERS = 3 * 256;
+int site_workers(const Layout& L) { return (int)std::min<size_t>(SITE_WORKERS, site_blocks(L)); }
 
 Layout layout_of(const OpposedArgs& a) {
     Layout L{};
@@ -1117,6 +1161,7 @@ Layout layout_of(const OpposedArgs& a) {
     L.region_last = (unsigned*)take(cap * 4);
     L.corrections = (float4*)take(cap * SLOTS * sizeof(float4));
     L.site_counts = (unsigned*)take(site_blocks(L) * 5 * sizeof(unsigned));
+    L.site_list = (unsigned*)take(site_blocks(L) * sizeof(unsigned));
     L.scalars = (unsigned*)take(64);
     L.log2_bitmap = a.log2_bitmap;
     return L;
@@ -1180,14 +1225,17 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
     RHD_TIMED(prof, s, "opp_sums", 0,
               hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(SUM_THREADS), 0, s, L));
+    const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
+    RHD_TIMED(prof, s, "opp_site_list", 16 * cells,
+              hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),
+                                 dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_sites", 8 * px + 16 * cells,
-              hipLaunchKernelGGL(opp_sites, dim3((L.cw + SX - 1) / SX, (L.ch + SY - 1) / SY, 3),
-                                 dim3(THREADS), 0, s, a.mosaic, L, a.pattern, l0, l1, l2,
-                                 a.has_ceiling, a.ceiling[0], a.ceiling[1], a.ceiling[2],
-                                 a.stats));
-    RHD_TIMED(prof, s, "opp_site_stats", 20 * site_blocks(L),
+              hipLaunchKernelGGL(opp_sites, dim3(site_workers(L)), dim3(THREADS), 0, s, a.mosaic, L,
+                                 a.pattern, l0, l1, l2, a.has_ceiling, a.ceiling[0], a.ceiling[1],
+                                 a.ceiling[2], a.stats));
+    RHD_TIMED(prof, s, "opp_site_stats", 20 * site_workers(L),
               hipLaunchKernelGGL(opp_site_stats, dim3(64), dim3(THREADS), 0, s, L,
-                                 (unsigned)site_blocks(L), a.stats));
+                                 (unsigned)site_workers(L), a.stats));
     return hipGetLastError();
 }
 
