@@ -49,6 +49,8 @@ constexpr unsigned MIN_BORDER_CELLS = 3;
 struct Layout {
     int w, h, cw, ch, cells, cap;  // cap: region-count bound (+1)
     float4* cell;                  // means (x, y, z), flags as bits in w
+    uint8_t* flag8;                // the same flags, one byte per cell, for the
+                                   // kernels that read nothing else of a cell
     float* pair_ref;               // [3][cells]: cube-root reference of the two channels
                                    // other than the plane's index, NaN when undefined
     float* pair_lr;                // [3][cells]: log2 ratio of those two channels, or NaN
@@ -206,6 +208,7 @@ __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m
         }
     const float4 cell = make_float4(means[0], means[1], means[2], __uint_as_float(flags));
     L.cell[i] = cell;
+    L.flag8[i] = (uint8_t)flags;
     float roots[3];
     for (int c = 0; c < 3; ++c) roots[c] = positive_finite(means[c]) ? cbrt_f32(means[c]) : 0.0f;
     for (int e = 0; e < 3; ++e) {
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(THREADS) void opp_label_tiles(Layout L) {
         const int x = x0 + k % LHX - 1, y = y0 + k / LHX - 1;
         bool clipped = false;
         if (x >= 0 && y >= 0 && x < L.cw && y < L.ch)
-            clipped = (flags_of(L.cell[(size_t)y * L.cw + x]) >> 3) != 0;
+            clipped = (L.flag8[(size_t)y * L.cw + x] >> 3) != 0;
         s_clip[k] = clipped;
         any |= clipped;
     }
@@ -367,7 +370,7 @@ __device__ __forceinline__ unsigned predicate(const Layout& L, int mode, int i) 
     if (i >= L.cells) return 0;
     if (mode == ROOTS) return L.label[i] == (unsigned)i ? 1u : 0u;
     unsigned lab = L.label[i];
-    return lab != 0 && (flags_of(L.cell[i]) >> 3) == 0 ? 1u : 0u;
+    return lab != 0 && (L.flag8[i] >> 3) == 0 ? 1u : 0u;
 }
 
 // Block-wide exclusive scan of one value per thread; returns the block total.
@@ -482,57 +485,76 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L, DeviceStats
 
 // regions.rs:44-55 (clipped channels per region) and region_correction.rs:
 // 125-187 (WantedSlots::scan).
-// Grid-stride: each workgroup walks REGION_STRIDE blocks of cells, so the
-// short per-cell work is not dominated by wavefront launches.
+// Each workgroup walks REGION_STRIDE consecutive blocks of cells; a
+// wavefront's successive slices lie along the same image rows, so they mostly
+// belong to the region it saw last. The wavefront remembers the bits it has
+// already set for that region and issues an atomic only for new ones, without
+// reading the region's words back; the next slice's cells load while the
+// current one is processed.
 constexpr int REGION_STRIDE = 32;
 __global__ __launch_bounds__(THREADS) void opp_regions(Layout L, float l0, float l1, float l2) {
     const float lv[3] = {l0, l1, l2};
     const int lane = threadIdx.x & 63;
     unsigned any = 0;
-    for (int i = blockIdx.x * THREADS + threadIdx.x; i - (int)threadIdx.x < L.cells;
-         i += gridDim.x * THREADS) {
-    unsigned lab = 0, clipped = 0, bits = 0;
-    bool has = false;
-    if (i < L.cells) {
-        lab = L.label[i];
-        float4 cell = L.cell[i];
-        unsigned f = flags_of(cell);
-        clipped = f >> 3;
-        if (lab != 0 && lab < (unsigned)L.cap && clipped != 0) {
-            has = true;
-            for (int c = 0; c < 3; ++c) {
-                if (!(clipped & (1u << c))) continue;
-                bool has_clean = false;
-                for (int o = 0; o < 3; ++o)
-                    if (o != c && usable(f, o) && clean(f, o)) has_clean = true;
-                if (!has_clean) continue;
-                int p = pattern_position(c, support_mask(cell, c, lv));
-                if (p >= 0) bits |= 1u << (c * 3 + p);
+    unsigned seen_label = 0, seen_clipped = 0, seen_wanted = 0;  // wavefront-uniform
+    const int begin = blockIdx.x * REGION_STRIDE * THREADS;
+    const int end = min(begin + REGION_STRIDE * THREADS, L.cells);
+    int i = begin + (int)threadIdx.x;
+    unsigned next_lab = i < end ? L.label[i] : 0u;
+    float4 next_cell = i < end ? L.cell[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (; i - (int)threadIdx.x < end; i += THREADS) {
+        const unsigned lab = next_lab;
+        const float4 cell = next_cell;
+        const int j = i + THREADS;
+        if (j < end) {
+            next_lab = L.label[j];
+            next_cell = L.cell[j];
+        }
+        unsigned clipped = 0, bits = 0;
+        bool has = false;
+        if (i < end) {
+            const unsigned f = flags_of(cell);
+            clipped = f >> 3;
+            if (lab != 0 && lab < (unsigned)L.cap && clipped != 0) {
+                has = true;
+                for (int c = 0; c < 3; ++c) {
+                    if (!(clipped & (1u << c))) continue;
+                    bool has_clean = false;
+                    for (int o = 0; o < 3; ++o)
+                        if (o != c && usable(f, o) && clean(f, o)) has_clean = true;
+                    if (!has_clean) continue;
+                    int p = pattern_position(c, support_mask(cell, c, lv));
+                    if (p >= 0) bits |= 1u << (c * 3 + p);
+                }
             }
         }
-    }
-    // Neighbouring cells share their region, so OR the bits per label across
-    // the wavefront and let one lane per distinct label touch memory; skip the
-    // atomic when every bit is already set.
-    unsigned long long active = __ballot(has);
-    while (active) {
-        const int leader = __ffsll((long long)active) - 1;
-        const unsigned target = __shfl(lab, leader, 64);
-        const bool mine = has && lab == target;
-        unsigned vc = mine ? clipped : 0u, vw = mine ? bits : 0u;
-        for (int off = 32; off > 0; off >>= 1) {
-            vc |= __shfl_xor(vc, off, 64);
-            vw |= __shfl_xor(vw, off, 64);
+        // OR the bits per label across the wavefront; one lane per distinct
+        // label sets the bits this wavefront has not set for it yet.
+        unsigned long long active = __ballot(has);
+        while (active) {
+            const int leader = __ffsll((long long)active) - 1;
+            const unsigned target = __shfl(lab, leader, 64);
+            const bool mine = has && lab == target;
+            unsigned vc = mine ? clipped : 0u, vw = mine ? bits : 0u;
+            for (int off = 32; off > 0; off >>= 1) {
+                vc |= __shfl_xor(vc, off, 64);
+                vw |= __shfl_xor(vw, off, 64);
+            }
+            if (target != seen_label) {
+                seen_label = target;
+                seen_clipped = 0;
+                seen_wanted = 0;
+            }
+            const unsigned new_c = vc & ~seen_clipped, new_w = vw & ~seen_wanted;
+            if (lane == leader) {
+                if (new_c) atomicOr(L.region_clipped + target, new_c);
+                if (new_w) atomicOr(L.region_wanted + target, new_w);
+            }
+            seen_clipped |= vc;
+            seen_wanted |= vw;
+            any |= vw;
+            active &= ~__ballot(mine);
         }
-        if (lane == leader) {
-            if ((L.region_clipped[target] | vc) != L.region_clipped[target])
-                atomicOr(L.region_clipped + target, vc);
-            if (vw && (L.region_wanted[target] | vw) != L.region_wanted[target])
-                atomicOr(L.region_wanted + target, vw);
-        }
-        any |= vw;
-        active &= ~__ballot(mine);
-    }
     }
     if (lane == 0 && any && (L.scalars[2] | any) != L.scalars[2]) atomicOr(L.scalars + 2, any);
 }
@@ -958,7 +980,7 @@ __global__ __launch_bounds__(THREADS) void opp_site_list(Layout L) {
 #pragma unroll
         for (int r = 0; r < SY; ++r)
             if (cx < L.cw && by0 + r < L.ch)
-                suspected |= flags_of(L.cell[(size_t)(by0 + r) * L.cw + cx]) >> 3;
+                suspected |= (unsigned)L.flag8[(size_t)(by0 + r) * L.cw + cx] >> 3;
         for (int c = 0; c < 3; ++c)
             if (__ballot((suspected >> c) & 1u) && lane == 0)
                 s_ids[atomicAdd(&s_n, 1u)] = (unsigned)(b + c * gx * gy);
@@ -1145,6 +1167,7 @@ Layout layout_of(const OpposedArgs& a) {
     const size_t cells = (size_t)L.cells, cap = (size_t)L.cap;
     const size_t scan_blocks = (cells + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
     L.cell = (float4*)take(cells * sizeof(float4));
+    L.flag8 = (uint8_t*)take(cells);
     L.pair_ref = (float*)take(cells * 3 * 4);
     L.pair_lr = (float*)take(cells * 3 * 4);
     L.label = (unsigned*)take(cells * 4);
@@ -1190,10 +1213,10 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     const uint64_t px = (uint64_t)L.w * L.h, cells = (uint64_t)L.cells;
     // Algorithmic bytes per launch. The ring-entry kernels (contrib, sums)
     // depend on the ring size, which only the device knows, and report 0.
-    RHD_TIMED(prof, s, "opp_cells", 4 * px + 16 * cells + 24 * cells,
+    RHD_TIMED(prof, s, "opp_cells", 4 * px + 17 * cells + 24 * cells,
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
-    RHD_TIMED(prof, s, "opp_label_tiles", 16 * cells + 4 * cells,
+    RHD_TIMED(prof, s, "opp_label_tiles", cells + 4 * cells,
               hipLaunchKernelGGL(opp_label_tiles, dim3(tiles_x, tiles_y), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_label_borders", 20ull * border_cells,
               hipLaunchKernelGGL(opp_label_borders, dim3((border_cells + THREADS - 1) / THREADS),
