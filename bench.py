@@ -72,6 +72,10 @@ def parse():
                    help="print per-kernel HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE "
                         "and WRITE_SIZE counter CSVs of this bench and of --calibrate-copy, "
                         "scaled by the copy kernel's known bytes, and exit (no GPU needed)")
+    p.add_argument("--counter-summary", metavar="COUNTER_CSV",
+                   help="print per-kernel means of a rocprofv3 --pmc VALUBusy LdsUtil TA_BUSY_avr "
+                        "MemUnitStalled SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_WAIT_INST_ANY "
+                        "SQ_WAVE_CYCLES counter CSV of this bench and exit (no GPU needed)")
     return p.parse_args()
 
 
@@ -130,6 +134,20 @@ def traffic_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
         fb = f * 1024 * sf
         wb = write.get(k, (0, 0.0))[1] * 1024 * sw
         print(f"{k:28s} {n:10d} {fb:14.0f} {wb:14.0f} {fb + wb:14.0f}")
+
+
+def counter_summary(path):
+    cols = ["VALUBusy", "LdsUtil", "TA_BUSY_avr", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD"]
+    means = {c: counter_means(path, c) for c in cols + ["SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"]}
+    kernels = sorted(means["SQ_WAVE_CYCLES"], key=lambda k: -means["SQ_WAVE_CYCLES"][k][1])
+    print(f"{'kernel':26s} {'VALUBusy':>9s} {'LdsUtil':>8s} {'TA_busy':>10s} {'VALU_insts':>12s} "
+          f"{'VMEM_rd':>10s} {'wait/wave':>9s}")
+    for k in kernels:
+        v = [means[c].get(k, (0, float("nan")))[1] for c in cols]
+        wait = means["SQ_WAIT_INST_ANY"].get(k, (0, 0.0))[1]
+        cycles = means["SQ_WAVE_CYCLES"][k][1]
+        print(f"{k:26s} {v[0]:9.1f} {v[1]:8.1f} {v[2]:10.3g} {v[3]:12.3g} {v[4]:10.3g} "
+              f"{(wait / cycles if cycles else 0.0):9.2f}")
 
 
 def measured_traffic(kernel):
@@ -280,6 +298,9 @@ def main():
         return
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary)
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
         return
     if args.batch:
         batch_bench(args)
