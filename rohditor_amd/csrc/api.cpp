@@ -30,8 +30,11 @@ void context_unref(rhd_context* ctx) {
         if (!L.open) continue;
         for (auto& ev : L.ev)
             if (ev) (void)hipEventDestroy(ev);
+        if (L.fork) (void)hipEventDestroy(L.fork);
+        if (L.join) (void)hipEventDestroy(L.join);
         if (L.d_stats) (void)hipFree(L.d_stats);
         if (L.h_stats) (void)hipHostFree(L.h_stats);
+        if (L.aux) (void)hipStreamDestroy(L.aux);
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     for (const PooledBlock& b : ctx->pool) (void)hipFree(b.ptr);
@@ -256,6 +259,9 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out) {
     if (!L.open) {
         L.index = i;
         bool ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&L.aux, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&L.join, hipEventDisableTiming) == hipSuccess &&
                   hipMalloc(&L.d_stats, sizeof(DeviceStats)) == hipSuccess &&
                   hipHostMalloc((void**)&L.h_stats, sizeof(DeviceStats), hipHostMallocDefault) ==
                       hipSuccess;
@@ -263,8 +269,11 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out) {
         if (!ok) {
             for (auto& ev : L.ev)
                 if (ev) (void)hipEventDestroy(ev);
+            if (L.fork) (void)hipEventDestroy(L.fork);
+            if (L.join) (void)hipEventDestroy(L.join);
             if (L.d_stats) (void)hipFree(L.d_stats);
             if (L.h_stats) (void)hipHostFree(L.h_stats);
+            if (L.aux) (void)hipStreamDestroy(L.aux);
             if (L.stream) (void)hipStreamDestroy(L.stream);
             L = Lane{};
             ctx->last_error = "could not create a stream, events or statistics for a lane";
@@ -533,6 +542,9 @@ rhd_status run_opposed(rhd_context* ctx, Lane& L, rhd_plane* plane, const rhd_se
     a.stats = L.d_stats;
     a.log2_bitmap = libm_log2_bitmap(ctx->device);
     if (!a.log2_bitmap) return fail(ctx, RHD_SYNCHRONIZATION, "libm tables are not set up");
+    a.aux = L.aux;
+    a.fork = L.fork;
+    a.join = L.join;
     hipError_t e = launch_opposed(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "opposed");
     return RHD_OK;

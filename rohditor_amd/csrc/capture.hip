@@ -505,17 +505,15 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
     RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.mask, a.mask, a.tmp, 0));
     RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.guide, a.tmp, a.mask, 1));
-    hipError_t e = hipMemcpyAsync(a.est_a, a.guide, (size_t)n * sizeof(float),
-                                  hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
-    float* cur = a.est_a;
-    float* next = a.est_b;
+    // The estimate starts as the guide (sharpening.rs:219): the first
+    // iteration reads the guide itself, the later ones alternate between the
+    // two estimate planes.
+    const float* cur = a.guide;
     for (int it = 0; it < a.iterations; ++it) {
+        float* out = (it & 1) ? a.est_b : a.est_a;
         // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
-        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, next));
-        float* t = cur;
-        cur = next;
-        next = t;
+        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, out));
+        cur = out;
     }
     RHD_TIMED(prof, s, "cap_apply", 36 * px,
               hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n, a.amount,

@@ -69,6 +69,8 @@ struct Layout {
     float4* corrections;           // per (group, slot): offset, ratio, valid, has_ratio
     unsigned* site_counts;         // per opp_sites workgroup: its five statistics
     unsigned* site_list;           // (sites block, channel) pairs with a suspected cell
+    float2* site_value;            // per (channel, cell): target reference, local median
+    uint8_t* site_code;            // per (channel, cell): LOCAL_REFERENCE | ring radius
     unsigned* scalars;             // [0] region count, [1] ring count, [2] any wanted,
                                    // [3] site_list length, [4] next site_list item
     const uint32_t* log2_bitmap;   // libm_f32.hpp exception bitmap
@@ -183,6 +185,11 @@ __device__ __forceinline__ int pattern_position(int channel, unsigned support) {
 __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m, Layout L,
                                                      int pattern, float l0, float l1, float l2) {
     int i = blockIdx.x * THREADS + threadIdx.x;
+    // The site list's length and next item (opp_site_list, opp_sites_local).
+    if (i == 0) {
+        L.scalars[3] = 0;
+        L.scalars[4] = 0;
+    }
     if (i >= L.cells) return;
     const float lv[3] = {l0, l1, l2};
     int cx = i % L.cw, cy = i / L.cw;
@@ -477,8 +484,6 @@ __global__ __launch_bounds__(THREADS) void opp_region_init(Layout L, DeviceStats
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         L.scalars[2] = 0;
-        L.scalars[3] = 0;
-        L.scalars[4] = 0;
         stats->regions = L.scalars[0];
     }
 }
@@ -841,11 +846,15 @@ constexpr int WX = SX + 2 * SR, WY = SY + 2 * SR;
 
 // The estimate for one (cell, channel): every suspected site of that channel
 // in the cell gets the same one (opposed.rs:243-278 reads only cell data).
+// Its parts that need no region data (the target reference and the local
+// ring median, opposed.rs:282-327) are formed by opp_sites_local, which
+// depends only on the cells and so runs on a second stream beside the
+// labelling and the region sums; opp_sites combines them with the region
+// correction once the sums are done.
 // Cell data comes from the block's LDS window, where cells outside the image
 // carry no flags, so they fail every candidate test as the reference's
 // image-clamped rings skip them.
-struct Site {
-    const Layout* L;
+struct SiteLocal {
     const float* lv;
     const float4* cells;  // LDS window
     const float* pref;    // LDS window: pair reference for the target channel
@@ -856,20 +865,6 @@ struct Site {
     bool ht;
     float tr;
     int* keys;  // this thread's LDS column, stride THREADS
-
-    __device__ bool lookup(unsigned label, float& offset, bool& has_ratio, float& ratio) const {
-        int p = pattern_position(target, support);
-        if (p < 0) return false;
-        int off = target * 3 + p;
-        float4 v = label < (unsigned)L->cap ? L->corrections[(size_t)label * SLOTS + off]
-                                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (v.z == 0.0f) v = L->corrections[off];
-        if (v.z == 0.0f) return false;
-        offset = v.x;
-        has_ratio = two;
-        ratio = v.y;
-        return true;
-    }
 
     // One candidate of the local search (opposed.rs:295-312), at window index j.
     __device__ void candidate(int j, int& count) const {
@@ -927,42 +922,19 @@ struct Site {
         }
         return false;
     }
-
-    // opposed.rs:243-278
-    __device__ bool estimate(const float4& cell, int centre, unsigned label, float& out) const {
-        float target_reference;
-        if (!reference(cell, support, pref[centre], target_reference)) return false;
-        float local = 0.0f;
-        int radius = 0;
-        const bool has_local = local_correction(centre, local, radius);
-        float region = 0.0f, br = 0.0f;
-        bool bhr = false;
-        const bool has_region =
-            lookup(label, region, bhr, br) && chromaticity_agrees(support, ht, tr, bhr, br);
-        float corr;
-        if (has_local && has_region) {
-            float weight = (float)(3 + 1 - radius) / 3.0f;
-            corr = weight * local + (1.0f - weight) * region;
-        } else if (has_local) {
-            corr = local;
-        } else if (has_region) {
-            corr = region;
-        } else {
-            return false;
-        }
-        float e = target_reference + corr;
-        if (!(is_finite(e) && e > 0.0f)) return false;
-        out = e;
-        return true;
-    }
 };
+
+// Per (channel, cell) result of the local part: the target reference and the
+// local median, and a byte with bit 7 set when the reference exists and the
+// median's ring radius (0: no local median) in bits 0-2.
+constexpr uint8_t LOCAL_REFERENCE = 0x80;
 
 // The (sites block, channel) pairs with a suspected cell of the channel,
 // appended to site_list in any order. A wavefront looks at LIST_PER_WAVE
 // sites blocks in turn (SX == 64: a lane per cell column); the workgroup
 // gathers its pairs in LDS and reserves their places in the list with one
-// atomic. The site estimates are independent of one another, so opp_sites
-// may take the pairs in any order.
+// atomic. The local parts are independent of one another, so
+// opp_sites_local may take the pairs in any order.
 constexpr int LIST_PER_WAVE = 4, LIST_PER_GROUP = LIST_PER_WAVE * (THREADS / 64);
 __global__ __launch_bounds__(THREADS) void opp_site_list(Layout L) {
     static_assert(SX == 64, "a lane per cell column");
@@ -992,16 +964,15 @@ __global__ __launch_bounds__(THREADS) void opp_site_list(Layout L) {
     for (unsigned k = threadIdx.x; k < n; k += THREADS) L.site_list[s_base + k] = s_ids[k];
 }
 
-// opposed.rs:97-162: one (cell, channel) with a suspected site of that
-// channel at a time; the estimate is computed once and applied to each such
-// site. A resident grid of workgroups takes the listed (sites block,
-// channel) pairs from a shared counter until none is left, so the blocks
-// without a suspected cell cost nothing and the long searches of large
-// clipped areas spread over every workgroup.
-__global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
-                                                     float l0, float l1, float l2, int has_ceiling,
-                                                     float c0, float c1, float c2,
-                                                     DeviceStats* stats) {
+// opposed.rs:243-327, the parts that need no region data, for every
+// suspected (cell, channel) with a supporting channel. A resident grid of
+// workgroups takes the listed (sites block, channel) pairs from a shared
+// counter until none is left, so the long searches of large clipped areas
+// spread over every workgroup. The block's cells are staged in LDS with the
+// three-cell search radius around them; the cells that need a search are
+// listed in LDS and taken densely, so the ring searches run in full
+// wavefronts.
+__global__ __launch_bounds__(THREADS) void opp_sites_local(Layout L, float l0, float l1, float l2) {
     __shared__ float4 s_cell[WY * WX];
     __shared__ float s_pref[WY * WX];
     __shared__ float s_plr[WY * WX];
@@ -1010,107 +981,176 @@ __global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layo
     __shared__ int s_ntask;
     __shared__ unsigned s_item;
     const float lv[3] = {l0, l1, l2};
-    const float ceiling[3] = {c0, c1, c2};
     const int gx = (L.cw + SX - 1) / SX, gy = (L.ch + SY - 1) / SY;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     constexpr int ROWS_PER_THREAD = SY / (THREADS / 64);
     const unsigned listed = L.scalars[3];
-    unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
     for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(L.scalars + 4, 1u);
-    __syncthreads();
-    const unsigned work = s_item;
-    // Every thread reads the same item, so the whole workgroup leaves
-    // together once the list is exhausted.
-    if (work >= listed) break;
-    const unsigned id = L.site_list[work];
-    const int channel = (int)(id / (unsigned)(gx * gy));
-    const int rest = (int)(id % (unsigned)(gx * gy));
-    const int bx0 = (rest % gx) * SX, by0 = (rest / gx) * SY;
-    const size_t plane = (size_t)channel * L.cells;
-    for (int k = threadIdx.x; k < WY * WX; k += THREADS) {
-        const int x = bx0 - SR + k % WX, y = by0 - SR + k / WX;
-        float4 cell = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float pr = 0.0f, pl = 0.0f;
-        if (x >= 0 && y >= 0 && x < L.cw && y < L.ch) {
-            const size_t j = (size_t)y * L.cw + x;
-            cell = L.cell[j];
-            pr = L.pair_ref[plane + j];
-            pl = L.pair_lr[plane + j];
+        if (threadIdx.x == 0) s_item = atomicAdd(L.scalars + 4, 1u);
+        __syncthreads();
+        const unsigned work = s_item;
+        // Every thread reads the same item, so the whole workgroup leaves
+        // together once the list is exhausted.
+        if (work >= listed) break;
+        const unsigned id = L.site_list[work];
+        const int channel = (int)(id / (unsigned)(gx * gy));
+        const int rest = (int)(id % (unsigned)(gx * gy));
+        const int bx0 = (rest % gx) * SX, by0 = (rest / gx) * SY;
+        const size_t plane = (size_t)channel * L.cells;
+        for (int k = threadIdx.x; k < WY * WX; k += THREADS) {
+            const int x = bx0 - SR + k % WX, y = by0 - SR + k / WX;
+            float4 cell = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float pr = 0.0f, pl = 0.0f;
+            if (x >= 0 && y >= 0 && x < L.cw && y < L.ch) {
+                const size_t j = (size_t)y * L.cw + x;
+                cell = L.cell[j];
+                pr = L.pair_ref[plane + j];
+                pl = L.pair_lr[plane + j];
+            }
+            s_cell[k] = cell;
+            s_pref[k] = pr;
+            s_plr[k] = pl;
         }
-        s_cell[k] = cell;
-        s_pref[k] = pr;
-        s_plr[k] = pl;
-    }
-    if (threadIdx.x == 0) s_ntask = 0;
-    __syncthreads();
-    // Cells whose suspected sites need an estimate (a supporting channel)
-    // are listed in LDS and taken densely below, so the ring searches run in
-    // full wavefronts; the others fall back at once.
-    for (int pass = 0; pass < 2; ++pass) {
-    const int items = pass == 0 ? ROWS_PER_THREAD : (s_ntask + THREADS - 1 - (int)threadIdx.x) / THREADS;
-    for (int item = 0; item < items; ++item) {
-        int r, c;
-        if (pass == 0) {
-            r = ty + 4 * item;
-            c = tx;
-        } else {
-            const int code = s_task[threadIdx.x + item * THREADS];
-            r = code / SX;
-            c = code % SX;
+        if (threadIdx.x == 0) s_ntask = 0;
+        __syncthreads();
+        // The suspected cells of the channel with a supporting channel.
+#pragma unroll
+        for (int q = 0; q < ROWS_PER_THREAD; ++q) {
+            const int r = ty + 4 * q, c = tx;
+            if (bx0 + c >= L.cw || by0 + r >= L.ch) continue;
+            const unsigned f = flags_of(s_cell[(r + SR) * WX + c + SR]);
+            if (!(f & (1u << (channel + 3)))) continue;
+            bool supported = false;
+            for (int o = 0; o < 3; ++o)
+                if (o != channel && usable(f, o) && clean(f, o)) supported = true;
+            if (supported) s_task[atomicAdd(&s_ntask, 1)] = (unsigned short)(r * SX + c);
         }
-        const int cx = bx0 + c, cy = by0 + r;
-        if (cx >= L.cw || cy >= L.ch) continue;
-        const int centre = (r + SR) * WX + c + SR;
-        const float4 target = s_cell[centre];
-        const unsigned f = flags_of(target);
-        if (!(f & (1u << (channel + 3)))) continue;
-        int support_count = 0;
-        for (int o = 0; o < 3; ++o)
-            if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
-        if (pass == 0 && support_count != 0) {
-            s_task[atomicAdd(&s_ntask, 1)] = (unsigned short)(r * SX + c);
-            continue;
-        }
-        float est = 0.0f;
-        bool has = false;
-        if (support_count != 0) {
-            Site site{&L, lv, s_cell, s_pref, s_plr, channel, support_mask(target, channel, lv),
-                      false, false, 0.0f, s_keys + threadIdx.x};
+        __syncthreads();
+        for (int k = threadIdx.x; k < s_ntask; k += THREADS) {
+            const int code = s_task[k], r = code / SX, c = code % SX;
+            const int centre = (r + SR) * WX + c + SR;
+            const float4 target = s_cell[centre];
+            SiteLocal site{lv, s_cell, s_pref, s_plr, channel, support_mask(target, channel, lv),
+                           false, false, 0.0f, s_keys + threadIdx.x};
             site.two = __popc(site.support) == 2;
             site.ht = site.two && support_log_ratio(site.support, s_plr[centre], site.tr);
-            has = site.estimate(target, centre, L.label[(size_t)cy * L.cw + cx], est);
+            float reference_value = 0.0f, local = 0.0f;
+            int radius = 0;
+            uint8_t code_out = 0;
+            if (reference(target, site.support, s_pref[centre], reference_value)) {
+                code_out = LOCAL_REFERENCE;
+                if (site.local_correction(centre, local, radius)) code_out |= (uint8_t)radius;
+            }
+            const size_t j = plane + (size_t)(by0 + r) * L.cw + bx0 + c;
+            L.site_value[j] = make_float2(reference_value, local);
+            L.site_code[j] = code_out;
         }
-        const float level = lv[channel];
-        for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) {
-                const int x = 2 * cx + dx, y = 2 * cy + dy;
-                if (x >= L.w || y >= L.h || color_at(pattern, x, y) != channel) continue;
-                float* sp = m + (size_t)y * L.w + x;
-                const float sample = *sp;
-                if (!(sample >= level)) continue;
-                bool neutralise = false;
-                if (!has) {
-                    fallback += 1;
-                    unsupported += support_count == 0;
-                    neutralise = true;
-                } else {
-                    reconstructed += 1;
-                    const float bounded = fminf(est, 4.0f * level);
-                    if (bounded > sample) {
-                        *sp = bounded;
-                        changed += 1;
+        __syncthreads();
+    }
+}
+
+// opposed.rs:97-162 and :243-278: for every suspected (cell, channel), the
+// estimate from the local part and the region correction, applied to each
+// suspected site of that channel in the cell. One thread per cell, in a
+// grid of SITE_GROUPS workgroups striding over the cells.
+constexpr int SITE_GROUPS = 1024;
+__global__ __launch_bounds__(THREADS) void opp_sites(float* __restrict__ m, Layout L, int pattern,
+                                                     float l0, float l1, float l2, int has_ceiling,
+                                                     float c0, float c1, float c2) {
+    const float lv[3] = {l0, l1, l2};
+    const float ceiling[3] = {c0, c1, c2};
+    unsigned reconstructed = 0, changed = 0, fallback = 0, unsupported = 0, neutralised = 0;
+    for (int i = blockIdx.x * THREADS + threadIdx.x; i < L.cells; i += gridDim.x * THREADS) {
+    const unsigned suspected = (unsigned)L.flag8[i] >> 3;
+    if (suspected) {
+        const int cx = i % L.cw, cy = i / L.cw;
+        const float4 cell = L.cell[i];
+        const unsigned f = flags_of(cell);
+        for (int channel = 0; channel < 3; ++channel) {
+            if (!((suspected >> channel) & 1u)) continue;
+            int support_count = 0;
+            for (int o = 0; o < 3; ++o)
+                if (o != channel && usable(f, o) && clean(f, o)) ++support_count;
+            float est = 0.0f;
+            bool has = false;
+            if (support_count != 0) {
+                const size_t j = (size_t)channel * L.cells + i;
+                const uint8_t code = L.site_code[j];
+                if (code & LOCAL_REFERENCE) {
+                    const float2 v = L.site_value[j];
+                    const unsigned support = support_mask(cell, channel, lv);
+                    const bool two = __popc(support) == 2;
+                    float tr = 0.0f;
+                    const bool ht = two && support_log_ratio(support, L.pair_lr[j], tr);
+                    const int radius = code & 7;
+                    const bool has_local = radius != 0;
+                    // region_correction.rs:278-300 (lookup) with the pool fallback.
+                    float region = 0.0f, br = 0.0f;
+                    bool has_region = false;
+                    const int p = pattern_position(channel, support);
+                    if (p >= 0) {
+                        const unsigned label = L.label[i];
+                        const int off = channel * 3 + p;
+                        float4 s = label < (unsigned)L.cap
+                                       ? L.corrections[(size_t)label * SLOTS + off]
+                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (s.z == 0.0f) s = L.corrections[off];
+                        if (s.z != 0.0f) {
+                            region = s.x;
+                            br = s.y;
+                            has_region = chromaticity_agrees(support, ht, tr, two, br);
+                        }
+                    }
+                    float corr = 0.0f;
+                    bool any = true;
+                    if (has_local && has_region) {
+                        float weight = (float)(3 + 1 - radius) / 3.0f;
+                        corr = weight * v.y + (1.0f - weight) * region;
+                    } else if (has_local) {
+                        corr = v.y;
+                    } else if (has_region) {
+                        corr = region;
                     } else {
-                        neutralise = true;
+                        any = false;
+                    }
+                    if (any) {
+                        const float e = v.x + corr;
+                        if (is_finite(e) && e > 0.0f) {
+                            est = e;
+                            has = true;
+                        }
                     }
                 }
-                if (neutralise && has_ceiling && sample > ceiling[channel]) {
-                    *sp = ceiling[channel];
-                    neutralised += 1;
-                }
             }
-    }
-    __syncthreads();
+            const float level = lv[channel];
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int x = 2 * cx + dx, y = 2 * cy + dy;
+                    if (x >= L.w || y >= L.h || color_at(pattern, x, y) != channel) continue;
+                    float* sp = m + (size_t)y * L.w + x;
+                    const float sample = *sp;
+                    if (!(sample >= level)) continue;
+                    bool neutralise = false;
+                    if (!has) {
+                        fallback += 1;
+                        unsupported += support_count == 0;
+                        neutralise = true;
+                    } else {
+                        reconstructed += 1;
+                        const float bounded = fminf(est, 4.0f * level);
+                        if (bounded > sample) {
+                            *sp = bounded;
+                            changed += 1;
+                        } else {
+                            neutralise = true;
+                        }
+                    }
+                    if (neutralise && has_ceiling && sample > ceiling[channel]) {
+                        *sp = ceiling[channel];
+                        neutralised += 1;
+                    }
+                }
+        }
     }
     }
     // Per-workgroup totals go to the workgroup's own slot (no shared
@@ -1144,7 +1184,7 @@ __global__ __launch_bounds__(THREADS) void opp_site_stats(Layout L, unsigned blo
 size_t site_blocks(const Layout& L) {
     return (size_t)((L.cw + SX - 1) / SX) * ((L.ch + SY - 1) / SY) * 3;
 }
-// Resident opp_sites workgroups: three per CU of the MI355X (LDS-bound),
+// Resident opp_sites_local workgroups: three per CU of the MI355X (LDS-bound),
 // never more than there are (sites block, channel) pairs.
 constexpr int SITE_WORKERS = 3 * 256;
 int site_workers(const Layout& L) { return (int)std::min<size_t>(SITE_WORKERS, site_blocks(L)); }
@@ -1183,8 +1223,10 @@ Layout layout_of(const OpposedArgs& a) {
     L.region_first = (unsigned*)take(cap * 4);
     L.region_last = (unsigned*)take(cap * 4);
     L.corrections = (float4*)take(cap * SLOTS * sizeof(float4));
-    L.site_counts = (unsigned*)take(site_blocks(L) * 5 * sizeof(unsigned));
+    L.site_counts = (unsigned*)take(SITE_GROUPS * 5 * sizeof(unsigned));
     L.site_list = (unsigned*)take(site_blocks(L) * sizeof(unsigned));
+    L.site_value = (float2*)take(cells * 3 * sizeof(float2));
+    L.site_code = (uint8_t*)take(cells * 3);
     L.scalars = (unsigned*)take(64);
     L.log2_bitmap = a.log2_bitmap;
     return L;
@@ -1216,6 +1258,27 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     RHD_TIMED(prof, s, "opp_cells", 4 * px + 17 * cells + 24 * cells,
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
+    // The local part of the site estimates needs only the cells: it runs on
+    // the auxiliary stream beside the labelling and the region sums, and the
+    // main stream waits for it before the final site pass. Without an
+    // auxiliary stream everything runs in order on s.
+    hipStream_t side = a.aux ? a.aux : s;
+    if (a.aux) {
+        hipError_t e = hipEventRecord(a.fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.fork, 0);
+        if (e != hipSuccess) return e;
+    }
+    const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
+    RHD_TIMED(prof, side, "opp_site_list", cells,
+              hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),
+                                 dim3(THREADS), 0, side, L));
+    RHD_TIMED(prof, side, "opp_sites_local", 16 * cells + 8 * cells,
+              hipLaunchKernelGGL(opp_sites_local, dim3(site_workers(L)), dim3(THREADS), 0, side, L,
+                                 l0, l1, l2));
+    if (a.aux) {
+        hipError_t e = hipEventRecord(a.join, a.aux);
+        if (e != hipSuccess) return e;
+    }
     RHD_TIMED(prof, s, "opp_label_tiles", cells + 4 * cells,
               hipLaunchKernelGGL(opp_label_tiles, dim3(tiles_x, tiles_y), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_label_borders", 20ull * border_cells,
@@ -1248,17 +1311,18 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
     RHD_TIMED(prof, s, "opp_sums", 0,
               hipLaunchKernelGGL(opp_sums, dim3(1024), dim3(SUM_THREADS), 0, s, L));
-    const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
-    RHD_TIMED(prof, s, "opp_site_list", 16 * cells,
-              hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),
-                                 dim3(THREADS), 0, s, L));
-    RHD_TIMED(prof, s, "opp_sites", 8 * px + 16 * cells,
-              hipLaunchKernelGGL(opp_sites, dim3(site_workers(L)), dim3(THREADS), 0, s, a.mosaic, L,
+    if (a.aux) {
+        hipError_t e = hipStreamWaitEvent(s, a.join, 0);
+        if (e != hipSuccess) return e;
+    }
+    const int site_groups = std::min(cell_blocks, SITE_GROUPS);
+    RHD_TIMED(prof, s, "opp_sites", 8 * px + cells,
+              hipLaunchKernelGGL(opp_sites, dim3(site_groups), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2, a.has_ceiling, a.ceiling[0], a.ceiling[1],
-                                 a.ceiling[2], a.stats));
-    RHD_TIMED(prof, s, "opp_site_stats", 20 * site_workers(L),
+                                 a.ceiling[2]));
+    RHD_TIMED(prof, s, "opp_site_stats", 20 * site_groups,
               hipLaunchKernelGGL(opp_site_stats, dim3(64), dim3(THREADS), 0, s, L,
-                                 (unsigned)site_workers(L), a.stats));
+                                 (unsigned)site_groups, a.stats));
     return hipGetLastError();
 }
 

@@ -31,6 +31,10 @@ struct Lane {
     int index = 0;
     bool open = false;
     hipStream_t stream = nullptr;
+    // A second stream for work that can run beside the lane's own, forked
+    // from and joined back into `stream` with the two events.
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     DeviceStats* d_stats = nullptr;
     DeviceStats* h_stats = nullptr;
     DeviceBuffer amaze, amaze_areas, rcd, capture, opposed, optics, raw;
