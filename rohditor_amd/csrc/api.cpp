@@ -30,8 +30,8 @@ void context_unref(rhd_context* ctx) {
         if (!L.open) continue;
         for (auto& ev : L.ev)
             if (ev) (void)hipEventDestroy(ev);
-        if (L.fork) (void)hipEventDestroy(L.fork);
-        if (L.join) (void)hipEventDestroy(L.join);
+        for (hipEvent_t ev : {L.fork, L.mid, L.join})
+            if (ev) (void)hipEventDestroy(ev);
         if (L.d_stats) (void)hipFree(L.d_stats);
         if (L.h_stats) (void)hipHostFree(L.h_stats);
         if (L.aux) (void)hipStreamDestroy(L.aux);
@@ -261,6 +261,7 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out) {
         bool ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&L.aux, hipStreamNonBlocking) == hipSuccess &&
                   hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&L.mid, hipEventDisableTiming) == hipSuccess &&
                   hipEventCreateWithFlags(&L.join, hipEventDisableTiming) == hipSuccess &&
                   hipMalloc(&L.d_stats, sizeof(DeviceStats)) == hipSuccess &&
                   hipHostMalloc((void**)&L.h_stats, sizeof(DeviceStats), hipHostMallocDefault) ==
@@ -269,8 +270,8 @@ rhd_status lane_open(rhd_context* ctx, int i, Lane** out) {
         if (!ok) {
             for (auto& ev : L.ev)
                 if (ev) (void)hipEventDestroy(ev);
-            if (L.fork) (void)hipEventDestroy(L.fork);
-            if (L.join) (void)hipEventDestroy(L.join);
+            for (hipEvent_t ev : {L.fork, L.mid, L.join})
+                if (ev) (void)hipEventDestroy(ev);
             if (L.d_stats) (void)hipFree(L.d_stats);
             if (L.h_stats) (void)hipHostFree(L.h_stats);
             if (L.aux) (void)hipStreamDestroy(L.aux);
@@ -544,6 +545,7 @@ rhd_status run_opposed(rhd_context* ctx, Lane& L, rhd_plane* plane, const rhd_se
     if (!a.log2_bitmap) return fail(ctx, RHD_SYNCHRONIZATION, "libm tables are not set up");
     a.aux = L.aux;
     a.fork = L.fork;
+    a.pairs = L.mid;
     a.join = L.join;
     hipError_t e = launch_opposed(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "opposed");

@@ -216,6 +216,16 @@ __global__ __launch_bounds__(THREADS) void opp_cells(const float* __restrict__ m
     const float4 cell = make_float4(means[0], means[1], means[2], __uint_as_float(flags));
     L.cell[i] = cell;
     L.flag8[i] = (uint8_t)flags;
+}
+
+// The pair reference and log ratio of every cell for each excluded channel
+// (pair_values); only the site searches and the ring contributions read
+// them, so this runs on the auxiliary stream beside the labelling.
+__global__ __launch_bounds__(THREADS) void opp_pairs(Layout L) {
+    const int i = blockIdx.x * THREADS + threadIdx.x;
+    if (i >= L.cells) return;
+    const float4 cell = L.cell[i];
+    const float means[3] = {cell.x, cell.y, cell.z};
     float roots[3];
     for (int c = 0; c < 3; ++c) roots[c] = positive_finite(means[c]) ? cbrt_f32(means[c]) : 0.0f;
     for (int e = 0; e < 3; ++e) {
@@ -1255,17 +1265,24 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
     const uint64_t px = (uint64_t)L.w * L.h, cells = (uint64_t)L.cells;
     // Algorithmic bytes per launch. The ring-entry kernels (contrib, sums)
     // depend on the ring size, which only the device knows, and report 0.
-    RHD_TIMED(prof, s, "opp_cells", 4 * px + 17 * cells + 24 * cells,
+    RHD_TIMED(prof, s, "opp_cells", 4 * px + 17 * cells,
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
-    // The local part of the site estimates needs only the cells: it runs on
-    // the auxiliary stream beside the labelling and the region sums, and the
-    // main stream waits for it before the final site pass. Without an
-    // auxiliary stream everything runs in order on s.
+    // The pair planes and the local part of the site estimates need only the
+    // cells: they run on the auxiliary stream beside the labelling and the
+    // region sums. The main stream waits for the pair planes before the ring
+    // contributions and for the local parts before the final site pass.
+    // Without an auxiliary stream everything runs in order on s.
     hipStream_t side = a.aux ? a.aux : s;
     if (a.aux) {
         hipError_t e = hipEventRecord(a.fork, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.fork, 0);
+        if (e != hipSuccess) return e;
+    }
+    RHD_TIMED(prof, side, "opp_pairs", 16 * cells + 24 * cells,
+              hipLaunchKernelGGL(opp_pairs, dim3(cell_blocks), dim3(THREADS), 0, side, L));
+    if (a.aux) {
+        hipError_t e = hipEventRecord(a.pairs, a.aux);
         if (e != hipSuccess) return e;
     }
     const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
@@ -1307,6 +1324,10 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
                                  (int)RING);
               hipLaunchKernelGGL(opp_scan_emit, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)RING));
+    if (a.aux) {
+        hipError_t e = hipStreamWaitEvent(s, a.pairs, 0);
+        if (e != hipSuccess) return e;
+    }
     RHD_TIMED(prof, s, "opp_contrib", 0,
               hipLaunchKernelGGL(opp_contrib, dim3(2048), dim3(THREADS), 0, s, L, l0, l1, l2));
     RHD_TIMED(prof, s, "opp_sums", 0,

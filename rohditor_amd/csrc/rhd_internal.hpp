@@ -170,10 +170,10 @@ struct OpposedArgs {
     size_t scratch_bytes;
     DeviceStats* stats;
     const uint32_t* log2_bitmap;  // libm_f32.hpp exception bitmap on this device
-    // Optional second stream (with two events) for the work that depends only
-    // on the cells; null runs everything on the launch stream.
+    // Optional second stream (with three events) for the work that depends
+    // only on the cells; null runs everything on the launch stream.
     hipStream_t aux;
-    hipEvent_t fork, join;
+    hipEvent_t fork, pairs, join;
 };
 size_t opposed_scratch_bytes(int width, int height);
 hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);
