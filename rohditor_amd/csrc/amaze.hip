@@ -1130,9 +1130,16 @@ __device__ __forceinline__ void back_finish(const BackLds& s, const AmazeArgs& a
         a.out[o] = rgb[0];
         a.out[n + o] = rgb[1];
         a.out[2 * n + o] = rgb[2];
+        if (a.guide) {
+            const CaptureInputs in =
+                capture_inputs_fast(rgb[0], rgb[1], rgb[2], a.ceilings, a.ceilings_inv);
+            a.guide[o] = in.guide;
+            a.mask[o] = in.mask;
+        }
         finite = finite && is_finite(rgb[0]) && is_finite(rgb[1]) && is_finite(rgb[2]);
     }
-    if (!finite) atomicOr(&a.stats->nonfinite, 2u);
+    // Bit 2 flags the capture input as well when its guide is made here.
+    if (!finite) atomicOr(&a.stats->nonfinite, a.guide ? 6u : 2u);
 }
 
 __global__ __launch_bounds__(BACK_THREADS) void amz_back(AmazeArgs a, float clip_pt) {

@@ -572,11 +572,27 @@ Balance resolve_balance(const rhd_sensor_desc* d) {
     return b;
 }
 
+// Capture sharpening is the only detail step: nothing changes the demosaiced
+// RGB before it, so its guide and mask can be made as the RGB is stored. The
+// fused form divides by the ceilings through their reciprocals, exact for
+// ceilings in [2^-20, 2^20].
+static bool capture_follows_demosaic(const rhd_detail_desc* dd) {
+    if (!capture_active(dd) || dd->false_colour != 0 || noise_active(dd)) return false;
+    for (int c = 0; c < 3; ++c)
+        if (!(dd->ceilings[c] >= 0x1p-20f && dd->ceilings[c] <= 0x1p20f)) return false;
+    return true;
+}
+
+// `dd` (may be null) is the detail stage that follows on `src`. When the
+// demosaic also writes capture sharpening's guide and mask into the lane's
+// capture scratch, *capture_ready is set, and run_detail takes it.
 rhd_status run_demosaic(rhd_context* ctx, Lane& L, const rhd_plane* plane,
-                        const rhd_sensor_desc* d, rhd_source* src) {
+                        const rhd_sensor_desc* d, rhd_source* src,
+                        const rhd_detail_desc* dd = nullptr, bool* capture_ready = nullptr) {
     const int w = (int)plane->width, h = (int)plane->height, pattern = plane->pattern;
     const Balance b = resolve_balance(d);
     hipError_t e = hipSuccess;
+    if (capture_ready) *capture_ready = false;
     switch (d->demosaic) {
         case RHD_DEMOSAIC_BILINEAR:
             e = launch_bilinear(plane->d, w, h, pattern, b.forward, b.inverse, src->d, L.d_stats,
@@ -614,6 +630,18 @@ rhd_status run_demosaic(rhd_context* ctx, Lane& L, const rhd_plane* plane,
             a.areas = (int*)L.amaze_areas.ptr;
             a.out = src->d;
             a.stats = L.d_stats;
+            if (capture_ready && capture_follows_demosaic(dd)) {
+                // The same size run_detail asks for, so the planes survive.
+                st = ensure_scratch(ctx, L, L.capture, detail_scratch_bytes(w, h, dd));
+                if (st != RHD_OK) return st;
+                a.guide = (float*)L.capture.ptr;
+                a.mask = a.guide + (size_t)w * h;
+                for (int c = 0; c < 3; ++c) {
+                    a.ceilings[c] = dd->ceilings[c];
+                    a.ceilings_inv[c] = 1.0f / dd->ceilings[c];
+                }
+                *capture_ready = true;
+            }
             for (int base = 0; base < tiles && e == hipSuccess; base += per_launch) {
                 a.tile_base = base;
                 e = launch_amaze(a, std::min(per_launch, tiles - base), L.stream, PROF(L));
@@ -659,7 +687,8 @@ int gaussian_weights(float sigma, float* w) {
     return radius;
 }
 
-rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd) {
+rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd,
+                       bool inputs_ready) {
     const int w = (int)src->width, h = (int)src->height;
     rhd_status st = ensure_scratch(ctx, L, L.capture, capture_scratch_bytes(w, h));
     if (st != RHD_OK) return st;
@@ -680,6 +709,7 @@ rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_det
     a.est_b = base + 3 * n;
     a.tmp = base + 4 * n;
     a.stats = L.d_stats;
+    a.inputs_ready = inputs_ready;
     hipError_t e = launch_capture(a, L.stream, PROF(L));
     if (e != hipSuccess) return hip_fail(ctx, e, "capture");
     return RHD_OK;
@@ -689,7 +719,8 @@ rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_det
 // pipeline/detail.rs:210-307): noise profiles of the stage input, false
 // colour, chroma NR, luminance NR, capture sharpening. Everything is queued
 // on the lane's stream; the profiles never leave the device.
-rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd) {
+rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd,
+                      bool capture_ready) {
     if (!detail_active(dd)) return RHD_OK;
     const int w = (int)src->width, h = (int)src->height;
     const uint64_t work = detail_work_bytes(w, h, dd);
@@ -724,7 +755,7 @@ rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_deta
                                       dd->balance, dd->ceilings, prof, scratch, L.stream, PROF(L));
         if (e != hipSuccess) return hip_fail(ctx, e, "luminance noise reduction");
     }
-    if (capture_active(dd)) return run_capture(ctx, L, src, dd);
+    if (capture_active(dd)) return run_capture(ctx, L, src, dd, capture_ready);
     return RHD_OK;
 }
 
@@ -804,8 +835,9 @@ rhd_status enqueue_develop(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
     if (e != hipSuccess) return hip_fail(ctx, e, "normalize");
     if (d->highlight == RHD_HIGHLIGHT_OPPOSED && (st = run_opposed(ctx, L, plane, d)) != RHD_OK)
         return st;
-    if ((st = run_demosaic(ctx, L, plane, d, src)) != RHD_OK) return st;
-    if (detail_active(dd) && (st = run_detail(ctx, L, src, dd)) != RHD_OK) return st;
+    bool ready = false;
+    if ((st = run_demosaic(ctx, L, plane, d, src, dd, &ready)) != RHD_OK) return st;
+    if (detail_active(dd) && (st = run_detail(ctx, L, src, dd, ready)) != RHD_OK) return st;
     RHD_HIP(ctx, hipMemcpyAsync(stats_out, L.d_stats, sizeof(DeviceStats), hipMemcpyDeviceToHost,
                                 L.stream));
     return RHD_OK;
@@ -865,7 +897,8 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
     if (st == RHD_OK && d->highlight == RHD_HIGHLIGHT_OPPOSED) st = run_opposed(ctx, L, plane, d);
     step(hipEventRecord(L.ev[2], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK && cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src);
+    bool ready = false;
+    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src, dd, &ready);
     step(hipEventRecord(L.ev[3], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK && base) {
         st = new_source(ctx, &L, src->width, src->height, base);
@@ -877,7 +910,7 @@ rhd_status develop_on(rhd_context* ctx, Lane& L, const rhd_mosaic* mosaic,
     }
     if (st == RHD_OK && detail_active(dd)) {
         if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_detail(ctx, L, src, dd);
+        if (st == RHD_OK) st = run_detail(ctx, L, src, dd, ready);
     }
     step(hipEventRecord(L.ev[4], L.stream) == hipSuccess, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK) st = finish_stats(ctx, L, m, d->highlight);
@@ -1149,12 +1182,13 @@ rhd_status rhd_demosaic(rhd_context* ctx, rhd_plane* plane, const rhd_sensor_des
     if (st == RHD_OK) st = reset_stats(ctx, L);
     if (st == RHD_OK && hipEventRecord(L.ev[2], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
-    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src);
+    bool ready = false;
+    if (st == RHD_OK) st = run_demosaic(ctx, L, plane, d, src, dd, &ready);
     if (st == RHD_OK && hipEventRecord(L.ev[3], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");
     if (st == RHD_OK && detail_active(dd)) {
         if (cancelled(cancel)) st = fail(ctx, RHD_CANCELLED, "cancelled");
-        if (st == RHD_OK) st = run_detail(ctx, L, src, dd);
+        if (st == RHD_OK) st = run_detail(ctx, L, src, dd, ready);
     }
     if (st == RHD_OK && hipEventRecord(L.ev[4], L.stream) != hipSuccess)
         st = fail(ctx, RHD_SYNCHRONIZATION, "event");

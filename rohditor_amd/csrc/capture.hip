@@ -20,17 +20,11 @@ namespace rhd {
 namespace {
 
 constexpr int TW = 64, TH = 16, MAXR = 4, THREADS = 256;
-constexpr float FLOOR = 1.0e-6f;
+constexpr float FLOOR = CAPTURE_FLOOR;
 
 struct Weights {
     float w[2 * MAXR + 1];
 };
-
-__device__ __forceinline__ float smoothstep(float low, float high, float value) {
-    float t = (value - low) / (high - low);
-    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-    return t * t * (3.0f - 2.0f * t);
-}
 
 // sharpening.rs:194-218
 __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rgb, int n,
@@ -42,16 +36,9 @@ __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rg
     if (i >= n) return;
     float r = rgb[i], g = rgb[n + i], b = rgb[2 * (size_t)n + i];
     if (!(is_finite(r) && is_finite(g) && is_finite(b))) atomicOr(&stats->nonfinite, 4u);
-    float s = -0.0f;
-    s += num_max(r, 0.0f) / 3.0f;
-    s += num_max(g, 0.0f) / 3.0f;
-    s += num_max(b, 0.0f) / 3.0f;
-    guide[i] = num_max(s, FLOOR);
-    float peak = 0.0f;
-    peak = num_max(peak, r / l0);
-    peak = num_max(peak, g / l1);
-    peak = num_max(peak, b / l2);
-    mask[i] = 1.0f - smoothstep(0.85f, 0.98f, peak);
+    const CaptureInputs in = capture_inputs(r, g, b, l0, l1, l2);
+    guide[i] = in.guide;
+    mask[i] = in.mask;
 }
 
 // The reference's sums start from 0.0 (interior, vertical) or -0.0 (the
@@ -480,10 +467,11 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     Weights k{};
     for (int t = 0; t <= 2 * a.radius; ++t) k.w[t] = a.weights[t];
     const int blocks = (n + THREADS - 1) / THREADS;
-    RHD_TIMED(prof, s, "cap_init", 20 * px,
-              hipLaunchKernelGGL(cap_init, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n,
-                                 a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide, a.mask,
-                                 a.stats));
+    if (!a.inputs_ready)
+        RHD_TIMED(prof, s, "cap_init", 20 * px,
+                  hipLaunchKernelGGL(cap_init, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n,
+                                     a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide,
+                                     a.mask, a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
     // The radius is a template parameter so the tap loops unroll.
     auto blur = [&](const float* in, const float* aux, float* out, int mode) {

@@ -129,12 +129,15 @@ uint64_t detail_scratch_bytes(int w, int h, const rhd_detail_desc* dd);
 rhd_status decode_arw_on(rhd_context* ctx, Lane& L, const uint8_t* file, uint64_t size,
                          const rhd_arw_layout* l, rhd_mosaic** out, float* gpu_ms);
 // False colour then capture sharpening on `src` (in place).
-rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
+// capture_ready: the demosaic already wrote capture's guide and mask planes.
+rhd_status run_detail(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd,
+                      bool capture_ready = false);
 rhd_status validate_detail(rhd_context* ctx, const rhd_detail_desc* dd);
 rhd_status new_source(rhd_context* ctx, Lane* L, uint32_t w, uint32_t h, rhd_source** out);
 rhd_status reset_stats(rhd_context* ctx, Lane& L);
 rhd_status finish_stats(rhd_context* ctx, Lane& L, rhd_metrics* m, int highlight);
-rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd);
+rhd_status run_capture(rhd_context* ctx, Lane& L, rhd_source* src, const rhd_detail_desc* dd,
+                       bool inputs_ready = false);
 rhd_status stats_to_metrics(rhd_context* ctx, const DeviceStats& s, rhd_metrics* m, int highlight);
 rhd_status validate_desc(rhd_context* ctx, const rhd_sensor_desc* d);
 int crop_pattern(const rhd_sensor_desc* d);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "capture_math.hpp"
+
 namespace rhd {
 
 // Bayer colour at (x, y) for a pattern code (0 RGGB, 1 BGGR, 2 GRBG, 3 GBRG):

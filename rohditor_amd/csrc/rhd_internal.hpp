@@ -89,6 +89,14 @@ struct AmazeArgs {
     int* areas;           // per tile: start_row, end_row, start_col, end_col
     float* out;           // planar R, G, B (3 * width * height)
     DeviceStats* stats;
+    // When set, capture sharpening's guide and mask planes (width * height
+    // each) are written from the output as it is stored, with the channel
+    // ceilings below (each in [2^-20, 2^20]) and their reciprocals; capture
+    // then starts from them.
+    float* guide;
+    float* mask;
+    float ceilings[3];
+    float ceilings_inv[3];
 };
 
 // Launchers (each returns the first HIP error, or hipSuccess).
@@ -128,6 +136,7 @@ struct CaptureArgs {
     float* est_b;
     float* tmp;
     DeviceStats* stats;
+    bool inputs_ready;  // guide and mask already hold this image's values
 };
 size_t capture_scratch_bytes(int width, int height);
 // False-colour suppression in place on planar RGB; `scratch` holds 4 planes.
