@@ -8,12 +8,17 @@
 // reference's tap order (0.0 start, taps in ascending order, -0.0 start in
 // the mirrored horizontal border) and half-sample mirroring.
 //
-// Kernels: init (guide, highlight mask), two single blurs with the mask
+// Kernels: init (guide, highlight mask; the AMaZE finish writes both instead
+// when capture directly follows it), two single blurs with the mask
 // epilogues, one fused kernel per RL iteration (blur estimate -> ratio ->
 // blur ratio -> clamped update, all in LDS for a 64x16 output tile with a
-// 2r halo), and the gain apply. Iterations ping-pong between two estimate
-// planes. HBM-bound: per iteration the estimate (with halo) and the guide are
-// read and the estimate written.
+// 2r halo), the last of which applies the gain to RGB instead of storing its
+// estimate, and a plain gain apply for zero iterations. Iterations ping-pong
+// between two estimate planes. Per iteration the estimate (with halo) and
+// the guide are read and the estimate written; the iterations are bound by
+// vector-ALU issue (the blur taps and the ratio's division), not by HBM.
+#include <type_traits>
+
 #include "rhd_internal.hpp"
 
 namespace rhd {
@@ -196,10 +201,33 @@ struct IterWindows {
     static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;  // guide / ratio
 };
 
-template <int R, bool FAST>
+// The gain apply (sharpening.rs:105-126) at pixel i of planar RGB with n
+// pixels per plane: gain = 1 + amount * mask * (estimate / guide - 1), and
+// the pixel is scaled only when the gain differs from 1 and every scaled
+// channel stays finite.
+struct Apply {
+    float* rgb;
+    const float* mask;
+    float amount;
+    size_t n;
+};
+__device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est, float guide) {
+    const float gain = 1.0f + ap.amount * ap.mask[i] * (est / guide - 1.0f);
+    if (gain == 1.0f) return;
+    const float r = ap.rgb[i], g = ap.rgb[ap.n + i], b = ap.rgb[2 * ap.n + i];
+    if (is_finite(r * gain) && is_finite(g * gain) && is_finite(b * gain)) {
+        ap.rgb[i] = r * gain;
+        ap.rgb[ap.n + i] = g * gain;
+        ap.rgb[2 * ap.n + i] = b * gain;
+    }
+}
+
+// APPLY: the last iteration applies the gain with its estimate instead of
+// storing the estimate.
+template <int R, bool FAST, bool APPLY>
 __device__ void iterate_tile(const float* __restrict__ est_in, const float* __restrict__ guide,
                              float* __restrict__ est_out, int width, int height, const Weights& k,
-                             float* s_a, float* s_b) {
+                             const Apply& ap, float* s_a, float* s_b) {
     using Win = IterWindows<R>;
     constexpr int EW = Win::EW, HW = Win::HW, GW = Win::GW;
     const uint3 blk = xcd_block();
@@ -235,7 +263,11 @@ __device__ void iterate_tile(const float* __restrict__ est_in, const float* __re
         const float g = guide[i];
         const float v = est_in[i] * b;
         const float lo = g * 0.5f, hi = g * 2.0f;
-        est_out[i] = v < lo ? lo : (v > hi ? hi : v);
+        const float est = v < lo ? lo : (v > hi ? hi : v);
+        if (APPLY)
+            apply_gain(ap, i, est, g);
+        else
+            est_out[i] = est;
     }
 }
 
@@ -341,10 +373,10 @@ __device__ void blur_interior(const float* __restrict__ in, const float* __restr
     }
 }
 
-template <int R>
+template <int R, bool APPLY>
 __device__ void iterate_interior(const float* __restrict__ est_in, const float* __restrict__ guide,
                                  float* __restrict__ est_out, int width, const Weights& k,
-                                 float* s_a, float* s_b) {
+                                 const Apply& ap, float* s_a, float* s_b) {
     using Win = InteriorWindows<R>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
     constexpr int NS1 = (GW + 3) / 4;   // column strips of the estimate's blur
@@ -404,22 +436,26 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
         float b[4];
         vstrip<R, P>(s_b + r0 * P + c, k, b);
-        float* out_row = est_out + (size_t)(y0 + r0) * width + x0;
+        const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float g = g_out[q][i];
             const float v = e_out[q][i] * b[i];
             const float lo = g * 0.5f, hi = g * 2.0f;
-            out_row[(size_t)i * width + c] = v < lo ? lo : (v > hi ? hi : v);
+            const float est = v < lo ? lo : (v > hi ? hi : v);
+            if (APPLY)
+                apply_gain(ap, o + (size_t)i * width, est, g);
+            else
+                est_out[o + (size_t)i * width] = est;
         }
     }
 }
 
-template <int R>
+template <int R, bool APPLY>
 __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__ est_in,
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
-                                                       int height, Weights k) {
+                                                       int height, Weights k, Apply ap) {
     using Win = IterWindows<R>;
     using IWin = InteriorWindows<R>;
     __shared__ __align__(16) float s_a[IWin::A_FLOATS];
@@ -432,9 +468,9 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
     const bool interior =
         x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
     if (interior)
-        iterate_interior<R>(est_in, guide, est_out, width, k, s_a, s_b);
+        iterate_interior<R, APPLY>(est_in, guide, est_out, width, k, ap, s_a, s_b);
     else
-        iterate_tile<R, false>(est_in, guide, est_out, width, height, k, s_a, s_b);
+        iterate_tile<R, false, APPLY>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
 }
 
 // sharpening.rs:105-126
@@ -444,14 +480,7 @@ __global__ __launch_bounds__(THREADS) void cap_apply(float* __restrict__ rgb, in
                                                      const float* __restrict__ est) {
     int i = blockIdx.x * THREADS + threadIdx.x;
     if (i >= n) return;
-    float gain = 1.0f + amount * mask[i] * (est[i] / guide[i] - 1.0f);
-    if (gain == 1.0f) return;
-    float r = rgb[i], g = rgb[n + i], b = rgb[2 * (size_t)n + i];
-    if (is_finite(r * gain) && is_finite(g * gain) && is_finite(b * gain)) {
-        rgb[i] = r * gain;
-        rgb[n + i] = g * gain;
-        rgb[2 * (size_t)n + i] = b * gain;
-    }
+    apply_gain(Apply{rgb, mask, amount, (size_t)n}, i, est[i], guide[i]);
 }
 
 }  // namespace
@@ -482,12 +511,14 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
             default: hipLaunchKernelGGL(cap_blur<4>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
         }
     };
-    auto iterate = [&](const float* in, float* out) {
+    const Apply ap{a.rgb, a.mask, a.amount, (size_t)n};
+    auto iterate = [&](const float* in, float* out, auto apply) {
+        constexpr bool APPLY = decltype(apply)::value;
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL(cap_iterate<1>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
-            case 2: hipLaunchKernelGGL(cap_iterate<2>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
-            case 3: hipLaunchKernelGGL(cap_iterate<3>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
-            default: hipLaunchKernelGGL(cap_iterate<4>, tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k); break;
+            case 1: hipLaunchKernelGGL((cap_iterate<1, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
+            case 2: hipLaunchKernelGGL((cap_iterate<2, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
+            case 3: hipLaunchKernelGGL((cap_iterate<3, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
+            default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
         }
     };
     // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
@@ -496,16 +527,24 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // The estimate starts as the guide (sharpening.rs:219): the first
     // iteration reads the guide itself, the later ones alternate between the
     // two estimate planes.
+    // The last iteration applies the gain itself; with no iterations the
+    // estimate is the guide and cap_apply does it.
     const float* cur = a.guide;
-    for (int it = 0; it < a.iterations; ++it) {
+    for (int it = 0; it + 1 < a.iterations; ++it) {
         float* out = (it & 1) ? a.est_b : a.est_a;
         // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
-        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, out));
+        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, out, std::false_type{}));
         cur = out;
     }
-    RHD_TIMED(prof, s, "cap_apply", 36 * px,
-              hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n, a.amount,
-                                 a.guide, a.mask, cur));
+    if (a.iterations > 0) {
+        // Reads the estimate, the guide, the mask and RGB, writes RGB.
+        RHD_TIMED(prof, s, "cap_iterate_apply", 36 * px,
+                  iterate(cur, nullptr, std::true_type{}));
+    } else {
+        RHD_TIMED(prof, s, "cap_apply", 36 * px,
+                  hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n,
+                                     a.amount, a.guide, a.mask, cur));
+    }
     return hipGetLastError();
 }
 
