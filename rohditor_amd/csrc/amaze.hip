@@ -114,10 +114,14 @@ constexpr int FG_W = FW + 4, FG_H = FH + 4;      // gradients: columns c0 - 2 ..
 constexpr int FD_W = FW / 2 + 2, FD_H = FH + 4;  // diagonal planes: pairs p0 - 1 .., rows r0 - 2 ..
 static_assert(TS % FW == 0 && TS % FH == 0 && FH % 4 == 0, "whole blocks");
 
+// The gradients are read only by the colour differences and the diagonal
+// planes only by the diagonal estimates, which run after them: the two sets
+// share one array, so five blocks fit a CU's LDS instead of three.
+constexpr int FG_N = FG_H * FG_W, FD_N = FD_H * FD_W;
+static_assert(4 * FD_N <= 2 * FG_N, "the diagonal planes fit the gradients' space");
 struct alignas(16) FrontLds {
     float cfa[FC_H * FC_W];
-    float w0[FG_H * FG_W], w1[FG_H * FG_W];
-    float dp[FD_H * FD_W], dm[FD_H * FD_W], qp[FD_H * FD_W], qm[FD_H * FD_W];
+    float work[2 * FG_N];  // w0, w1, then dp, dm, qp, qm
 };
 
 __device__ __forceinline__ float adaptive_or(float ratio, float centre, float fallback) {
@@ -149,9 +153,15 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     const Tile t = tile_of(a, blk.z);
     const int c0 = blk.x * FW, r0 = blk.y * FH, p0 = c0 / 2;
     __shared__ FrontLds s;
+    float* const s_w0 = s.work;
+    float* const s_w1 = s.work + FG_N;
+    float* const s_dp = s.work;
+    float* const s_dm = s.work + FD_N;
+    float* const s_qp = s.work + 2 * FD_N;
+    float* const s_qm = s.work + 3 * FD_N;
     auto C = [&](int rr, int cc) { return s.cfa[(rr - r0 + 4) * FC_W + (cc - c0 + 4)]; };
-    auto W0 = [&](int rr, int cc) { return s.w0[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
-    auto W1 = [&](int rr, int cc) { return s.w1[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
+    auto W0 = [&](int rr, int cc) { return s_w0[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
+    auto W1 = [&](int rr, int cc) { return s_w1[(rr - r0 + 2) * FG_W + (cc - c0 + 2)]; };
     auto D = [&](const float* pl, int rr, int p) { return pl[(rr - r0 + 2) * FD_W + (p - p0 + 1)]; };
 
     // Lanes take columns, wavefronts rows: a lane's source column, mirror
@@ -192,8 +202,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         }
     }
     __syncthreads();
-    // Gradients over [2, TS - 2) and the diagonal planes over [6, TS - 6)
-    // (even columns, one entry per pair); zero elsewhere, unread.
+    // Gradients over [2, TS - 2); zero elsewhere, unread.
     static_assert(FG_W > 64 && FG_W <= 128 && FD_W <= 64, "column slots per lane");
     for (int ly = wave; ly < FG_H; ly += 4) {
         const int rr = r0 - 2 + ly;
@@ -206,35 +215,9 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
                 w0 = vgrad(C, rr, cc);
                 w1 = hgrad(C, rr, cc);
             }
-            s.w0[ly * FG_W + lx] = w0;
-            s.w1[ly * FG_W + lx] = w1;
+            s_w0[ly * FG_W + lx] = w0;
+            s_w1[ly * FG_W + lx] = w1;
         }
-    }
-    for (int ly = wave; ly < FD_H; ly += 4) {
-        const int rr = r0 - 2 + ly;
-        if (lane >= FD_W) continue;
-        const int cc = c0 - 2 + 2 * lane;
-        float delp = 0.0f, delm = 0.0f, sqp = 0.0f, sqm = 0.0f;
-        if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
-            if (gae(a.pattern, rr) == 0) {
-                delp = fabsf(C(rr - 1, cc + 1) - C(rr + 1, cc - 1));
-                delm = fabsf(C(rr + 1, cc + 1) - C(rr - 1, cc - 1));
-                const float g = C(rr, cc + 1);
-                sqp = sqr(g - C(rr + 1, cc)) + sqr(g - C(rr - 1, cc + 2));
-                sqm = sqr(g - C(rr - 1, cc)) + sqr(g - C(rr + 1, cc + 2));
-            } else {
-                const float g = C(rr, cc);
-                sqp = sqr(g - C(rr + 1, cc - 1)) + sqr(g - C(rr - 1, cc + 1));
-                sqm = sqr(g - C(rr - 1, cc - 1)) + sqr(g - C(rr + 1, cc + 1));
-                delp = fabsf(C(rr - 1, cc + 2) - C(rr + 1, cc));
-                delm = fabsf(C(rr + 1, cc + 2) - C(rr - 1, cc));
-            }
-        }
-        const int k = ly * FD_W + lane;
-        s.dp[k] = delp;
-        s.dm[k] = delm;
-        s.qp[k] = sqp;
-        s.qm[k] = sqm;
     }
     __syncthreads();
 
@@ -303,6 +286,37 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         t.full(VCD)[i] = vcd;
         t.full(HCD)[i] = hcd;
     }
+    // Every read of the gradients is done before the diagonal planes
+    // overwrite them: the planes over [6, TS - 6) (even columns, one entry
+    // per pair); zero elsewhere, unread.
+    __syncthreads();
+    for (int ly = wave; ly < FD_H; ly += 4) {
+        const int rr = r0 - 2 + ly;
+        if (lane >= FD_W) continue;
+        const int cc = c0 - 2 + 2 * lane;
+        float delp = 0.0f, delm = 0.0f, sqp = 0.0f, sqm = 0.0f;
+        if (rr >= 6 && rr < TS - 6 && cc >= 6 && cc < TS - 6) {
+            if (gae(a.pattern, rr) == 0) {
+                delp = fabsf(C(rr - 1, cc + 1) - C(rr + 1, cc - 1));
+                delm = fabsf(C(rr + 1, cc + 1) - C(rr - 1, cc - 1));
+                const float g = C(rr, cc + 1);
+                sqp = sqr(g - C(rr + 1, cc)) + sqr(g - C(rr - 1, cc + 2));
+                sqm = sqr(g - C(rr - 1, cc)) + sqr(g - C(rr + 1, cc + 2));
+            } else {
+                const float g = C(rr, cc);
+                sqp = sqr(g - C(rr + 1, cc - 1)) + sqr(g - C(rr - 1, cc + 1));
+                sqm = sqr(g - C(rr - 1, cc - 1)) + sqr(g - C(rr + 1, cc + 1));
+                delp = fabsf(C(rr - 1, cc + 2) - C(rr + 1, cc));
+                delm = fabsf(C(rr + 1, cc + 2) - C(rr - 1, cc));
+            }
+        }
+        const int k = ly * FD_W + lane;
+        s_dp[k] = delp;
+        s_dm[k] = delm;
+        s_qp[k] = sqp;
+        s_qm[k] = sqm;
+    }
+    __syncthreads();
 
     // Per red/blue site (one per pair): the zeroed Nyquist flags and the
     // diagonal estimates over [8, TS - 8) (diagonal.rs:98-144).
@@ -318,11 +332,11 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
         const float nw1 = C(rr - 1, cc - 1), nw2 = C(rr - 2, cc - 2);
         const float ne1 = C(rr - 1, cc + 1), ne2 = C(rr - 2, cc + 2);
         const float sw1 = C(rr + 1, cc - 1), sw2 = C(rr + 2, cc - 2);
-        const float dm_c = D(s.dm, rr, p), dp_c = D(s.dp, rr, p);
-        const float dm_se1 = D(s.dm, rr + 1, p + par), dm_se2 = D(s.dm, rr + 2, p + 1);
-        const float dm_nw1 = D(s.dm, rr - 1, p + par - 1), dm_nw2 = D(s.dm, rr - 2, p - 1);
-        const float dp_ne1 = D(s.dp, rr - 1, p + par), dp_ne2 = D(s.dp, rr - 2, p + 1);
-        const float dp_sw1 = D(s.dp, rr + 1, p + par - 1), dp_sw2 = D(s.dp, rr + 2, p - 1);
+        const float dm_c = D(s_dm, rr, p), dp_c = D(s_dp, rr, p);
+        const float dm_se1 = D(s_dm, rr + 1, p + par), dm_se2 = D(s_dm, rr + 2, p + 1);
+        const float dm_nw1 = D(s_dm, rr - 1, p + par - 1), dm_nw2 = D(s_dm, rr - 2, p - 1);
+        const float dp_ne1 = D(s_dp, rr - 1, p + par), dp_ne2 = D(s_dp, rr - 2, p + 1);
+        const float dp_sw1 = D(s_dp, rr + 1, p + par - 1), dp_sw2 = D(s_dp, rr + 2, p - 1);
         // even_kernel (diagonal.rs:46-61) over rows rr - 2 .. rr + 2.
         auto even = [&](const float* q) {
             return GAUSSEVEN[0] * (D(q, rr - 1, p) + D(q, rr, p + par - 1) + D(q, rr, p + par) +
@@ -332,7 +346,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
                                    D(q, rr + 1, p - 1) + D(q, rr + 1, p + 1) +
                                    D(q, rr + 2, p + par - 1) + D(q, rr + 2, p + par));
         };
-        const float rbvarm_even = even(s.qm), rbvarp_even = even(s.qp);
+        const float rbvarm_even = even(s_qm), rbvarp_even = even(s_qp);
         float rbse = diagonal_estimate(centre, se1, se2);
         float rbnw = diagonal_estimate(centre, nw1, nw2);
         float rbne = diagonal_estimate(centre, ne1, ne2);
