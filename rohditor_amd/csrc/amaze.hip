@@ -611,7 +611,10 @@ constexpr int WC_H = WH + 6;                 // CFA and vcd rows r0 - 3 .. r0 + 
 constexpr int WK_H = WH + 4;                 // hcd and dgv rows r0 - 2 .. r0 + WH + 2
 constexpr int WD_W = WW + 4, WD_H = WH + 4;  // dhv: columns c0 - 2 .., rows r0 - 2 ..
 constexpr int WQ_W = WD_W / 2;               // cddiffsq: the red/blue sites of those columns
-static_assert(TS % WW == 0 && TS % WH == 0 && WH % 4 == 0 && WW == 128, "whole blocks");
+// One wavefront per block row: eight wavefronts, so the three blocks whose
+// staging fits a CU's LDS keep six wavefronts per SIMD.
+constexpr int W_WAVES = WH, W_THREADS = 64 * W_WAVES;
+static_assert(TS % WW == 0 && TS % WH == 0 && WW == 128, "whole blocks");
 
 struct alignas(16) WeightsLds {
     float cfa[WC_H * WS_W];
@@ -632,7 +635,7 @@ __device__ __forceinline__ int split_at(int width, int lc) { return (lc & 1) * (
 // zero outside the tile.
 __device__ __forceinline__ void stage_rows(float* dst, const float* plane, int row0, int rows,
                                            int col0, int quads) {
-    for (int k = threadIdx.x; k < rows * quads; k += 256) {
+    for (int k = threadIdx.x; k < rows * quads; k += W_THREADS) {
         const int ly = k / quads, q = k - ly * quads;
         const int rr = row0 + ly, cc = col0 + 4 * q;
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -713,7 +716,7 @@ __device__ __forceinline__ bool weights_site(const WeightsLds& s, float* hvwt, i
     return test > 0.0f;
 }
 
-__global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
+__global__ __launch_bounds__(W_THREADS) void amz_weights(AmazeArgs a) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
     const int c0 = blk.x * WW, r0 = blk.y * WH;
@@ -733,7 +736,7 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     static_assert(WD_W > 128 && WD_W <= 192, "three column slots per lane");
-    for (int ly = wave; ly < WD_H; ly += 4) {
+    for (int ly = wave; ly < WD_H; ly += W_WAVES) {
         const int rr = r0 - 2 + ly;
         const int par = gae(a.pattern, rr);
 #pragma unroll
@@ -748,8 +751,8 @@ __global__ __launch_bounds__(256) void amz_weights(AmazeArgs a) {
     }
     __syncthreads();
     const int p = blk.x * 64 + lane;
-    for (int j = 0; j < WH / 4; ++j) {
-        const int ly = wave * (WH / 4) + j, rr = r0 + ly;
+    {
+        const int ly = wave, rr = r0 + ly;
         const int par = gae(a.pattern, rr);
         const int cc = 2 * p + par;
         bool flagged = false;
@@ -1244,7 +1247,8 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
               hipLaunchKernelGGL(amz_bound_h, dim3((TS - 8 + BH_ROWS - 1) / BH_ROWS, 1, tiles),
                                  dim3(BH_ROWS), 0, s, a, a.clip_point));
     RHD_TIMED(prof, s, "amz_weights", 5 * f + h,
-              hipLaunchKernelGGL(amz_weights, dim3(TS / WW, TS / WH, tiles), blk, 0, s, a));
+              hipLaunchKernelGGL(amz_weights, dim3(TS / WW, TS / WH, tiles), dim3(W_THREADS), 0,
+                                 s, a));
     RHD_TIMED(prof, s, "amz_box", 16ull * BOX_SLOTS * tiles,
               hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
     RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
