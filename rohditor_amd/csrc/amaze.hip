@@ -112,7 +112,10 @@ constexpr int FW = 64, FH = 32;
 constexpr int FC_W = FW + 8, FC_H = FH + 8;      // CFA: columns c0 - 4 .., rows r0 - 4 ..
 constexpr int FG_W = FW + 4, FG_H = FH + 4;      // gradients: columns c0 - 2 .., rows r0 - 2 ..
 constexpr int FD_W = FW / 2 + 2, FD_H = FH + 4;  // diagonal planes: pairs p0 - 1 .., rows r0 - 2 ..
-static_assert(TS % FW == 0 && TS % FH == 0 && FH % 4 == 0, "whole blocks");
+// Eight wavefronts per block: four blocks (the LDS of five) fill a CU's 32
+// wavefront slots.
+constexpr int F_WAVES = 8, F_THREADS = 64 * F_WAVES;
+static_assert(TS % FW == 0 && TS % FH == 0 && FH % F_WAVES == 0, "whole blocks");
 
 // The gradients are read only by the colour differences and the diagonal
 // planes only by the diagonal estimates, which run after them: the two sets
@@ -148,7 +151,7 @@ __device__ __forceinline__ float delhv(const F& f, int r, int c) {
     return sqr(delh) + sqr(delv);
 }
 
-__global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, float clip_pt) {
+__global__ __launch_bounds__(F_THREADS) void amz_front(AmazeArgs a, float clip_pt8, float clip_pt) {
     const uint3 blk = xcd_block();
     const Tile t = tile_of(a, blk.z);
     const int c0 = blk.x * FW, r0 = blk.y * FH, p0 = c0 / 2;
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
                 gain[m][rp] = ch == 0 ? a.forward[0] : (ch == 1 ? a.forward[1] : a.forward[2]);
             }
         }
-        for (int ly = wave; ly < FC_H; ly += 4) {
+        for (int ly = wave; ly < FC_H; ly += F_WAVES) {
             const int rr = r0 - 4 + ly;
             const bool row_in = rr >= 0 && rr < TS;
             const float* src = a.mosaic + (size_t)(row_in ? mirror_whole_near(oy + rr, a.height) : 0) * a.width;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     __syncthreads();
     // Gradients over [2, TS - 2); zero elsewhere, unread.
     static_assert(FG_W > 64 && FG_W <= 128 && FD_W <= 64, "column slots per lane");
-    for (int ly = wave; ly < FG_H; ly += 4) {
+    for (int ly = wave; ly < FG_H; ly += F_WAVES) {
         const int rr = r0 - 2 + ly;
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
@@ -225,8 +228,8 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     // vcd and hcd are zero in the 4-pixel band, where later stages read them
     // but nothing writes them.
     const int cc = c0 + lane;
-    for (int j = 0; j < FH / 4; ++j) {
-        const int rr = r0 + wave * (FH / 4) + j;
+    for (int j = 0; j < FH / F_WAVES; ++j) {
+        const int rr = r0 + wave * (FH / F_WAVES) + j;
         const int i = rr * TS + cc;
         const float cur = C(rr, cc);
         t.full(CFA)[i] = cur;
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
     // overwrite them: the planes over [6, TS - 6) (even columns, one entry
     // per pair); zero elsewhere, unread.
     __syncthreads();
-    for (int ly = wave; ly < FD_H; ly += 4) {
+    for (int ly = wave; ly < FD_H; ly += F_WAVES) {
         const int rr = r0 - 2 + ly;
         if (lane >= FD_W) continue;
         const int cc = c0 - 2 + 2 * lane;
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(256) void amz_front(AmazeArgs a, float clip_pt8, fl
 
     // Per red/blue site (one per pair): the zeroed Nyquist flags and the
     // diagonal estimates over [8, TS - 8) (diagonal.rs:98-144).
-    for (int k = threadIdx.x; k < FH * (FW / 2); k += 256) {
+    for (int k = threadIdx.x; k < FH * (FW / 2); k += F_THREADS) {
         const int rr = r0 + k / (FW / 2), p = p0 + k % (FW / 2);
         const int par = gae(a.pattern, rr);
         const int cc = 2 * p + par, hp = rr * TSH + p;
@@ -1238,8 +1241,8 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
     // tile (f = full-plane floats per tile, h = half-plane floats per tile).
     const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
     RHD_TIMED(prof, s, "amz_front", f + 5 * f + 3 * h + h / 2,
-              hipLaunchKernelGGL(amz_front, dim3(TS / FW, TS / FH, tiles), blk, 0, s, a, clip_pt8,
-                                 a.clip_point));
+              hipLaunchKernelGGL(amz_front, dim3(TS / FW, TS / FH, tiles), dim3(F_THREADS), 0, s,
+                                 a, clip_pt8, a.clip_point));
     RHD_TIMED(prof, s, "amz_bound_v", 2 * f + f,
               hipLaunchKernelGGL(amz_bound_v, dim3(TS / 64, 1, tiles), dim3(64), 0, s, a,
                                  a.clip_point));
