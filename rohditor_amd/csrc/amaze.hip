@@ -803,6 +803,12 @@ __device__ __forceinline__ bool in_area(const Area& ar, int pattern, int rr, int
            cc < ar.end_col;
 }
 
+// The Nyquist vote and area blocks: AR_BLOCK_ROWS rows x 64 pairs of a
+// tile, so that a launch is a few workgroups per CU rather than dozens of
+// short ones.
+constexpr int AR_BLOCK_ROWS = 16;
+static_assert(TS % AR_BLOCK_ROWS == 0 && AR_BLOCK_ROWS % 4 == 0, "whole blocks");
+
 // nyquist.rs:100-118: neighbourhood vote.
 __global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
     const uint3 blk = xcd_block();
@@ -810,62 +816,108 @@ __global__ __launch_bounds__(256) void amz_vote(AmazeArgs a) {
     if (!tile_area(a, blk.z, ar)) return;
     const Tile t = tile_of(a, blk.z);
     const int p = blk.x * 64 + (threadIdx.x & 63);
-    const int rr = blk.y * 4 + (threadIdx.x >> 6);
-    const int cc = 2 * p + gae(a.pattern, rr);
-    if (!in_area(ar, a.pattern, rr, cc)) return;
-    const int i = rr * TS + cc;
     const uint8_t* n = t.nyq();
-    unsigned votes = n[(i - 2 * TS) >> 1] + n[(i - TS - 1) >> 1] + n[(i - TS + 1) >> 1] +
-                     n[(i - 2) >> 1] + n[(i + 2) >> 1] + n[(i + TS - 1) >> 1] +
-                     n[(i + TS + 1) >> 1] + n[(i + 2 * TS) >> 1];
-    t.nyq2()[i >> 1] = votes >= 5 ? 1 : (votes == 4 ? n[i >> 1] : 0);
+#pragma unroll
+    for (int k = 0; k < AR_BLOCK_ROWS / 4; ++k) {
+        const int rr = blk.y * AR_BLOCK_ROWS + 4 * k + (threadIdx.x >> 6);
+        const int cc = 2 * p + gae(a.pattern, rr);
+        if (!in_area(ar, a.pattern, rr, cc)) continue;
+        const int i = rr * TS + cc;
+        unsigned votes = n[(i - 2 * TS) >> 1] + n[(i - TS - 1) >> 1] + n[(i - TS + 1) >> 1] +
+                         n[(i - 2) >> 1] + n[(i + 2) >> 1] + n[(i + TS - 1) >> 1] +
+                         n[(i + TS + 1) >> 1] + n[(i + 2 * TS) >> 1];
+        t.nyq2()[i >> 1] = votes >= 5 ? 1 : (votes == 4 ? n[i >> 1] : 0);
+    }
 }
 
 // nyquist.rs:120-152: area interpolation of hvwt at the sites the vote
-// kept (about a tenth of them). A workgroup first lists its kept sites in
-// LDS, then its lanes take them densely, so the 7x7 gathers run in full
-// wavefronts; each site's result is independent of the order.
+// kept (about a tenth of them). A workgroup (AR_BLOCK_ROWS rows x 64 pairs)
+// first lists its kept sites in LDS; when it has any, it stages the CFA
+// samples and the vote flags its 7x7 stencils reach, then its lanes take the
+// listed sites densely, so the gathers run from LDS in full wavefronts; each
+// site's result is independent of the order.
+//   CFA: rows r0 - 7 .. r0 + AR_BLOCK_ROWS + 6, columns c_base .. c_base + 143
+//        (c_base = 128 blk.x - 8): the taps and their four neighbours.
+//   flags: rows r0 - 6 .. r0 + AR_BLOCK_ROWS + 5, pairs p_base .. p_base + 71
+//        (p_base = 64 blk.x - 4).
+// Both read as zero outside the tile, where no valid site's stencil reaches.
+constexpr int AR_ROWS = AR_BLOCK_ROWS + 14, AR_COLS = 144;
+constexpr int AR_FROWS = AR_BLOCK_ROWS + 12, AR_FPAIRS = 72;
+constexpr int AR_SITES = AR_BLOCK_ROWS * 64;
+
 __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
     const uint3 blk = xcd_block();
     Area ar;
     if (!tile_area(a, blk.z, ar)) return;
     const Tile t = tile_of(a, blk.z);
-    __shared__ int s_sites[256];
+    __shared__ int s_sites[AR_SITES];
     __shared__ int s_count;
+    __shared__ __align__(16) float s_cfa[AR_ROWS * AR_COLS];
+    __shared__ __align__(4) uint8_t s_flag[AR_FROWS * AR_FPAIRS];
     if (threadIdx.x == 0) s_count = 0;
     __syncthreads();
+    const int r0 = blk.y * AR_BLOCK_ROWS;
     {
         const int p = blk.x * 64 + (threadIdx.x & 63);
-        const int rr = blk.y * 4 + (threadIdx.x >> 6);
-        const int cc = 2 * p + gae(a.pattern, rr);
-        if (in_area(ar, a.pattern, rr, cc) && t.nyq2()[(rr * TS + cc) >> 1] != 0)
-            s_sites[atomicAdd(&s_count, 1)] = rr * TS + cc;
+        bool kept[AR_BLOCK_ROWS / 4];
+#pragma unroll
+        for (int k = 0; k < AR_BLOCK_ROWS / 4; ++k) {
+            const int rr = r0 + 4 * k + (threadIdx.x >> 6);
+            const int cc = 2 * p + gae(a.pattern, rr);
+            kept[k] = in_area(ar, a.pattern, rr, cc) && t.nyq2()[(rr * TS + cc) >> 1] != 0;
+        }
+#pragma unroll
+        for (int k = 0; k < AR_BLOCK_ROWS / 4; ++k) {
+            const int rr = r0 + 4 * k + (threadIdx.x >> 6);
+            if (kept[k]) s_sites[atomicAdd(&s_count, 1)] = rr * TS + 2 * p + gae(a.pattern, rr);
+        }
     }
     __syncthreads();
     const int n = s_count;
-    const uint8_t* n2 = t.nyq2();
+    if (n == 0) return;
+    const int c_base = blk.x * 128 - 8, p_base = blk.x * 64 - 4;
     const float* cfa = t.full(CFA);
+    for (int k = threadIdx.x; k < AR_ROWS * (AR_COLS / 4); k += 256) {
+        const int ly = k / (AR_COLS / 4), q = k - ly * (AR_COLS / 4);
+        const int rr = r0 - 7 + ly, cc = c_base + 4 * q;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (rr >= 0 && rr < TS && cc >= 0 && cc < TS)
+            v = *reinterpret_cast<const float4*>(cfa + rr * TS + cc);
+        *reinterpret_cast<float4*>(s_cfa + ly * AR_COLS + 4 * q) = v;
+    }
+    const uint8_t* n2 = t.nyq2();
+    for (int k = threadIdx.x; k < AR_FROWS * (AR_FPAIRS / 4); k += 256) {
+        const int ly = k / (AR_FPAIRS / 4), q = k - ly * (AR_FPAIRS / 4);
+        const int rr = r0 - 6 + ly, pp = p_base + 4 * q;
+        unsigned v = 0;
+        if (rr >= 0 && rr < TS && pp >= 0 && pp < TSH)
+            v = *reinterpret_cast<const unsigned*>(n2 + rr * TSH + pp);
+        *reinterpret_cast<unsigned*>(s_flag + ly * AR_FPAIRS + 4 * q) = v;
+    }
+    __syncthreads();
     for (int k = threadIdx.x; k < n; k += 256) {
         const int i = s_sites[k];
-        // Row by row of taps: the row's flags, then the flagged taps' loads
-        // (all in flight together), then the sums in the reference's order.
+        const int rr = i / TS, cc = i % TS;
+        // Row by row of taps: the row's flags, then the flagged taps' samples,
+        // then the sums in the reference's order.
         float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
         for (int di = -6; di < 7; di += 2) {
-            const int row = i + di * TS - 6;
+            const uint8_t* frow = s_flag + (rr + di - (r0 - 6)) * AR_FPAIRS - p_base;
+            const float* crow = s_cfa + (rr + di - (r0 - 7)) * AR_COLS - c_base;
             bool f[7];
             float c[7], l[7], r[7], u[7], d[7];
 #pragma unroll
-            for (int j = 0; j < 7; ++j) f[j] = n2[(row + 2 * j) >> 1] != 0;
+            for (int j = 0; j < 7; ++j) f[j] = frow[(cc - 6 + 2 * j) >> 1] != 0;
 #pragma unroll
             for (int j = 0; j < 7; ++j) {
-                const int i1 = row + 2 * j;
+                const int col = cc - 6 + 2 * j;
                 c[j] = l[j] = r[j] = u[j] = d[j] = 0.0f;
                 if (f[j]) {
-                    c[j] = cfa[i1];
-                    l[j] = cfa[i1 - 1];
-                    r[j] = cfa[i1 + 1];
-                    u[j] = cfa[i1 - TS];
-                    d[j] = cfa[i1 + TS];
+                    c[j] = crow[col];
+                    l[j] = crow[col - 1];
+                    r[j] = crow[col + 1];
+                    u[j] = crow[col - AR_COLS];
+                    d[j] = crow[col + AR_COLS];
                 }
             }
 #pragma unroll
@@ -1236,7 +1288,7 @@ size_t amaze_scratch_bytes_per_tile() { return TILE_FLOATS * sizeof(float) + 4 *
 // a.tile_base; a.scratch and a.areas hold one slot per tile of this launch.
 hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* prof) {
     const float clip_pt8 = 0.8f * a.clip_point;
-    const dim3 pair_grid(TSH / 64, TS / 4, tiles), blk(256);
+    const dim3 blk(256);
     // Algorithmic bytes: the planes each stage must read and write once, per
     // tile (f = full-plane floats per tile, h = half-plane floats per tile).
     const uint64_t f = 4ull * FULL * tiles, h = 4ull * HALF * tiles;
@@ -1254,8 +1306,9 @@ hipError_t launch_amaze(const AmazeArgs& a, int tiles, hipStream_t s, Profiler* 
                                  s, a));
     RHD_TIMED(prof, s, "amz_box", 16ull * BOX_SLOTS * tiles,
               hipLaunchKernelGGL(amz_box, dim3(tiles), blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, pair_grid, blk, 0, s, a));
-    RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, pair_grid, blk, 0, s, a));
+    const dim3 area_grid(TSH / 64, TS / AR_BLOCK_ROWS, tiles);
+    RHD_TIMED(prof, s, "amz_vote", h / 2, hipLaunchKernelGGL(amz_vote, area_grid, blk, 0, s, a));
+    RHD_TIMED(prof, s, "amz_area", h / 4, hipLaunchKernelGGL(amz_area, area_grid, blk, 0, s, a));
     // The back end reads CFA, vcd, hcd (full planes), hvwt, pmwt, rbm, rbp
     // and the Nyquist flags, and writes the RGB core of the tile.
     RHD_TIMED(prof, s, "amz_back",
