@@ -310,17 +310,32 @@ __global__ __launch_bounds__(THREADS) void opp_label_tiles(Layout L) {
         s_par[k] = fg ? (unsigned)k : NONE;
     }
     __syncthreads();
+    // Unions with the visited in-tile neighbours, leaving out those that
+    // reach the cell through another union: with N in the mask, NW and NE
+    // are N's own W and E neighbours; otherwise W, when in the mask, has NW
+    // as its N. By induction over the rows every 8-adjacent pair still ends
+    // in one set, so the components (and their smallest-index roots) are
+    // those of all four unions. Neighbours in other tiles are left to
+    // opp_label_borders, which takes all of them.
     for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
         if (s_par[k] == NONE) continue;
         const int lx = k % LX, ly = k / LX;
-        const int nx[4] = {-1, -1, 0, 1}, ny[4] = {0, -1, -1, -1};
-        for (int q = 0; q < 4; ++q) {
-            const int x = lx + nx[q], y = ly + ny[q];
-            if (x < 0 || y < 0 || x >= LX) continue;
-            const unsigned j = (unsigned)(y * LX + x);
-            if (load_parent<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, j) != NONE)
-                merge<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, (unsigned)k, j);
+        auto in_mask = [&](int x, int y) {
+            return x >= 0 && y >= 0 && x < LX &&
+                   load_parent<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, (unsigned)(y * LX + x)) != NONE;
+        };
+        auto unite = [&](int x, int y) {
+            merge<__HIP_MEMORY_SCOPE_WORKGROUP>(s_par, (unsigned)k, (unsigned)(y * LX + x));
+        };
+        if (in_mask(lx, ly - 1)) {
+            unite(lx, ly - 1);
+            continue;
         }
+        if (in_mask(lx - 1, ly))
+            unite(lx - 1, ly);
+        else if (in_mask(lx - 1, ly - 1))
+            unite(lx - 1, ly - 1);
+        if (in_mask(lx + 1, ly - 1)) unite(lx + 1, ly - 1);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < LY * LX; k += THREADS) {
