@@ -17,6 +17,8 @@
 // between two estimate planes. Per iteration the estimate (with halo) and
 // the guide are read and the estimate written; the iterations are bound by
 // vector-ALU issue (the blur taps and the ratio's division), not by HBM.
+// Tiles at the image edge run the same strip code as interior ones on
+// windows staged half-sample mirrored, so no tap computes a mirrored index.
 #include <type_traits>
 
 #include "rhd_internal.hpp"
@@ -52,52 +54,17 @@ __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rg
 // so the first product p is >= +0 and 0.0 + p == -0.0 + p == p exactly: the
 // sums start from the first product instead, one add fewer per output.
 
-// Horizontal taps at image column x of an LDS row whose first element is
-// image column `col0`.
-template <int R, bool FAST>
-__device__ __forceinline__ float hblur(const float* row, int col0, int x, int width,
-                                       const Weights& k) {
-    float v = row[(FAST ? x - R : mirror_half(x - R, width)) - col0] * k.w[0];
-#pragma unroll
-    for (int t = 1; t <= 2 * R; ++t) {
-        const int src = FAST ? x + t - R : mirror_half(x + t - R, width);
-        v += row[src - col0] * k.w[t];
-    }
-    return v;
-}
-
-// Vertical taps at image row y over an LDS plane whose first row is image row
-// `row0` (pitch `pitch`), at LDS column `c`.
-template <int R, bool FAST>
-__device__ __forceinline__ float vblur(const float* plane, int pitch, int row0, int y, int c,
-                                       int height, const Weights& k) {
-    float v = plane[((FAST ? y - R : mirror_half(y - R, height)) - row0) * pitch + c] * k.w[0];
-#pragma unroll
-    for (int t = 1; t <= 2 * R; ++t) {
-        const int src = FAST ? y + t - R : mirror_half(y + t - R, height);
-        v += plane[(src - row0) * pitch + c] * k.w[t];
-    }
-    return v;
-}
-
-// Threads walk a (rows x cols) window in row-major order, THREADS elements
-// per step, so every lane has work whatever the window width; the (row,
-// column) position advances by a fixed step with one carry.
-#define FOR_WINDOW(rows, cols, r, c)                                                    \
-    for (int r = (int)threadIdx.x / (cols), c = (int)threadIdx.x % (cols),              \
-             r##_step = THREADS / (cols), c##_step = THREADS % (cols);                  \
-         r < (rows); r += r##_step, c += c##_step, (c >= (cols) ? (c -= (cols), ++r) : 0))
-
-// Copies a (rows x cols) window of a global plane into LDS. The trip counts
-// are compile-time bounds (MAXROWS x MAXCOLS), so every load of the window is
-// issued before the first LDS store and the memory latency is paid once.
-// A wavefront stages whole rows, so each row's address is wave-uniform and
-// formed in scalar registers.
+// A (rows x cols) window of a global plane held in registers, one value per
+// lane per slot. The trip counts are compile-time bounds (MAXROWS x MAXCOLS),
+// so every load of the window is issued before the first LDS store and the
+// memory latency is paid once. A wavefront stages whole rows, so each row's
+// address is wave-uniform and formed in scalar registers.
 template <int MAXROWS, int MAXCOLS>
 struct Staged {
     static constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
     float v[RI * CI];
 
+    // The window whose first element is src (inside the image).
     __device__ __forceinline__ void load(const float* __restrict__ src, size_t src_pitch, int rows,
                                          int cols) {
         const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
@@ -109,6 +76,26 @@ struct Staged {
             for (int j = 0; j < CI; ++j) {
                 const int c = tx + 64 * j;
                 v[i * CI + j] = (r < rows && c < cols) ? row[c] : 0.0f;
+            }
+        }
+    }
+    // The window whose first element is image position (top, left), every
+    // position half-sample mirrored into the image (sharpening.rs:318-330).
+    __device__ __forceinline__ void load_mirrored(const float* __restrict__ plane, int width,
+                                                  int height, int top, int left, int rows,
+                                                  int cols) {
+        const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
+        int col[CI];
+#pragma unroll
+        for (int j = 0; j < CI; ++j) col[j] = mirror_near(left + tx + 64 * j, width);
+#pragma unroll
+        for (int i = 0; i < RI; ++i) {
+            const int r = ty + 4 * i;
+            const float* row = plane + (size_t)mirror_near(top + r, height) * width;
+#pragma unroll
+            for (int j = 0; j < CI; ++j) {
+                const int c = tx + 64 * j;
+                v[i * CI + j] = (r < rows && c < cols) ? row[col[j]] : 0.0f;
             }
         }
     }
@@ -124,167 +111,30 @@ struct Staged {
     }
 };
 
-template <int MAXROWS, int MAXCOLS>
-__device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ src,
-                                      size_t src_pitch, int rows, int cols) {
-    Staged<MAXROWS, MAXCOLS> w;
-    w.load(src, src_pitch, rows, cols);
-    w.store(lds, pitch, rows, cols);
-}
-
-// One blur of `in` with an epilogue:
-//   mode 0: out = min(aux, b^4)                          (sharpening.rs:227-237)
-//   mode 1: out = aux * detail gate(guide = in, b)       (sharpening.rs:238-262)
-template <int R, bool FAST>
-__device__ void blur_tile(const float* __restrict__ in, const float* __restrict__ aux,
-                          float* __restrict__ out, int width, int height, const Weights& k,
-                          int mode, float noise_protection, float* s_in, float* s_h) {
-    const uint3 blk = xcd_block();
-    const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const int cx0 = max(0, x0 - R), cx1 = min(width, x0 + TW + R);
-    const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
-    const int pw = cx1 - cx0, ph = ry1 - ry0;
-    const int ow = min(width, x0 + TW) - x0, oh = min(height, y0 + TH) - y0;
-    stage<TH + 2 * MAXR, TW + 2 * MAXR>(s_in, pw, in + (size_t)ry0 * width + cx0, width, ph, pw);
-    __syncthreads();
-    FOR_WINDOW(ph, ow, r, c) s_h[r * TW + c] = hblur<R, FAST>(s_in + r * pw, cx0, x0 + c, width, k);
-    __syncthreads();
-    FOR_WINDOW(oh, ow, r, c) {
-        const int y = y0 + r, x = x0 + c;
-        const float b = vblur<R, FAST>(s_h, TW, ry0, y, c, height, k);
-        const size_t i = (size_t)y * width + x;
-        if (mode == 0) {
-            out[i] = num_min(aux[i], (b * b) * (b * b));
-        } else {
-            const float g = in[i];
-            const float threshold = 0.0005f + noise_protection * 0.008f + b * noise_protection * 0.02f;
-            const float detail = fabsf(g - b);
-            out[i] = aux[i] * (smoothstep(threshold, threshold * 3.0f, detail) *
-                               smoothstep(0.001f, 0.01f, g));
-        }
-    }
-}
-
-template <int R>
-__device__ void blur_interior(const float* __restrict__ in, const float* __restrict__ aux,
-                              float* __restrict__ out, int width, const Weights& k, int mode,
-                              float noise_protection, float* s_in, float* s_h);
-
-template <int R>
-__global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
-                                                    const float* __restrict__ aux,
-                                                    float* __restrict__ out, int width,
-                                                    int height, Weights k, int mode,
-                                                    float noise_protection) {
-    __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
-    __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
-    const uint3 blk = xcd_block();
-    const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const bool fast = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
-    if (fast)
-        blur_interior<R>(in, aux, out, width, k, mode, noise_protection, s_in, s_h);
+// Copies the (ROWS x COLS) window at image position (top, left) into LDS,
+// mirrored at the image edges when BORDER.
+template <int ROWS, int COLS, bool BORDER>
+__device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ plane,
+                                      int width, int height, int top, int left) {
+    Staged<ROWS, COLS> w;
+    if (BORDER)
+        w.load_mirrored(plane, width, height, top, left, ROWS, COLS);
     else
-        blur_tile<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+        w.load(plane + (size_t)top * width + left, width, ROWS, COLS);
+    w.store(lds, pitch, ROWS, COLS);
 }
 
-// One RL iteration (sharpening.rs:268-304) for a 64x16 tile: blur(estimate)
-// over the tile widened by r, ratio = guide / max(b, floor), blur(ratio) over
-// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide). Two LDS
-// arrays sized for the radius: the first holds the estimate with a 2r halo
-// and, once its horizontal blur is done, the guide with an r halo, which the
-// ratio overwrites in place; the second holds the horizontal blurs. Sharing
-// the first array keeps eight workgroups (eight waves per SIMD) resident.
+// Window geometry for radius R. Blurs and the RL iteration compute strips of
+// four outputs per thread (4 columns of a row in the horizontal passes, read
+// as float4 chunks; 4 rows of a column in the vertical ones), so loads,
+// addressing and loop control are shared by 4 results; every output keeps
+// the reference's tap order. Strips that reach past a window's edge compute
+// throw-away values from the padding and are not stored.
 template <int R>
-struct IterWindows {
-    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;  // estimate
-    static constexpr int HW = TW + 2 * R;                   // horizontal blurs
-    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;  // guide / ratio
-};
-
-// The gain apply (sharpening.rs:105-126) at pixel i of planar RGB with n
-// pixels per plane: gain = 1 + amount * mask * (estimate / guide - 1), and
-// the pixel is scaled only when the gain differs from 1 and every scaled
-// channel stays finite.
-struct Apply {
-    float* rgb;
-    const float* mask;
-    float amount;
-    size_t n;
-};
-__device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est, float guide) {
-    const float gain = 1.0f + ap.amount * ap.mask[i] * (est / guide - 1.0f);
-    if (gain == 1.0f) return;
-    const float r = ap.rgb[i], g = ap.rgb[ap.n + i], b = ap.rgb[2 * ap.n + i];
-    if (is_finite(r * gain) && is_finite(g * gain) && is_finite(b * gain)) {
-        ap.rgb[i] = r * gain;
-        ap.rgb[ap.n + i] = g * gain;
-        ap.rgb[2 * ap.n + i] = b * gain;
-    }
-}
-
-// APPLY: the last iteration applies the gain with its estimate instead of
-// storing the estimate.
-template <int R, bool FAST, bool APPLY>
-__device__ void iterate_tile(const float* __restrict__ est_in, const float* __restrict__ guide,
-                             float* __restrict__ est_out, int width, int height, const Weights& k,
-                             const Apply& ap, float* s_a, float* s_b) {
-    using Win = IterWindows<R>;
-    constexpr int EW = Win::EW, HW = Win::HW, GW = Win::GW;
-    const uint3 blk = xcd_block();
-    const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const int ox1 = min(width, x0 + TW), oy1 = min(height, y0 + TH);
-    const int ex0 = max(0, x0 - 2 * R), ex1 = min(width, x0 + TW + 2 * R);
-    const int ey0 = max(0, y0 - 2 * R), ey1 = min(height, y0 + TH + 2 * R);
-    const int rx0 = max(0, x0 - R), rx1 = min(width, x0 + TW + R);
-    const int ry0 = max(0, y0 - R), ry1 = min(height, y0 + TH + R);
-    const int ew = ex1 - ex0, eh = ey1 - ey0, rw = rx1 - rx0, rh = ry1 - ry0;
-    const int ow = ox1 - x0, oh = oy1 - y0;
-    // The guide window waits in registers while the estimate window, which
-    // shares its LDS, is blurred.
-    Staged<Win::GH, Win::GW> g;
-    g.load(guide + (size_t)ry0 * width + rx0, width, rh, rw);
-    stage<Win::EH, Win::EW>(s_a, EW, est_in + (size_t)ey0 * width + ex0, width, eh, ew);
-    __syncthreads();
-    FOR_WINDOW(eh, rw, r, c) s_b[r * HW + c] = hblur<R, FAST>(s_a + r * EW, ex0, rx0 + c, width, k);
-    __syncthreads();
-    g.store(s_a, GW, rh, rw);
-    __syncthreads();
-    FOR_WINDOW(rh, rw, r, c) {
-        const float b = vblur<R, FAST>(s_b, HW, ey0, ry0 + r, c, height, k);
-        s_a[r * GW + c] = s_a[r * GW + c] / num_max(b, FLOOR);
-    }
-    __syncthreads();
-    FOR_WINDOW(rh, ow, r, c) s_b[r * HW + c] = hblur<R, FAST>(s_a + r * GW, rx0, x0 + c, width, k);
-    __syncthreads();
-    FOR_WINDOW(oh, ow, r, c) {
-        const int y = y0 + r, x = x0 + c;
-        const float b = vblur<R, FAST>(s_b, HW, ry0, y, c, height, k);
-        const size_t i = (size_t)y * width + x;
-        const float g = guide[i];
-        const float v = est_in[i] * b;
-        const float lo = g * 0.5f, hi = g * 2.0f;
-        const float est = v < lo ? lo : (v > hi ? hi : v);
-        if (APPLY)
-            apply_gain(ap, i, est, g);
-        else
-            est_out[i] = est;
-    }
-}
-
-// The same iteration for a tile whose windows lie inside the image: every
-// horizontal tap is interior (the sum starts at 0.0), no mirroring, and all
-// LDS offsets are compile-time constants. Each thread computes strips of 4
-// outputs (4 columns of a row in the horizontal passes, read as float4
-// chunks; 4 rows of a column in the vertical ones), so loads, addressing and
-// loop control are shared by 4 results; every output keeps the reference's
-// tap order. Strips that reach past a window's edge compute throw-away
-// values from the padding and are not stored. The guide values of the tile's
-// outputs are kept in registers before the ratio overwrites the guide window.
-template <int R>
-struct InteriorWindows {
-    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;   // estimate
-    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;   // guide / ratio
-    static constexpr int P = (GW + 3) / 4 * 4;              // pitch of s_h and s_g
+struct Windows {
+    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;   // estimate (2r halo)
+    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;   // guide / ratio (r halo)
+    static constexpr int P = (GW + 3) / 4 * 4;              // pitch of the r-halo windows
     static constexpr int E_FLOATS = EH * EW + 8;
     static constexpr int H_FLOATS = (EH + 4) * P;
     static constexpr int G_FLOATS = GH * P;
@@ -296,7 +146,7 @@ struct InteriorWindows {
 // 4 horizontal outputs from the row segment starting at `src` (16-byte aligned).
 template <int R>
 __device__ __forceinline__ float4 hstrip(const float* src, const Weights& k) {
-    constexpr int N = InteriorWindows<R>::CHUNKS * 4;
+    constexpr int N = Windows<R>::CHUNKS * 4;
     float x[N];
 #pragma unroll
     for (int q = 0; q < N / 4; ++q) {
@@ -332,21 +182,49 @@ __device__ __forceinline__ void vstrip(const float* src, const Weights& k, float
     }
 }
 
-// blur_tile for a tile whose r halo lies inside the image (no mirroring, the
-// horizontal sums start from 0.0): strips of four outputs per thread, as in
-// the RL iteration.
-template <int R>
-__device__ void blur_interior(const float* __restrict__ in, const float* __restrict__ aux,
-                              float* __restrict__ out, int width, const Weights& k, int mode,
-                              float noise_protection, float* s_in, float* s_h) {
+// Border tiles stage their windows half-sample mirrored, so every window row
+// holds an image row and every horizontal or vertical result at an image
+// position is the reference's. A result at a window position outside the
+// image (within r of it) is not the reference's value there; the windows
+// that later passes read across the edge are repaired by copying each such
+// position from its mirrored image position: whole rows first, then columns,
+// so corners take the doubly mirrored value. Sources lie inside the image and
+// destinations outside, so no copy reads another copy's output.
+template <int R, int ROWS, int COLS>
+__device__ void mirror_window_rows(float* s, int pitch, int top, int height) {
+    for (int item = threadIdx.x; item < 2 * R * COLS; item += THREADS) {
+        const int k = item / COLS, c = item - k * COLS;
+        const int y = k < R ? k - R : height + (k - R);
+        const int r = y - top, src = mirror_near(y, height) - top;
+        if (r >= 0 && r < ROWS && src >= 0 && src < ROWS) s[r * pitch + c] = s[src * pitch + c];
+    }
+}
+template <int R, int ROWS, int COLS>
+__device__ void mirror_window_cols(float* s, int pitch, int left, int width) {
+    for (int item = threadIdx.x; item < 2 * R * ROWS; item += THREADS) {
+        const int r = item / (2 * R), k = item - r * (2 * R);
+        const int x = k < R ? k - R : width + (k - R);
+        const int c = x - left, src = mirror_near(x, width) - left;
+        if (c >= 0 && c < COLS && src >= 0 && src < COLS) s[r * pitch + c] = s[r * pitch + src];
+    }
+}
+
+// One blur of `in` over a 64x16 tile with an epilogue:
+//   mode 0: out = min(aux, b^4)                          (sharpening.rs:227-237)
+//   mode 1: out = aux * detail gate(guide = in, b)       (sharpening.rs:238-262)
+// Only the tile's columns are blurred horizontally, over every window row, so
+// a border tile needs no repair: its mirrored rows are image rows.
+template <int R, bool BORDER>
+__device__ void blur_strips(const float* __restrict__ in, const float* __restrict__ aux,
+                            float* __restrict__ out, int width, int height, const Weights& k,
+                            int mode, float noise_protection, float* s_in, float* s_h) {
     constexpr int W = TW + 2 * R, H = TH + 2 * R, P = (W + 3) / 4 * 4;
-    static_assert(4 * (TW / 4 - 1) + InteriorWindows<R>::CHUNKS * 4 <= P, "strips stay in a row");
+    static_assert(4 * (TW / 4 - 1) + Windows<R>::CHUNKS * 4 <= P, "strips stay in a row");
     static_assert(H * P <= (TH + 2 * MAXR) * (TW + 2 * MAXR), "the window fits s_in");
     static_assert(TH * TW == 4 * THREADS, "one vertical strip per thread");
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    stage<TH + 2 * MAXR, TW + 2 * MAXR>(s_in, P, in + (size_t)(y0 - R) * width + x0 - R, width, H,
-                                       W);
+    stage<H, W, BORDER>(s_in, P, in, width, height, y0 - R, x0 - R);
     __syncthreads();
     for (int item = threadIdx.x; item < H * (TW / 4); item += THREADS) {
         const int r = item / (TW / 4), j = item % (TW / 4);
@@ -359,7 +237,9 @@ __device__ void blur_interior(const float* __restrict__ in, const float* __restr
     vstrip<R, TW>(s_h + r0 * TW + c, k, b);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const size_t i = (size_t)(y0 + r0 + q) * width + x0 + c;
+        const int y = y0 + r0 + q, x = x0 + c;
+        if (BORDER && (y >= height || x >= width)) continue;
+        const size_t i = (size_t)y * width + x;
         if (mode == 0) {
             out[i] = num_min(aux[i], (b[q] * b[q]) * (b[q] * b[q]));
         } else {
@@ -373,11 +253,59 @@ __device__ void blur_interior(const float* __restrict__ in, const float* __restr
     }
 }
 
-template <int R, bool APPLY>
-__device__ void iterate_interior(const float* __restrict__ est_in, const float* __restrict__ guide,
-                                 float* __restrict__ est_out, int width, const Weights& k,
-                                 const Apply& ap, float* s_a, float* s_b) {
-    using Win = InteriorWindows<R>;
+template <int R>
+__global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
+                                                    const float* __restrict__ aux,
+                                                    float* __restrict__ out, int width,
+                                                    int height, Weights k, int mode,
+                                                    float noise_protection) {
+    __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
+    __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    const bool interior = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
+    if (interior)
+        blur_strips<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+    else
+        blur_strips<R, true>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+}
+
+// The gain apply (sharpening.rs:105-126) at pixel i of planar RGB with n
+// pixels per plane: gain = 1 + amount * mask * (estimate / guide - 1), and
+// the pixel is scaled only when the gain differs from 1 and every scaled
+// channel stays finite.
+struct Apply {
+    float* rgb;
+    const float* mask;
+    float amount;
+    size_t n;
+};
+__device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est, float guide) {
+    const float gain = 1.0f + ap.amount * ap.mask[i] * (est / guide - 1.0f);
+    if (gain == 1.0f) return;
+    const float r = ap.rgb[i], g = ap.rgb[ap.n + i], b = ap.rgb[2 * ap.n + i];
+    if (is_finite(r * gain) && is_finite(g * gain) && is_finite(b * gain)) {
+        ap.rgb[i] = r * gain;
+        ap.rgb[ap.n + i] = g * gain;
+        ap.rgb[2 * ap.n + i] = b * gain;
+    }
+}
+
+// One RL iteration (sharpening.rs:268-304) for a 64x16 tile: blur(estimate)
+// over the tile widened by r, ratio = guide / max(b, floor), blur(ratio) over
+// the tile, estimate = clamp(estimate * b, guide / 2, 2 guide). Two LDS
+// arrays: the first holds the estimate with a 2r halo and, once its
+// horizontal blur is done, the guide with an r halo, which the ratio
+// overwrites in place; the second holds the horizontal blurs. Sharing the
+// first array keeps eight workgroups (eight waves per SIMD) resident. The
+// guide values of the tile's outputs are kept in registers before the ratio
+// overwrites the guide window. APPLY: the last iteration applies the gain
+// with its estimate instead of storing the estimate.
+template <int R, bool APPLY, bool BORDER>
+__device__ void iterate_strips(const float* __restrict__ est_in, const float* __restrict__ guide,
+                               float* __restrict__ est_out, int width, int height,
+                               const Weights& k, const Apply& ap, float* s_a, float* s_b) {
+    using Win = Windows<R>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
     constexpr int NS1 = (GW + 3) / 4;   // column strips of the estimate's blur
     constexpr int NR2 = (GH + 3) / 4;   // row strips of the blurred estimate
@@ -391,8 +319,11 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
     // The guide window waits in registers while the estimate window, which
     // shares its LDS, is blurred.
     Staged<GH, GW> gw;
-    gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
-    stage<EH, EW>(s_a, EW, est_in + (size_t)(y0 - 2 * R) * width + x0 - 2 * R, width, EH, EW);
+    if (BORDER)
+        gw.load_mirrored(guide, width, height, y0 - R, x0 - R, GH, GW);
+    else
+        gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
+    stage<EH, EW, BORDER>(s_a, EW, est_in, width, height, y0 - 2 * R, x0 - 2 * R);
     __syncthreads();
     float e_out[OUT_STRIPS][4], g_out[OUT_STRIPS][4];
 #pragma unroll
@@ -424,6 +355,13 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
             if (r0 + i < GH) s_a[(r0 + i) * P + c] = s_a[(r0 + i) * P + c] / num_max(b[i], FLOOR);
     }
     __syncthreads();
+    if (BORDER) {
+        // The ratio's blur reads up to r positions across the image edge.
+        mirror_window_rows<R, GH, GW>(s_a, P, y0 - R, height);
+        __syncthreads();
+        mirror_window_cols<R, GH, GW>(s_a, P, x0 - R, width);
+        __syncthreads();
+    }
     for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
         const int r = item / NS3, j = item % NS3;
         *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
@@ -439,10 +377,13 @@ __device__ void iterate_interior(const float* __restrict__ est_in, const float* 
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
+            if (BORDER && (y0 + r0 + i >= height || x0 + c >= width)) continue;
             const float g = g_out[q][i];
             const float v = e_out[q][i] * b[i];
-            const float lo = g * 0.5f, hi = g * 2.0f;
-            const float est = v < lo ? lo : (v > hi ? hi : v);
+            // clamp(v, g / 2, 2 g): g >= the floor, so the bounds are ordered,
+            // and v is a product of positive values, never NaN for finite
+            // input; the median of three is then the reference's clamp.
+            const float est = __builtin_amdgcn_fmed3f(v, g * 0.5f, g * 2.0f);
             if (APPLY)
                 apply_gain(ap, o + (size_t)i * width, est, g);
             else
@@ -456,21 +397,17 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
                                                        int height, Weights k, Apply ap) {
-    using Win = IterWindows<R>;
-    using IWin = InteriorWindows<R>;
-    __shared__ __align__(16) float s_a[IWin::A_FLOATS];
-    __shared__ __align__(16) float s_b[IWin::H_FLOATS];
-    static_assert(IWin::A_FLOATS >= Win::EH * Win::EW && IWin::A_FLOATS >= Win::GH * Win::GW &&
-                      IWin::H_FLOATS >= Win::EH * Win::HW,
-                  "the border path's windows fit the interior path's arrays");
+    using Win = Windows<R>;
+    __shared__ __align__(16) float s_a[Win::A_FLOATS];
+    __shared__ __align__(16) float s_b[Win::H_FLOATS];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool interior =
         x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
     if (interior)
-        iterate_interior<R, APPLY>(est_in, guide, est_out, width, k, ap, s_a, s_b);
+        iterate_strips<R, APPLY, false>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
     else
-        iterate_tile<R, false, APPLY>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
+        iterate_strips<R, APPLY, true>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
 }
 
 // sharpening.rs:105-126
