@@ -9,12 +9,13 @@
 // the mirrored horizontal border) and half-sample mirroring.
 //
 // Kernels: init (guide, highlight mask; the AMaZE finish writes both instead
-// when capture directly follows it), two single blurs with the mask
-// epilogues, one fused kernel per RL iteration (blur estimate -> ratio ->
-// blur ratio -> clamped update, all in LDS for a 64x16 output tile with a
-// 2r halo), the last of which applies the gain to RGB instead of storing its
-// estimate, and a plain gain apply for zero iterations. Iterations ping-pong
-// between two estimate planes. Per iteration the estimate (with halo) and
+// when capture directly follows it), prep (the mask's two blurs and
+// epilogues, and the first iteration's ratio from the same blur of the
+// guide), one fused kernel per RL iteration (blur estimate -> ratio -> blur
+// ratio -> clamped update, all in LDS for a 64x16 output tile with a 2r
+// halo; the first starts from prep's ratio), the last of which applies the
+// gain to RGB instead of storing its estimate, and a plain gain apply for
+// zero iterations. Iterations ping-pong between two estimate planes. Per iteration the estimate (with halo) and
 // the guide are read and the estimate written; the iterations are bound by
 // vector-ALU issue (the blur taps and the ratio's division), not by HBM.
 // Tiles at the image edge run the same strip code as interior ones on
@@ -209,65 +210,89 @@ __device__ void mirror_window_cols(float* s, int pitch, int left, int width) {
     }
 }
 
-// One blur of `in` over a 64x16 tile with an epilogue:
-//   mode 0: out = min(aux, b^4)                          (sharpening.rs:227-237)
-//   mode 1: out = aux * detail gate(guide = in, b)       (sharpening.rs:238-262)
+// The highlight mask's two blurs (sharpening.rs:227-262) in one pass over a
+// 64x16 tile, and the first RL ratio. The estimate starts as the guide
+// (sharpening.rs:219), so the first iteration's blur of the estimate is this
+// same blur of the guide; its ratio guide / max(blur, floor) is formed here
+// once, and the first iteration starts from it.
+//   mask_out  = min(mask, blur(mask)^4) * detail gate(guide, blur(guide))
+//   ratio_out = guide / max(blur(guide), floor)
 // Only the tile's columns are blurred horizontally, over every window row, so
 // a border tile needs no repair: its mirrored rows are image rows.
 template <int R, bool BORDER>
-__device__ void blur_strips(const float* __restrict__ in, const float* __restrict__ aux,
-                            float* __restrict__ out, int width, int height, const Weights& k,
-                            int mode, float noise_protection, float* s_in, float* s_h) {
+__device__ void prep_strips(const float* __restrict__ mask, const float* __restrict__ guide,
+                            float* __restrict__ mask_out, float* __restrict__ ratio_out,
+                            int width, int height, const Weights& k, float noise_protection,
+                            float* s_in, float* s_h) {
     constexpr int W = TW + 2 * R, H = TH + 2 * R, P = (W + 3) / 4 * 4;
     static_assert(4 * (TW / 4 - 1) + Windows<R>::CHUNKS * 4 <= P, "strips stay in a row");
     static_assert(H * P <= (TH + 2 * MAXR) * (TW + 2 * MAXR), "the window fits s_in");
     static_assert(TH * TW == 4 * THREADS, "one vertical strip per thread");
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    stage<H, W, BORDER>(s_in, P, in, width, height, y0 - R, x0 - R);
-    __syncthreads();
-    for (int item = threadIdx.x; item < H * (TW / 4); item += THREADS) {
-        const int r = item / (TW / 4), j = item % (TW / 4);
-        *reinterpret_cast<float4*>(s_h + r * TW + 4 * j) = hstrip<R>(s_in + r * P + 4 * j, k);
-    }
-    __syncthreads();
     const int c = threadIdx.x & 63;
     const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    float b[4];
-    vstrip<R, TW>(s_h + r0 * TW + c, k, b);
+    // Both windows are requested up front; the guide's waits in registers.
+    Staged<H, W> mw, gw;
+    if (BORDER) {
+        mw.load_mirrored(mask, width, height, y0 - R, x0 - R, H, W);
+        gw.load_mirrored(guide, width, height, y0 - R, x0 - R, H, W);
+    } else {
+        mw.load(mask + (size_t)(y0 - R) * width + x0 - R, width, H, W);
+        gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, H, W);
+    }
+    float m[4], bm[4], g[4], bg[4];
+    // One blur of the window in s_in into b (this thread's 4 outputs); the
+    // window's values at the outputs go to centre.
+    auto blur = [&](float b[4], float centre[4]) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) centre[q] = s_in[(r0 + q + R) * P + c + R];
+        for (int item = threadIdx.x; item < H * (TW / 4); item += THREADS) {
+            const int r = item / (TW / 4), j = item % (TW / 4);
+            *reinterpret_cast<float4*>(s_h + r * TW + 4 * j) = hstrip<R>(s_in + r * P + 4 * j, k);
+        }
+        __syncthreads();
+        vstrip<R, TW>(s_h + r0 * TW + c, k, b);
+    };
+    mw.store(s_in, P, H, W);
+    blur(bm, m);
+    // Every read of s_in precedes blur's second barrier, and the guide's
+    // horizontal pass writes s_h only after its first one.
+    gw.store(s_in, P, H, W);
+    blur(bg, g);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int y = y0 + r0 + q, x = x0 + c;
         if (BORDER && (y >= height || x >= width)) continue;
         const size_t i = (size_t)y * width + x;
-        if (mode == 0) {
-            out[i] = num_min(aux[i], (b[q] * b[q]) * (b[q] * b[q]));
-        } else {
-            const float g = in[i];
-            const float threshold =
-                0.0005f + noise_protection * 0.008f + b[q] * noise_protection * 0.02f;
-            const float detail = fabsf(g - b[q]);
-            out[i] = aux[i] * (smoothstep(threshold, threshold * 3.0f, detail) *
-                               smoothstep(0.001f, 0.01f, g));
-        }
+        const float expanded = num_min(m[q], (bm[q] * bm[q]) * (bm[q] * bm[q]));
+        const float threshold =
+            0.0005f + noise_protection * 0.008f + bg[q] * noise_protection * 0.02f;
+        const float detail = fabsf(g[q] - bg[q]);
+        mask_out[i] = expanded * (smoothstep(threshold, threshold * 3.0f, detail) *
+                                  smoothstep(0.001f, 0.01f, g[q]));
+        ratio_out[i] = g[q] / num_max(bg[q], FLOOR);
     }
 }
 
 template <int R>
-__global__ __launch_bounds__(THREADS) void cap_blur(const float* __restrict__ in,
-                                                    const float* __restrict__ aux,
-                                                    float* __restrict__ out, int width,
-                                                    int height, Weights k, int mode,
-                                                    float noise_protection) {
+__global__ __launch_bounds__(THREADS) void cap_prep(const float* __restrict__ mask,
+                                                    const float* __restrict__ guide,
+                                                    float* __restrict__ mask_out,
+                                                    float* __restrict__ ratio_out, int width,
+                                                    int height, Weights k, float noise_protection) {
     __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
     __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
     const bool interior = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
     if (interior)
-        blur_strips<R, false>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+        prep_strips<R, false>(mask, guide, mask_out, ratio_out, width, height, k, noise_protection,
+                              s_in, s_h);
     else
-        blur_strips<R, true>(in, aux, out, width, height, k, mode, noise_protection, s_in, s_h);
+        prep_strips<R, true>(mask, guide, mask_out, ratio_out, width, height, k, noise_protection,
+                             s_in, s_h);
 }
 
 // The gain apply (sharpening.rs:105-126) at pixel i of planar RGB with n
@@ -410,6 +435,78 @@ __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__
         iterate_strips<R, APPLY, true>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
 }
 
+// The first RL iteration from cap_prep's ratio: blur(ratio) over the tile,
+// estimate = clamp(guide * b, guide / 2, 2 guide) (the estimate it updates is
+// the guide). The ratio window is staged half-sample mirrored for a border
+// tile, which gives the positions across the edge their mirrored values.
+template <int R, bool APPLY, bool BORDER>
+__device__ void iterate_first_strips(const float* __restrict__ ratio,
+                                     const float* __restrict__ guide, float* __restrict__ est_out,
+                                     int width, int height, const Weights& k, const Apply& ap,
+                                     float* s_a, float* s_b) {
+    using Win = Windows<R>;
+    constexpr int GW = Win::GW, GH = Win::GH, P = Win::P;
+    constexpr int NS3 = TW / 4;
+    constexpr int OUT_STRIPS = (TH / 4) * TW / THREADS;
+    static_assert(TW == 64 && OUT_STRIPS >= 1, "output strips map one wavefront to one tile row");
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    const int c = threadIdx.x & 63;
+    float g_out[OUT_STRIPS][4];
+#pragma unroll
+    for (int q = 0; q < OUT_STRIPS; ++q) {
+        const int r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y = y0 + r0 + i, x = x0 + c;
+            g_out[q][i] = !BORDER || (y < height && x < width) ? guide[(size_t)y * width + x] : 1.0f;
+        }
+    }
+    stage<GH, GW, BORDER>(s_a, P, ratio, width, height, y0 - R, x0 - R);
+    __syncthreads();
+    for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
+        const int r = item / NS3, j = item % NS3;
+        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < OUT_STRIPS; ++q) {
+        const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+        float b[4];
+        vstrip<R, P>(s_b + r0 * P + c, k, b);
+        const size_t o = (size_t)(y0 + r0) * width + x0 + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (BORDER && (y0 + r0 + i >= height || x0 + c >= width)) continue;
+            const float g = g_out[q][i];
+            const float est = __builtin_amdgcn_fmed3f(g * b[i], g * 0.5f, g * 2.0f);
+            if (APPLY)
+                apply_gain(ap, o + (size_t)i * width, est, g);
+            else
+                est_out[o + (size_t)i * width] = est;
+        }
+    }
+}
+
+template <int R, bool APPLY>
+__global__ __launch_bounds__(THREADS) void cap_iterate_first(const float* __restrict__ ratio,
+                                                             const float* __restrict__ guide,
+                                                             float* __restrict__ est_out,
+                                                             int width, int height, Weights k,
+                                                             Apply ap) {
+    using Win = Windows<R>;
+    __shared__ __align__(16) float s_a[Win::G_FLOATS];
+    __shared__ __align__(16) float s_b[Win::G_FLOATS];
+    const uint3 blk = xcd_block();
+    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    const bool interior =
+        x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
+    if (interior)
+        iterate_first_strips<R, APPLY, false>(ratio, guide, est_out, width, height, k, ap, s_a, s_b);
+    else
+        iterate_first_strips<R, APPLY, true>(ratio, guide, est_out, width, height, k, ap, s_a, s_b);
+}
+
 // sharpening.rs:105-126
 __global__ __launch_bounds__(THREADS) void cap_apply(float* __restrict__ rgb, int n, float amount,
                                                      const float* __restrict__ guide,
@@ -439,16 +536,29 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
                                      a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide,
                                      a.mask, a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
+    // The final mask goes to tmp and the first iteration's ratio to est_b,
+    // which the first iteration reads before the second one overwrites it.
+    float* mask = a.tmp;
+    float* ratio = a.est_b;
+    const Apply ap{a.rgb, mask, a.amount, (size_t)n};
     // The radius is a template parameter so the tap loops unroll.
-    auto blur = [&](const float* in, const float* aux, float* out, int mode) {
+    auto prep = [&] {
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL(cap_blur<1>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
-            case 2: hipLaunchKernelGGL(cap_blur<2>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
-            case 3: hipLaunchKernelGGL(cap_blur<3>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
-            default: hipLaunchKernelGGL(cap_blur<4>, tiles, dim3(THREADS), 0, s, in, aux, out, a.width, a.height, k, mode, a.noise_protection); break;
+            case 1: hipLaunchKernelGGL(cap_prep<1>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
+            case 2: hipLaunchKernelGGL(cap_prep<2>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
+            case 3: hipLaunchKernelGGL(cap_prep<3>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
+            default: hipLaunchKernelGGL(cap_prep<4>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
         }
     };
-    const Apply ap{a.rgb, a.mask, a.amount, (size_t)n};
+    auto first = [&](float* out, auto apply) {
+        constexpr bool APPLY = decltype(apply)::value;
+        switch (a.radius) {
+            case 1: hipLaunchKernelGGL((cap_iterate_first<1, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
+            case 2: hipLaunchKernelGGL((cap_iterate_first<2, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
+            case 3: hipLaunchKernelGGL((cap_iterate_first<3, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
+            default: hipLaunchKernelGGL((cap_iterate_first<4, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
+        }
+    };
     auto iterate = [&](const float* in, float* out, auto apply) {
         constexpr bool APPLY = decltype(apply)::value;
         switch (a.radius) {
@@ -458,29 +568,30 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
             default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
         }
     };
-    // mask = min(mask, blur(mask)^4), then the detail gate on blur(guide).
-    RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.mask, a.mask, a.tmp, 0));
-    RHD_TIMED(prof, s, "cap_blur", 12 * px, blur(a.guide, a.tmp, a.mask, 1));
-    // The estimate starts as the guide (sharpening.rs:219): the first
-    // iteration reads the guide itself, the later ones alternate between the
-    // two estimate planes.
+    // Reads the mask and the guide, writes the final mask and the ratio.
+    RHD_TIMED(prof, s, "cap_prep", 16 * px, prep());
     // The last iteration applies the gain itself; with no iterations the
-    // estimate is the guide and cap_apply does it.
-    const float* cur = a.guide;
-    for (int it = 0; it + 1 < a.iterations; ++it) {
-        float* out = (it & 1) ? a.est_b : a.est_a;
-        // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
-        RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, out, std::false_type{}));
-        cur = out;
-    }
-    if (a.iterations > 0) {
-        // Reads the estimate, the guide, the mask and RGB, writes RGB.
-        RHD_TIMED(prof, s, "cap_iterate_apply", 36 * px,
-                  iterate(cur, nullptr, std::true_type{}));
-    } else {
+    // estimate is the guide and cap_apply does it. The others alternate
+    // between the two estimate planes.
+    if (a.iterations == 0) {
         RHD_TIMED(prof, s, "cap_apply", 36 * px,
                   hipLaunchKernelGGL(cap_apply, dim3(blocks), dim3(THREADS), 0, s, a.rgb, n,
-                                     a.amount, a.guide, a.mask, cur));
+                                     a.amount, a.guide, mask, a.guide));
+    } else if (a.iterations == 1) {
+        // Reads the ratio, the guide, the mask and RGB, writes RGB.
+        RHD_TIMED(prof, s, "cap_iterate_first", 36 * px, first(nullptr, std::true_type{}));
+    } else {
+        // Reads the ratio and the guide, writes the estimate: 12 B per pixel.
+        RHD_TIMED(prof, s, "cap_iterate_first", 12 * px, first(a.est_a, std::false_type{}));
+        const float* cur = a.est_a;
+        for (int it = 1; it + 1 < a.iterations; ++it) {
+            float* out = (it & 1) ? a.est_b : a.est_a;
+            // Reads the estimate and the guide, writes the estimate: 12 B per pixel.
+            RHD_TIMED(prof, s, "cap_iterate", 12 * px, iterate(cur, out, std::false_type{}));
+            cur = out;
+        }
+        // Reads the estimate, the guide, the mask and RGB, writes RGB.
+        RHD_TIMED(prof, s, "cap_iterate_apply", 36 * px, iterate(cur, nullptr, std::true_type{}));
     }
     return hipGetLastError();
 }

@@ -42,17 +42,19 @@ def test_profile_only_times_one_group(gpu):
         lib.rhd_profile_enable(ctx, 1)
         gpu.develop_resident(resident, desc, detail).release()
         every = _entries(gpu)
-        assert {"normalize", "opp_sums", "amz_weights", "cap_iterate",
-                "cap_iterate_apply"} <= set(every)
-        # The last of the four iterations applies the gain.
-        assert every["cap_iterate"][1] == 3 and every["cap_iterate"][2] > 0
+        assert {"normalize", "opp_sums", "amz_weights", "cap_prep", "cap_iterate_first",
+                "cap_iterate", "cap_iterate_apply"} <= set(every)
+        # The first of the four iterations starts from cap_prep's ratio and
+        # the last applies the gain.
+        assert every["cap_prep"][1] == 1 and every["cap_iterate_first"][1] == 1
+        assert every["cap_iterate"][1] == 2 and every["cap_iterate"][2] > 0
         assert every["cap_iterate_apply"][1] == 1 and "cap_apply" not in every
 
         lib.rhd_profile_reset(ctx)
         lib.rhd_profile_only(ctx, b"cap_iterate")
         gpu.develop_resident(resident, desc, detail).release()
         only = _entries(gpu)
-        assert set(only) == {"cap_iterate"} and only["cap_iterate"][1] == 3
+        assert set(only) == {"cap_iterate"} and only["cap_iterate"][1] == 2
 
         lib.rhd_profile_reset(ctx)
         lib.rhd_profile_only(ctx, None)
