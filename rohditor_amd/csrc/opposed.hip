@@ -244,12 +244,24 @@ template <int SCOPE>
 __device__ __forceinline__ unsigned load_parent(unsigned* p, unsigned x) {
     return __hip_atomic_load(p + x, __ATOMIC_RELAXED, SCOPE);
 }
+// The root of x, pointing every node passed on the way at its grandparent
+// (path splitting), so that chains of tile roots built by the border unions
+// are walked once rather than by every cell. Only a node seen with a parent
+// other than itself is written, and roots never become non-roots again, so a
+// root's link is never touched. A parent only moves to a smaller member of
+// the same set. A split store may land after a concurrent merge lowered the
+// same node's parent with atomicMin and undo that link; that merge saw the
+// old parent returned and goes on to unite it with its other side, so no
+// union is lost (the pointer jumping of ECL-CC, Jaiganesh and Burtscher,
+// SC 2018, with atomicMin hooking).
 template <int SCOPE>
 __device__ __forceinline__ unsigned find_root(unsigned* p, unsigned x) {
     unsigned y = load_parent<SCOPE>(p, x);
     while (y != x) {
+        const unsigned z = load_parent<SCOPE>(p, y);
+        if (z != y) __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, SCOPE);
         x = y;
-        y = load_parent<SCOPE>(p, x);
+        y = z;
     }
     return x;
 }
