@@ -953,8 +953,9 @@ __global__ __launch_bounds__(256) void amz_area(AmazeArgs a) {
 //            s - 3, group 2 interpolates chroma on row s - 7, group 3
 //            finishes and writes row s - 9,
 //   barrier,
-//   phase 2: group 0 passes row s's weights on; groups 0 and 1 store their
-//            share of the inputs of row s + 1 (loaded eight steps earlier),
+//   phase 2: group 0 passes row s's weights on; the first two thread halves
+//            store their share of the inputs of row s + 1 (loaded eight
+//            steps earlier),
 //   barrier.
 // Each stage reads only rows that the stages before it completed in earlier
 // steps, so every value equals the one the stage sequence produces. The
@@ -1220,17 +1221,23 @@ __global__ __launch_bounds__(BACK_THREADS) void amz_back(AmazeArgs a, float clip
     const Tile t = tile_of(a, blk.x);
     __shared__ BackLds s;
     const int tid = threadIdx.x, p = tid & (TSH - 1);
-    const int group = __builtin_amdgcn_readfirstlane(tid / TSH);
-    const bool loader = tid < TS;  // groups 0 and 1 bring the inputs in
+    // Wavefront w issues on SIMD w % 4, so the thread halves 0 and 2 share two
+    // SIMDs and halves 1 and 3 the other two. The halves take the stages in
+    // the order chains, chroma, refinements, finish: the light chains share
+    // their SIMDs with the heavy refinements, chroma with the finish.
+    const int half = __builtin_amdgcn_readfirstlane(tid / TSH);
+    constexpr int STAGE_OF_HALF[4] = {0, 2, 1, 3};
+    const int group = STAGE_OF_HALF[half];
+    const bool loader = tid < TS;  // halves 0 and 1 bring the inputs in
     Area ar;
     const bool has_area = tile_area(a, blk.x, ar);
     // Rows before the pipeline: CFA rows 0 .. 8, original hvwt / pmwt row 8,
     // step 8's inputs; d2h and d2v read as zero until the chains write them.
     if (loader) {
         for (int r = 0; r <= 8; ++r) s.cfa[ring(r, R_LONG)][tid] = t.full(CFA)[r * TS + tid];
-        (group == 0 ? s.hvo : s.pmo)[ring(8, R_SHORT)][p] =
-            (group == 0 ? t.half(HVWT) : t.half(PMWT))[8 * TSH + p];
-        for (int r = 0; r < R_MID; ++r) (group == 0 ? s.d2h : s.d2v)[r][p] = 0.0f;
+        (half == 0 ? s.hvo : s.pmo)[ring(8, R_SHORT)][p] =
+            (half == 0 ? t.half(HVWT) : t.half(PMWT))[8 * TSH + p];
+        for (int r = 0; r < R_MID; ++r) (half == 0 ? s.d2h : s.d2v)[r][p] = 0.0f;
         BackIn in;
         back_load(t, a.pattern, 8, in);
         back_store(s, 8, in);
