@@ -87,10 +87,10 @@ COPY_BYTES = 1 << 30
 
 
 def short_kernel_name(name):
-    """`rhd::(anonymous namespace)::cap_iterate<3>(...)` -> `cap_iterate<3>`."""
+    """`rhd::(anonymous namespace)::cap_iterate<3, false>(...)` -> `cap_iterate<3,false>`."""
     import re
-    m = re.search(r"::(\w+)(<\w+>)?\(", name)
-    return (m.group(1) + (m.group(2) or "")) if m else name.split("(")[0][:48]
+    m = re.search(r"::(\w+)(<[^<>()]*>)?\(", name)
+    return (m.group(1) + (m.group(2) or "").replace(" ", "")) if m else name.split("(")[0][:48]
 
 
 def rocprof_summary(db):
@@ -373,11 +373,12 @@ def main():
     dom_ms, dom_launches, dom_bytes = profile_table(proc)[dominant]
     avg_ms = dom_ms / max(dom_launches, 1)
     achieved = dom_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    # rocprof names the dominant kernel's instantiation (cap_iterate<3> for
-    # sigma 0.7); the traffic file is keyed by that name.
+    # rocprof names the dominant kernel's instantiation (cap_iterate<3,false>
+    # for sigma 0.7: radius 3, estimate stored); the traffic file is keyed by
+    # that name.
     traffic = None
     valu_busy = None
-    for name in (dominant, dominant + "<3>"):
+    for name in (dominant, dominant + "<3>", dominant + "<3,false>"):
         traffic = traffic if traffic is not None else measured_traffic(name)
         valu_busy = valu_busy if valu_busy is not None else measured_valu_busy(name)
     # The roofline priced here is HBM's; the kernel's own limiter comes from

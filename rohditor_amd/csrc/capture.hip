@@ -15,9 +15,10 @@
 // ratio -> clamped update, all in LDS for a 64x16 output tile with a 2r
 // halo; the first starts from prep's ratio), the last of which applies the
 // gain to RGB instead of storing its estimate, and a plain gain apply for
-// zero iterations. Iterations ping-pong between two estimate planes. Per iteration the estimate (with halo) and
-// the guide are read and the estimate written; the iterations are bound by
-// vector-ALU issue (the blur taps and the ratio's division), not by HBM.
+// zero iterations. Iterations ping-pong between two estimate planes. Per
+// iteration the estimate (with halo) and the guide are read and the estimate
+// written; the iterations are bound by vector-ALU issue (the blur taps and
+// the ratio's division), not by HBM.
 // Tiles at the image edge run the same strip code as interior ones on
 // windows staged half-sample mirrored, so no tap computes a mirrored index.
 #include <type_traits>
