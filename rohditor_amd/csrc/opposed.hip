@@ -1296,10 +1296,11 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
     // The pair planes and the local part of the site estimates need only the
-    // cells: they run on the auxiliary stream beside the labelling and the
-    // region sums. The main stream waits for the pair planes before the ring
-    // contributions and for the local parts before the final site pass.
-    // Without an auxiliary stream everything runs in order on s.
+    // cells: they run on the auxiliary stream, the planes beside the
+    // labelling and the local parts beside the region sums. The main stream
+    // waits for the pair planes before the ring contributions and for the
+    // local parts before the final site pass. Without an auxiliary stream
+    // everything runs in order on s.
     hipStream_t side = a.aux ? a.aux : s;
     if (a.aux) {
         hipError_t e = hipEventRecord(a.fork, s);
@@ -1310,17 +1311,6 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_pairs, dim3(cell_blocks), dim3(THREADS), 0, side, L));
     if (a.aux) {
         hipError_t e = hipEventRecord(a.pairs, a.aux);
-        if (e != hipSuccess) return e;
-    }
-    const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
-    RHD_TIMED(prof, side, "opp_site_list", cells,
-              hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),
-                                 dim3(THREADS), 0, side, L));
-    RHD_TIMED(prof, side, "opp_sites_local", 16 * cells + 8 * cells,
-              hipLaunchKernelGGL(opp_sites_local, dim3(site_workers(L)), dim3(THREADS), 0, side, L,
-                                 l0, l1, l2));
-    if (a.aux) {
-        hipError_t e = hipEventRecord(a.join, a.aux);
         if (e != hipSuccess) return e;
     }
     RHD_TIMED(prof, s, "opp_label_tiles", cells + 4 * cells,
@@ -1353,6 +1343,25 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
                                  (int)RING));
     if (a.aux) {
         hipError_t e = hipStreamWaitEvent(s, a.pairs, 0);
+        if (e != hipSuccess) return e;
+    }
+    // The local site searches wait until the labelling is done, so that they
+    // run beside the region sums, whose sequential f64 chains leave most of
+    // the device idle, rather than beside the labelling kernels.
+    if (a.aux) {
+        hipError_t e = hipEventRecord(a.labelled, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.labelled, 0);
+        if (e != hipSuccess) return e;
+    }
+    const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
+    RHD_TIMED(prof, side, "opp_site_list", cells,
+              hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),
+                                 dim3(THREADS), 0, side, L));
+    RHD_TIMED(prof, side, "opp_sites_local", 16 * cells + 8 * cells,
+              hipLaunchKernelGGL(opp_sites_local, dim3(site_workers(L)), dim3(THREADS), 0, side, L,
+                                 l0, l1, l2));
+    if (a.aux) {
+        hipError_t e = hipEventRecord(a.join, a.aux);
         if (e != hipSuccess) return e;
     }
     RHD_TIMED(prof, s, "opp_contrib", 0,

@@ -182,7 +182,7 @@ struct OpposedArgs {
     // Optional second stream (with three events) for the work that depends
     // only on the cells; null runs everything on the launch stream.
     hipStream_t aux;
-    hipEvent_t fork, pairs, join;
+    hipEvent_t fork, pairs, labelled, join;
 };
 size_t opposed_scratch_bytes(int width, int height);
 hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);
