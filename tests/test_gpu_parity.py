@@ -192,6 +192,26 @@ def test_capture_bit_exact(gpu, oracle, size, sigma):
     assert bits_equal(out, expected), describe_mismatch(out, expected)
 
 
+@pytest.mark.parametrize("size", [(63, 17), (65, 15), (128, 16), (129, 33), (40, 70), (100, 9),
+                                  (136, 44)])
+@pytest.mark.parametrize("sigma", [0.3, 0.6, 0.7, 1.2])
+def test_capture_tile_edges_bit_exact(gpu, oracle, size, sigma):
+    # Widths and heights one below, at and one above the 64x16 output tile,
+    # and windows that reach across two image edges at once, for every
+    # radius: the edge tiles' mirrored windows and the repaired ratio rows
+    # and columns, through the prepared first iteration, a middle one and
+    # the applying last one.
+    w, h = size
+    spec = synth.SceneSpec(max(w, 8), max(h, 8), 7 + w * h, clipped_fraction=0.05)
+    m = ((synth.mosaic(spec)[:h, :w].astype(F) - F(512)) / F(16383 - 512)).astype(F)
+    dd = _detail(amount=0.9, sigma=sigma, noise=0.3, ceilings=(0.9, 1.0, 0.95), iterations=3)
+    out = gpu_demosaic(gpu, m, 0, (1, 1, 1), "mhc", detail=dd)
+    _, base = oracle.demosaic(m, 0, (1, 1, 1), "mhc")
+    st, expected = oracle.capture(base, 0.9, sigma, 0.3, (0.9, 1.0, 0.95), iterations=3, threads=8)
+    assert st == 0
+    assert bits_equal(out, expected), f"{size} sigma {sigma}: " + describe_mismatch(out, expected)
+
+
 @pytest.mark.parametrize("iterations", [0, 1, 10, 20])
 def test_capture_iteration_counts(gpu, oracle, iterations):
     spec = synth.SceneSpec(200, 150, 5)
