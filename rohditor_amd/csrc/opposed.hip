@@ -624,10 +624,20 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
         const float4 cell = L.cell[i];
         const unsigned rc = lab < (unsigned)L.cap ? L.region_clipped[lab] : 0u;
         unsigned flags = 0;
+        // The pair values of the ring cell, formed here as opp_pairs forms
+        // them (the pair planes are written beside the region sums).
+        float roots[3];
+        bool rooted = false;
         for (int c = 0; c < 3; ++c) {
             if (!(rc & (1u << c))) continue;
             if (mean_of(cell, c) < 0.5f * lv[c]) continue;
-            const float pref = L.pair_ref[(size_t)c * L.cells + i];
+            if (!rooted) {
+                for (int q = 0; q < 3; ++q)
+                    roots[q] = positive_finite(mean_of(cell, q)) ? cbrt_f32(mean_of(cell, q)) : 0.0f;
+                rooted = true;
+            }
+            float pref, plr;
+            pair_values(cell, roots, c, L.log2_bitmap, pref, plr);
             for (int p = 0; p < 3; ++p) {
                 const int off = c * 3 + p;
                 if (!(any & (1u << off))) continue;
@@ -637,8 +647,7 @@ __global__ __launch_bounds__(THREADS) void opp_contrib(Layout L, float l0, float
                 flags |= 1u << off;
                 if (p == 2) {
                     float ratio;
-                    if (support_log_ratio(support_pattern(c, 2),
-                                          L.pair_lr[(size_t)c * L.cells + i], ratio)) {
+                    if (support_log_ratio(support_pattern(c, 2), plr, ratio)) {
                         L.contrib_ratio[(size_t)k * 3 + c] = ratio;
                         flags |= 1u << (SLOTS + c);
                     }
@@ -1296,23 +1305,10 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
               hipLaunchKernelGGL(opp_cells, dim3(cell_blocks), dim3(THREADS), 0, s, a.mosaic, L,
                                  a.pattern, l0, l1, l2));
     // The pair planes and the local part of the site estimates need only the
-    // cells: they run on the auxiliary stream, the planes beside the
-    // labelling and the local parts beside the region sums. The main stream
-    // waits for the pair planes before the ring contributions and for the
-    // local parts before the final site pass. Without an auxiliary stream
-    // everything runs in order on s.
+    // cells: they run on the auxiliary stream beside the region sums, and the
+    // main stream waits for them before the final site pass. Without an
+    // auxiliary stream everything runs in order on s.
     hipStream_t side = a.aux ? a.aux : s;
-    if (a.aux) {
-        hipError_t e = hipEventRecord(a.fork, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.fork, 0);
-        if (e != hipSuccess) return e;
-    }
-    RHD_TIMED(prof, side, "opp_pairs", 16 * cells + 24 * cells,
-              hipLaunchKernelGGL(opp_pairs, dim3(cell_blocks), dim3(THREADS), 0, side, L));
-    if (a.aux) {
-        hipError_t e = hipEventRecord(a.pairs, a.aux);
-        if (e != hipSuccess) return e;
-    }
     RHD_TIMED(prof, s, "opp_label_tiles", cells + 4 * cells,
               hipLaunchKernelGGL(opp_label_tiles, dim3(tiles_x, tiles_y), dim3(THREADS), 0, s, L));
     RHD_TIMED(prof, s, "opp_label_borders", 20ull * border_cells,
@@ -1341,18 +1337,17 @@ hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof) {
                                  (int)RING);
               hipLaunchKernelGGL(opp_scan_emit, dim3(scan_blocks), dim3(THREADS), 0, s, L,
                                  (int)RING));
+    // The pair planes and the local site searches wait until the labelling is
+    // done, so that they run beside the region sums, whose sequential f64
+    // chains leave most of the device idle, rather than beside the labelling
+    // kernels. The ring contributions form their own pair values.
     if (a.aux) {
-        hipError_t e = hipStreamWaitEvent(s, a.pairs, 0);
+        hipError_t e = hipEventRecord(a.fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.fork, 0);
         if (e != hipSuccess) return e;
     }
-    // The local site searches wait until the labelling is done, so that they
-    // run beside the region sums, whose sequential f64 chains leave most of
-    // the device idle, rather than beside the labelling kernels.
-    if (a.aux) {
-        hipError_t e = hipEventRecord(a.labelled, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(a.aux, a.labelled, 0);
-        if (e != hipSuccess) return e;
-    }
+    RHD_TIMED(prof, side, "opp_pairs", 16 * cells + 24 * cells,
+              hipLaunchKernelGGL(opp_pairs, dim3(cell_blocks), dim3(THREADS), 0, side, L));
     const int sites_blocks = (L.cw + SX - 1) / SX * ((L.ch + SY - 1) / SY);
     RHD_TIMED(prof, side, "opp_site_list", cells,
               hipLaunchKernelGGL(opp_site_list, dim3((sites_blocks + LIST_PER_GROUP - 1) / LIST_PER_GROUP),

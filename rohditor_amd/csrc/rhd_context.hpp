@@ -34,7 +34,7 @@ struct Lane {
     // A second stream for work that can run beside the lane's own, forked
     // from and joined back into `stream` with the events.
     hipStream_t aux = nullptr;
-    hipEvent_t fork = nullptr, mid = nullptr, late = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     DeviceStats* d_stats = nullptr;
     DeviceStats* h_stats = nullptr;
     DeviceBuffer amaze, amaze_areas, rcd, capture, opposed, optics, raw;

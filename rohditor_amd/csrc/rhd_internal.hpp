@@ -179,10 +179,10 @@ struct OpposedArgs {
     size_t scratch_bytes;
     DeviceStats* stats;
     const uint32_t* log2_bitmap;  // libm_f32.hpp exception bitmap on this device
-    // Optional second stream (with three events) for the work that depends
+    // Optional second stream (with two events) for the work that depends
     // only on the cells; null runs everything on the launch stream.
     hipStream_t aux;
-    hipEvent_t fork, pairs, labelled, join;
+    hipEvent_t fork, join;
 };
 size_t opposed_scratch_bytes(int width, int height);
 hipError_t launch_opposed(const OpposedArgs& a, hipStream_t s, Profiler* prof);
