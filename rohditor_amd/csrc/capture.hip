@@ -21,6 +21,7 @@
 // the ratio's division), not by HBM.
 // Tiles at the image edge run the same strip code as interior ones on
 // windows staged half-sample mirrored, so no tap computes a mirrored index.
+#include <cstdint>
 #include <type_traits>
 
 #include "rhd_internal.hpp"
@@ -34,6 +35,28 @@ constexpr float FLOOR = CAPTURE_FLOOR;
 struct Weights {
     float w[2 * MAXR + 1];
 };
+
+// Guide values below this bound let the RL ratio take div_in_range.
+constexpr float GUIDE_LIMIT = 0x1p64f;
+
+// a / b, correctly rounded, for a in [2^-20, 2^64] and b in [2^-20, 2^66]:
+// the reciprocal refined by one Newton-Raphson step, then two residual
+// corrections of the quotient (Markstein). This is the division sequence the
+// compiler emits for a / b without its operand scaling (which only acts when
+// an operand, the quotient or the reciprocal is denormal, or the exponents
+// differ by 96 or more) and its special-case fixup (zeros, infinities,
+// NaNs, overflow), none of which these operands reach; the results are
+// identical.
+__device__ __forceinline__ float div_in_range(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y1 = __builtin_fmaf(e, y0, y0);
+    const float q0 = a * y1;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y1, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y1, q1);
+}
 
 // sharpening.rs:194-218
 __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rgb, int n,
@@ -132,18 +155,98 @@ __device__ __forceinline__ void stage(float* lds, int pitch, const float* __rest
 // addressing and loop control are shared by 4 results; every output keeps
 // the reference's tap order. Strips that reach past a window's edge compute
 // throw-away values from the padding and are not stored.
+//
+// The LDS arrays of the iteration begin at 16-byte aligned image columns, so
+// interior windows move between HBM and LDS as float4 and every horizontal
+// strip reads aligned chunks. For a tile at (x0, y0), LDS column 0 of
+//   the estimate window (EH rows from y0 - 2r)   is image column x0 - EL,
+//   its horizontal blur                          is x0 - EL + r,
+//   the guide / ratio window (GH rows from y0 - r) is x0 - 4,
+//   the ratio's horizontal blur                  is x0 - 4 + r.
 template <int R>
 struct Windows {
     static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;   // estimate (2r halo)
     static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;   // guide / ratio (r halo)
-    static constexpr int P = (GW + 3) / 4 * 4;              // pitch of the r-halo windows
-    static constexpr int E_FLOATS = EH * EW + 8;
+    static constexpr int CHUNKS = (4 + 2 * R + 3) / 4;      // float4 reads per strip
+    static constexpr int EL = (2 * R + 3) / 4 * 4;          // estimate halo rounded up to 4
+    static constexpr int EP = TW + 2 * EL;                  // estimate pitch
+    static constexpr int ED = EL - 2 * R;                   // LDS column of image column x0 - 2r
+    static constexpr int GD = 4 - R;                        // LDS column of image column x0 - r
+    static constexpr int P = 72;                            // guide / ratio and blur pitch
+    static constexpr int NS1 = (TW + EL) / 4;               // estimate strips per row
+    static constexpr int NS3 = (TW + GD + 3) / 4;           // ratio strips per row
+    static constexpr int E_FLOATS = EH * EP;
     static constexpr int H_FLOATS = (EH + 4) * P;
-    static constexpr int G_FLOATS = GH * P;
+    static constexpr int G_FLOATS = GH * P + 4;
     // The estimate window, then the guide / ratio window, share one array.
     static constexpr int A_FLOATS = E_FLOATS > G_FLOATS ? E_FLOATS : G_FLOATS;
-    static constexpr int CHUNKS = (4 + 2 * R + 3) / 4;      // float4 reads per strip
+    static_assert(GD + GW <= P && ED + EW <= EP, "windows fit their pitch");
+    static_assert(4 * NS1 <= P && 4 * (NS1 - 1) + 4 * CHUNKS <= EP,
+                  "estimate strips cover the blur the ratio needs and stay in a row");
+    static_assert((GH - 1) * P + 4 * (NS3 - 1) + 4 * CHUNKS <= G_FLOATS,
+                  "the last ratio row's strips stay in the array");
 };
+
+// A window of ROWS rows of 4 * F4 floats from a global plane, its first
+// element 16-byte aligned, held in registers as float4 while earlier LDS work
+// runs. Lane l of a wavefront takes chunk l % F4 of row l / F4 of each group of
+// 64 / F4 rows the wavefront owns, so row addresses stay wave-uniform and a
+// lane's offset within the group is formed once.
+template <int ROWS, int F4>
+struct Staged4 {
+    using vf4 = float __attribute__((ext_vector_type(4)));
+    static constexpr int RPG = 64 / F4, GROUPS = (ROWS + RPG - 1) / RPG;
+    static constexpr int WAVES = THREADS / 64, SLOTS = (GROUPS + WAVES - 1) / WAVES;
+    vf4 v[SLOTS];
+
+    __device__ __forceinline__ void load(const float* __restrict__ src, size_t pitch) {
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+        const int dr = l / F4, dc = l - dr * F4;
+        const unsigned lane = (unsigned)(dr * pitch + 4 * dc);
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            const int g = w + WAVES * k;
+            const float* group = src + (size_t)(g * RPG) * pitch;
+            if (g < GROUPS && dr < RPG && g * RPG + dr < ROWS)
+                v[k] = *reinterpret_cast<const vf4*>(group + lane);
+        }
+    }
+    __device__ __forceinline__ void store(float* lds, int pitch) const {
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+        const int dr = l / F4, dc = l - dr * F4;
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            const int g = w + WAVES * k, r = g * RPG + dr;
+            if (g < GROUPS && dr < RPG && r < ROWS)
+                *reinterpret_cast<vf4*>(lds + r * pitch + 4 * dc) = v[k];
+        }
+    }
+};
+
+// Calls f(row, col) for every item of a ROWS x COLS grid that this thread
+// owns (items threadIdx.x + THREADS * t in row-major order). The row and
+// column advance by THREADS per trip instead of being divided out of the item
+// each time; only the last trip tests the item against the grid, and a
+// wavefront whose first item is past the grid skips that trip as a whole.
+template <int ROWS, int COLS, typename F>
+__device__ __forceinline__ void for_items(F f) {
+    constexpr int N = ROWS * COLS, TRIPS = (N + THREADS - 1) / THREADS;
+    constexpr int DR = THREADS / COLS, DC = THREADS % COLS;
+    const int wave_first = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
+    int r = (int)threadIdx.x / COLS, c = (int)threadIdx.x % COLS;
+#pragma unroll
+    for (int t = 0; t < TRIPS; ++t) {
+        if (t + 1 < TRIPS ||
+            (wave_first + THREADS * t < N && (int)threadIdx.x + THREADS * t < N))
+            f(r, c);
+        r += DR;
+        c += DC;
+        if (c >= COLS) {
+            c -= COLS;
+            ++r;
+        }
+    }
+}
 
 // 4 horizontal outputs from the row segment starting at `src` (16-byte aligned).
 template <int R>
@@ -224,7 +327,7 @@ template <int R, bool BORDER>
 __device__ void prep_strips(const float* __restrict__ mask, const float* __restrict__ guide,
                             float* __restrict__ mask_out, float* __restrict__ ratio_out,
                             int width, int height, const Weights& k, float noise_protection,
-                            float* s_in, float* s_h) {
+                            unsigned* range_flag, float* s_in, float* s_h) {
     constexpr int W = TW + 2 * R, H = TH + 2 * R, P = (W + 3) / 4 * 4;
     static_assert(4 * (TW / 4 - 1) + Windows<R>::CHUNKS * 4 <= P, "strips stay in a row");
     static_assert(H * P <= (TH + 2 * MAXR) * (TW + 2 * MAXR), "the window fits s_in");
@@ -262,10 +365,12 @@ __device__ void prep_strips(const float* __restrict__ mask, const float* __restr
     // horizontal pass writes s_h only after its first one.
     gw.store(s_in, P, H, W);
     blur(bg, g);
+    bool wide = false;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int y = y0 + r0 + q, x = x0 + c;
         if (BORDER && (y >= height || x >= width)) continue;
+        wide |= !(g[q] < GUIDE_LIMIT);
         const size_t i = (size_t)y * width + x;
         const float expanded = num_min(m[q], (bm[q] * bm[q]) * (bm[q] * bm[q]));
         const float threshold =
@@ -275,6 +380,9 @@ __device__ void prep_strips(const float* __restrict__ mask, const float* __restr
                                   smoothstep(0.001f, 0.01f, g[q]));
         ratio_out[i] = g[q] / num_max(bg[q], FLOOR);
     }
+    // Every pixel's guide passes through one tile's outputs here, so the
+    // flag is set before the iterations if any guide value is out of range.
+    if (wide) atomicOr(range_flag, 1u);
 }
 
 template <int R>
@@ -282,7 +390,8 @@ __global__ __launch_bounds__(THREADS) void cap_prep(const float* __restrict__ ma
                                                     const float* __restrict__ guide,
                                                     float* __restrict__ mask_out,
                                                     float* __restrict__ ratio_out, int width,
-                                                    int height, Weights k, float noise_protection) {
+                                                    int height, Weights k, float noise_protection,
+                                                    unsigned* range_flag) {
     __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
     __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
     const uint3 blk = xcd_block();
@@ -290,10 +399,10 @@ __global__ __launch_bounds__(THREADS) void cap_prep(const float* __restrict__ ma
     const bool interior = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
     if (interior)
         prep_strips<R, false>(mask, guide, mask_out, ratio_out, width, height, k, noise_protection,
-                              s_in, s_h);
+                              range_flag, s_in, s_h);
     else
         prep_strips<R, true>(mask, guide, mask_out, ratio_out, width, height, k, noise_protection,
-                             s_in, s_h);
+                             range_flag, s_in, s_h);
 }
 
 // The gain apply (sharpening.rs:105-126) at pixel i of planar RGB with n
@@ -327,15 +436,14 @@ __device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est,
 // guide values of the tile's outputs are kept in registers before the ratio
 // overwrites the guide window. APPLY: the last iteration applies the gain
 // with its estimate instead of storing the estimate.
-template <int R, bool APPLY, bool BORDER>
+template <int R, bool APPLY, bool BORDER, bool IN_RANGE>
 __device__ void iterate_strips(const float* __restrict__ est_in, const float* __restrict__ guide,
                                float* __restrict__ est_out, int width, int height,
                                const Weights& k, const Apply& ap, float* s_a, float* s_b) {
     using Win = Windows<R>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
-    constexpr int NS1 = (GW + 3) / 4;   // column strips of the estimate's blur
+    constexpr int EP = Win::EP, EL = Win::EL, GD = Win::GD, NS1 = Win::NS1, NS3 = Win::NS3;
     constexpr int NR2 = (GH + 3) / 4;   // row strips of the blurred estimate
-    constexpr int NS3 = TW / 4;         // column strips of the ratio's blur
     constexpr int NR4 = TH / 4;         // row strips of the output
     constexpr int OUT_STRIPS = NR4 * TW / THREADS;
     static_assert(TW == 64, "output strips map one wavefront to one tile row");
@@ -343,55 +451,68 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
     // The guide window waits in registers while the estimate window, which
-    // shares its LDS, is blurred.
-    Staged<GH, GW> gw;
-    if (BORDER)
+    // shares its LDS, is blurred. Border tiles stage both windows element by
+    // element, half-sample mirrored; interior ones as aligned float4 rows
+    // that include the columns between the aligned start and the window.
+    std::conditional_t<BORDER, Staged<GH, GW>, Staged4<GH, P / 4>> gw;
+    if constexpr (BORDER) {
         gw.load_mirrored(guide, width, height, y0 - R, x0 - R, GH, GW);
-    else
-        gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, GH, GW);
-    stage<EH, EW, BORDER>(s_a, EW, est_in, width, height, y0 - 2 * R, x0 - 2 * R);
+        stage<EH, EW, true>(s_a + Win::ED, EP, est_in, width, height, y0 - 2 * R, x0 - 2 * R);
+    } else {
+        gw.load(guide + (size_t)(y0 - R) * width + x0 - 4, width);
+        Staged4<EH, EP / 4> ew;
+        ew.load(est_in + (size_t)(y0 - 2 * R) * width + x0 - EL, width);
+        ew.store(s_a, EP);
+    }
     __syncthreads();
     float e_out[OUT_STRIPS][4], g_out[OUT_STRIPS][4];
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) e_out[q][i] = s_a[(r0 + i + 2 * R) * EW + c + 2 * R];
+        for (int i = 0; i < 4; ++i) e_out[q][i] = s_a[(r0 + i + 2 * R) * EP + c + EL];
     }
-    for (int item = threadIdx.x; item < EH * NS1; item += THREADS) {
-        const int r = item / NS1, j = item % NS1;
-        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * EW + 4 * j, k);
-    }
+    for_items<EH, NS1>([&](int r, int j) {
+        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * EP + 4 * j, k);
+    });
     __syncthreads();
-    gw.store(s_a, P, GH, GW);
+    if constexpr (BORDER)
+        gw.store(s_a + GD, P, GH, GW);
+    else
+        gw.store(s_a, P);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + R];
+        for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + 4];
     }
     __syncthreads();
-    for (int item = threadIdx.x; item < NR2 * GW; item += THREADS) {
-        const int c = item % GW, r0 = 4 * (item / GW);
+    // The ratio at guide column c (LDS column c + GD) takes the estimate's
+    // blur at LDS column c + ED of s_b, the same image column.
+    for_items<NR2, GW>([&](int strip, int c) {
+        const int r0 = 4 * strip;
         float b[4];
-        vstrip<R, P>(s_b + r0 * P + c, k, b);
+        vstrip<R, P>(s_b + r0 * P + c + Win::ED, k, b);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (r0 + i < GH) s_a[(r0 + i) * P + c] = s_a[(r0 + i) * P + c] / num_max(b[i], FLOOR);
-    }
+        for (int i = 0; i < 4; ++i) {
+            if (r0 + i >= GH) continue;
+            float* ratio = s_a + (r0 + i) * P + c + GD;
+            const float den = num_max(b[i], FLOOR);
+            *ratio = IN_RANGE ? div_in_range(*ratio, den) : *ratio / den;
+        }
+    });
     __syncthreads();
-    if (BORDER) {
+    if constexpr (BORDER) {
         // The ratio's blur reads up to r positions across the image edge.
-        mirror_window_rows<R, GH, GW>(s_a, P, y0 - R, height);
+        mirror_window_rows<R, GH, GW>(s_a + GD, P, y0 - R, height);
         __syncthreads();
-        mirror_window_cols<R, GH, GW>(s_a, P, x0 - R, width);
+        mirror_window_cols<R, GH, GW>(s_a + GD, P, x0 - R, width);
         __syncthreads();
     }
-    for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
-        const int r = item / NS3, j = item % NS3;
+    for_items<GH, NS3>([&](int r, int j) {
         *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
-    }
+    });
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
@@ -399,7 +520,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
         const int c = threadIdx.x & 63;
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
         float b[4];
-        vstrip<R, P>(s_b + r0 * P + c, k, b);
+        vstrip<R, P>(s_b + r0 * P + c + GD, k, b);
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -422,18 +543,32 @@ template <int R, bool APPLY>
 __global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__ est_in,
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
-                                                       int height, Weights k, Apply ap) {
+                                                       int height, Weights k, Apply ap,
+                                                       bool aligned,
+                                                       const unsigned* __restrict__ range_flag) {
     using Win = Windows<R>;
     __shared__ __align__(16) float s_a[Win::A_FLOATS];
     __shared__ __align__(16) float s_b[Win::H_FLOATS];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const bool interior =
-        x0 - 2 * R >= 0 && x0 + TW + 2 * R <= width && y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
-    if (interior)
-        iterate_strips<R, APPLY, false>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
+    // Interior: the aligned windows lie inside the image and the planes' rows
+    // start 16-byte aligned (aligned: the width is a multiple of 4 and the
+    // planes are 16-byte aligned).
+    const bool interior = aligned && x0 - Win::EL >= 0 && x0 + TW + Win::EL <= width &&
+                          y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
+    // With every guide value in [floor, 2^64) (cap_prep's flag), every
+    // estimate lies in [floor / 2, 2^65] (the clamp) and so does its blur,
+    // whose weights sum to 1 within a few ulps: the ratio's operands are in
+    // div_in_range's domain.
+    if (interior && *range_flag == 0)
+        iterate_strips<R, APPLY, false, true>(est_in, guide, est_out, width, height, k, ap, s_a,
+                                              s_b);
+    else if (interior)
+        iterate_strips<R, APPLY, false, false>(est_in, guide, est_out, width, height, k, ap, s_a,
+                                               s_b);
     else
-        iterate_strips<R, APPLY, true>(est_in, guide, est_out, width, height, k, ap, s_a, s_b);
+        iterate_strips<R, APPLY, true, false>(est_in, guide, est_out, width, height, k, ap, s_a,
+                                              s_b);
 }
 
 // The first RL iteration from cap_prep's ratio: blur(ratio) over the tile,
@@ -446,8 +581,7 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
                                      int width, int height, const Weights& k, const Apply& ap,
                                      float* s_a, float* s_b) {
     using Win = Windows<R>;
-    constexpr int GW = Win::GW, GH = Win::GH, P = Win::P;
-    constexpr int NS3 = TW / 4;
+    constexpr int GW = Win::GW, GH = Win::GH, P = Win::P, GD = Win::GD, NS3 = Win::NS3;
     constexpr int OUT_STRIPS = (TH / 4) * TW / THREADS;
     static_assert(TW == 64 && OUT_STRIPS >= 1, "output strips map one wavefront to one tile row");
     const uint3 blk = xcd_block();
@@ -463,18 +597,23 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
             g_out[q][i] = !BORDER || (y < height && x < width) ? guide[(size_t)y * width + x] : 1.0f;
         }
     }
-    stage<GH, GW, BORDER>(s_a, P, ratio, width, height, y0 - R, x0 - R);
-    __syncthreads();
-    for (int item = threadIdx.x; item < GH * NS3; item += THREADS) {
-        const int r = item / NS3, j = item % NS3;
-        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
+    if constexpr (BORDER) {
+        stage<GH, GW, true>(s_a + GD, P, ratio, width, height, y0 - R, x0 - R);
+    } else {
+        Staged4<GH, P / 4> rw;
+        rw.load(ratio + (size_t)(y0 - R) * width + x0 - 4, width);
+        rw.store(s_a, P);
     }
+    __syncthreads();
+    for_items<GH, NS3>([&](int r, int j) {
+        *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
+    });
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
         float b[4];
-        vstrip<R, P>(s_b + r0 * P + c, k, b);
+        vstrip<R, P>(s_b + r0 * P + c + GD, k, b);
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -494,14 +633,16 @@ __global__ __launch_bounds__(THREADS) void cap_iterate_first(const float* __rest
                                                              const float* __restrict__ guide,
                                                              float* __restrict__ est_out,
                                                              int width, int height, Weights k,
-                                                             Apply ap) {
+                                                             Apply ap, bool aligned) {
     using Win = Windows<R>;
     __shared__ __align__(16) float s_a[Win::G_FLOATS];
     __shared__ __align__(16) float s_b[Win::G_FLOATS];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const bool interior =
-        x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
+    // Interior: the aligned ratio window (image columns x0 - 4 to x0 + 67)
+    // lies inside the image and the planes' rows start 16-byte aligned.
+    const bool interior = aligned && x0 - 4 >= 0 && x0 + TW + 4 <= width && y0 - R >= 0 &&
+                          y0 + TH + R <= height;
     if (interior)
         iterate_first_strips<R, APPLY, false>(ratio, guide, est_out, width, height, k, ap, s_a, s_b);
     else
@@ -525,7 +666,7 @@ size_t capture_scratch_bytes(int width, int height) {
 }
 
 hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
-    if (a.radius < 1 || a.radius > MAXR) return hipErrorInvalidValue;
+    if (a.radius < 1 || a.radius > MAXR || !a.stats) return hipErrorInvalidValue;
     const int n = a.width * a.height;
     const uint64_t px = (uint64_t)n;
     Weights k{};
@@ -537,6 +678,11 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
                                      a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide,
                                      a.mask, a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
+    // Interior tiles move their windows as float4 when every row of the
+    // planes they read starts 16-byte aligned.
+    auto aligned16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+    const bool aligned = a.width % 4 == 0 && aligned16(a.guide) && aligned16(a.est_a) &&
+                         aligned16(a.est_b);
     // The final mask goes to tmp and the first iteration's ratio to est_b,
     // which the first iteration reads before the second one overwrites it.
     float* mask = a.tmp;
@@ -545,28 +691,28 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // The radius is a template parameter so the tap loops unroll.
     auto prep = [&] {
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL(cap_prep<1>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
-            case 2: hipLaunchKernelGGL(cap_prep<2>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
-            case 3: hipLaunchKernelGGL(cap_prep<3>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
-            default: hipLaunchKernelGGL(cap_prep<4>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection); break;
+            case 1: hipLaunchKernelGGL(cap_prep<1>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
+            case 2: hipLaunchKernelGGL(cap_prep<2>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
+            case 3: hipLaunchKernelGGL(cap_prep<3>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
+            default: hipLaunchKernelGGL(cap_prep<4>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
         }
     };
     auto first = [&](float* out, auto apply) {
         constexpr bool APPLY = decltype(apply)::value;
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL((cap_iterate_first<1, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
-            case 2: hipLaunchKernelGGL((cap_iterate_first<2, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
-            case 3: hipLaunchKernelGGL((cap_iterate_first<3, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
-            default: hipLaunchKernelGGL((cap_iterate_first<4, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap); break;
+            case 1: hipLaunchKernelGGL((cap_iterate_first<1, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap, aligned); break;
+            case 2: hipLaunchKernelGGL((cap_iterate_first<2, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap, aligned); break;
+            case 3: hipLaunchKernelGGL((cap_iterate_first<3, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap, aligned); break;
+            default: hipLaunchKernelGGL((cap_iterate_first<4, APPLY>), tiles, dim3(THREADS), 0, s, ratio, a.guide, out, a.width, a.height, k, ap, aligned); break;
         }
     };
     auto iterate = [&](const float* in, float* out, auto apply) {
         constexpr bool APPLY = decltype(apply)::value;
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL((cap_iterate<1, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
-            case 2: hipLaunchKernelGGL((cap_iterate<2, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
-            case 3: hipLaunchKernelGGL((cap_iterate<3, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
-            default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap); break;
+            case 1: hipLaunchKernelGGL((cap_iterate<1, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            case 2: hipLaunchKernelGGL((cap_iterate<2, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            case 3: hipLaunchKernelGGL((cap_iterate<3, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
         }
     };
     // Reads the mask and the guide, writes the final mask and the ratio.

@@ -99,7 +99,8 @@ struct DeviceStats {
     unsigned int nonfinite;         // bit 0: highlight input, bit 1: demosaic output,
                                     // bit 2: capture input
     unsigned int regions;
-    unsigned int pad[2];
+    unsigned int capture_range;     // nonzero: a capture guide value is >= 2^64 or not finite
+    unsigned int pad;
 };
 
 }  // namespace rhd

@@ -212,6 +212,36 @@ def test_capture_tile_edges_bit_exact(gpu, oracle, size, sigma):
     assert bits_equal(out, expected), f"{size} sigma {sigma}: " + describe_mismatch(out, expected)
 
 
+@pytest.mark.parametrize("size", [(301, 90), (298, 90), (300, 91), (264, 60)])
+def test_capture_unaligned_widths_bit_exact(gpu, oracle, size):
+    # Interior tiles load their windows as float4 rows only when every plane
+    # row starts 16-byte aligned: widths 301 and 298 send every tile through
+    # the element-wise staging, 300 and 264 mix both kinds.
+    w, h = size
+    spec = synth.SceneSpec(w, h, 11 + w, clipped_fraction=0.05)
+    m = ((synth.mosaic(spec).astype(F) - F(512)) / F(16383 - 512)).astype(F)
+    dd = _detail(amount=0.8, sigma=0.7, noise=0.4, ceilings=(1.0, 1.0, 1.0), iterations=4)
+    out = gpu_demosaic(gpu, m, 0, (1, 1, 1), "mhc", detail=dd)
+    _, base = oracle.demosaic(m, 0, (1, 1, 1), "mhc")
+    _, expected = oracle.capture(base, 0.8, 0.7, 0.4, (1, 1, 1), iterations=4, threads=8)
+    assert bits_equal(out, expected), f"{size}: " + describe_mismatch(out, expected)
+
+
+@pytest.mark.parametrize("scale", [1e18, 1e20])
+def test_capture_large_guides_bit_exact(gpu, oracle, scale):
+    # A region scaled by 1e18 keeps every guide value below 2^64, where the
+    # ratio takes the shortened division; 1e20 pushes guide values past it,
+    # so every tile takes the full division. Both are the oracle's bits.
+    spec = synth.SceneSpec(320, 96, 23)
+    m = ((synth.mosaic(spec).astype(F) - F(512)) / F(16383 - 512)).astype(F)
+    m[20:70, 100:230] *= F(scale)
+    dd = _detail(amount=0.5, sigma=0.7, noise=0.5, ceilings=(1.0, 1.0, 1.0), iterations=3)
+    out = gpu_demosaic(gpu, m, 0, (1, 1, 1), "mhc", detail=dd)
+    _, base = oracle.demosaic(m, 0, (1, 1, 1), "mhc")
+    _, expected = oracle.capture(base, 0.5, 0.7, 0.5, (1, 1, 1), iterations=3, threads=8)
+    assert bits_equal(out, expected), f"scale {scale}: " + describe_mismatch(out, expected)
+
+
 @pytest.mark.parametrize("iterations", [0, 1, 10, 20])
 def test_capture_iteration_counts(gpu, oracle, iterations):
     spec = synth.SceneSpec(200, 150, 5)
