@@ -12,8 +12,9 @@
 // when capture directly follows it), prep (the mask's two blurs and
 // epilogues, and the first iteration's ratio from the same blur of the
 // guide), one fused kernel per RL iteration (blur estimate -> ratio -> blur
-// ratio -> clamped update, all in LDS for a 64x16 output tile with a 2r
-// halo; the first starts from prep's ratio), the last of which applies the
+// ratio -> clamped update, all in LDS for a 64x32 output tile with a 2r
+// halo; the first starts from prep's ratio on 64x16 tiles), the last of
+// which applies the
 // gain to RGB instead of storing its estimate, and a plain gain apply for
 // zero iterations. Iterations ping-pong between two estimate planes. Per
 // iteration the estimate (with halo) and the guide are read and the estimate
@@ -30,6 +31,9 @@ namespace rhd {
 namespace {
 
 constexpr int TW = 64, TH = 16, MAXR = 4, THREADS = 256;
+// The RL iteration's tiles are taller, with a wider workgroup: the vertical
+// halos are paid per tile.
+constexpr int ITH = 32, ITHREADS = 512;
 constexpr float FLOOR = CAPTURE_FLOOR;
 
 struct Weights {
@@ -84,9 +88,10 @@ __global__ __launch_bounds__(THREADS) void cap_init(const float* __restrict__ rg
 // so every load of the window is issued before the first LDS store and the
 // memory latency is paid once. A wavefront stages whole rows, so each row's
 // address is wave-uniform and formed in scalar registers.
-template <int MAXROWS, int MAXCOLS>
+template <int MAXROWS, int MAXCOLS, int NT = THREADS>
 struct Staged {
-    static constexpr int RI = (MAXROWS + 3) / 4, CI = (MAXCOLS + 63) / 64;
+    static constexpr int WAVES = NT / 64;
+    static constexpr int RI = (MAXROWS + WAVES - 1) / WAVES, CI = (MAXCOLS + 63) / 64;
     float v[RI * CI];
 
     // The window whose first element is src (inside the image).
@@ -95,7 +100,7 @@ struct Staged {
         const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tx = threadIdx.x & 63;
 #pragma unroll
         for (int i = 0; i < RI; ++i) {
-            const int r = ty + 4 * i;
+            const int r = ty + WAVES * i;
             const float* row = src + (size_t)r * src_pitch;
 #pragma unroll
             for (int j = 0; j < CI; ++j) {
@@ -115,7 +120,7 @@ struct Staged {
         for (int j = 0; j < CI; ++j) col[j] = mirror_near(left + tx + 64 * j, width);
 #pragma unroll
         for (int i = 0; i < RI; ++i) {
-            const int r = ty + 4 * i;
+            const int r = ty + WAVES * i;
             const float* row = plane + (size_t)mirror_near(top + r, height) * width;
 #pragma unroll
             for (int j = 0; j < CI; ++j) {
@@ -130,7 +135,7 @@ struct Staged {
         for (int i = 0; i < RI; ++i)
 #pragma unroll
             for (int j = 0; j < CI; ++j) {
-                const int r = ty + 4 * i, c = tx + 64 * j;
+                const int r = ty + WAVES * i, c = tx + 64 * j;
                 if (r < rows && c < cols) lds[r * pitch + c] = v[i * CI + j];
             }
     }
@@ -138,10 +143,10 @@ struct Staged {
 
 // Copies the (ROWS x COLS) window at image position (top, left) into LDS,
 // mirrored at the image edges when BORDER.
-template <int ROWS, int COLS, bool BORDER>
+template <int ROWS, int COLS, bool BORDER, int NT = THREADS>
 __device__ __forceinline__ void stage(float* lds, int pitch, const float* __restrict__ plane,
                                       int width, int height, int top, int left) {
-    Staged<ROWS, COLS> w;
+    Staged<ROWS, COLS, NT> w;
     if (BORDER)
         w.load_mirrored(plane, width, height, top, left, ROWS, COLS);
     else
@@ -163,10 +168,10 @@ __device__ __forceinline__ void stage(float* lds, int pitch, const float* __rest
 //   its horizontal blur                          is x0 - EL + r,
 //   the guide / ratio window (GH rows from y0 - r) is x0 - 4,
 //   the ratio's horizontal blur                  is x0 - 4 + r.
-template <int R>
+template <int R, int TILE_H = TH>
 struct Windows {
-    static constexpr int EW = TW + 4 * R, EH = TH + 4 * R;   // estimate (2r halo)
-    static constexpr int GW = TW + 2 * R, GH = TH + 2 * R;   // guide / ratio (r halo)
+    static constexpr int EW = TW + 4 * R, EH = TILE_H + 4 * R;   // estimate (2r halo)
+    static constexpr int GW = TW + 2 * R, GH = TILE_H + 2 * R;   // guide / ratio (r halo)
     static constexpr int CHUNKS = (4 + 2 * R + 3) / 4;      // float4 reads per strip
     static constexpr int EL = (2 * R + 3) / 4 * 4;          // estimate halo rounded up to 4
     static constexpr int EP = TW + 2 * EL;                  // estimate pitch
@@ -192,11 +197,11 @@ struct Windows {
 // runs. Lane l of a wavefront takes chunk l % F4 of row l / F4 of each group of
 // 64 / F4 rows the wavefront owns, so row addresses stay wave-uniform and a
 // lane's offset within the group is formed once.
-template <int ROWS, int F4>
+template <int ROWS, int F4, int NT = THREADS>
 struct Staged4 {
     using vf4 = float __attribute__((ext_vector_type(4)));
     static constexpr int RPG = 64 / F4, GROUPS = (ROWS + RPG - 1) / RPG;
-    static constexpr int WAVES = THREADS / 64, SLOTS = (GROUPS + WAVES - 1) / WAVES;
+    static constexpr int WAVES = NT / 64, SLOTS = (GROUPS + WAVES - 1) / WAVES;
     vf4 v[SLOTS];
 
     __device__ __forceinline__ void load(const float* __restrict__ src, size_t pitch) {
@@ -228,16 +233,15 @@ struct Staged4 {
 // column advance by THREADS per trip instead of being divided out of the item
 // each time; only the last trip tests the item against the grid, and a
 // wavefront whose first item is past the grid skips that trip as a whole.
-template <int ROWS, int COLS, typename F>
+template <int ROWS, int COLS, int NT = THREADS, typename F>
 __device__ __forceinline__ void for_items(F f) {
-    constexpr int N = ROWS * COLS, TRIPS = (N + THREADS - 1) / THREADS;
-    constexpr int DR = THREADS / COLS, DC = THREADS % COLS;
+    constexpr int N = ROWS * COLS, TRIPS = (N + NT - 1) / NT;
+    constexpr int DR = NT / COLS, DC = NT % COLS;
     const int wave_first = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
     int r = (int)threadIdx.x / COLS, c = (int)threadIdx.x % COLS;
 #pragma unroll
     for (int t = 0; t < TRIPS; ++t) {
-        if (t + 1 < TRIPS ||
-            (wave_first + THREADS * t < N && (int)threadIdx.x + THREADS * t < N))
+        if (t + 1 < TRIPS || (wave_first + NT * t < N && (int)threadIdx.x + NT * t < N))
             f(r, c);
         r += DR;
         c += DC;
@@ -295,18 +299,18 @@ __device__ __forceinline__ void vstrip(const float* src, const Weights& k, float
 // position from its mirrored image position: whole rows first, then columns,
 // so corners take the doubly mirrored value. Sources lie inside the image and
 // destinations outside, so no copy reads another copy's output.
-template <int R, int ROWS, int COLS>
+template <int R, int ROWS, int COLS, int NT = THREADS>
 __device__ void mirror_window_rows(float* s, int pitch, int top, int height) {
-    for (int item = threadIdx.x; item < 2 * R * COLS; item += THREADS) {
+    for (int item = threadIdx.x; item < 2 * R * COLS; item += NT) {
         const int k = item / COLS, c = item - k * COLS;
         const int y = k < R ? k - R : height + (k - R);
         const int r = y - top, src = mirror_near(y, height) - top;
         if (r >= 0 && r < ROWS && src >= 0 && src < ROWS) s[r * pitch + c] = s[src * pitch + c];
     }
 }
-template <int R, int ROWS, int COLS>
+template <int R, int ROWS, int COLS, int NT = THREADS>
 __device__ void mirror_window_cols(float* s, int pitch, int left, int width) {
-    for (int item = threadIdx.x; item < 2 * R * ROWS; item += THREADS) {
+    for (int item = threadIdx.x; item < 2 * R * ROWS; item += NT) {
         const int r = item / (2 * R), k = item - r * (2 * R);
         const int x = k < R ? k - R : width + (k - R);
         const int c = x - left, src = mirror_near(x, width) - left;
@@ -440,27 +444,28 @@ template <int R, bool APPLY, bool BORDER, bool IN_RANGE>
 __device__ void iterate_strips(const float* __restrict__ est_in, const float* __restrict__ guide,
                                float* __restrict__ est_out, int width, int height,
                                const Weights& k, const Apply& ap, float* s_a, float* s_b) {
-    using Win = Windows<R>;
+    using Win = Windows<R, ITH>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
     constexpr int EP = Win::EP, EL = Win::EL, GD = Win::GD, NS1 = Win::NS1, NS3 = Win::NS3;
     constexpr int NR2 = (GH + 3) / 4;   // row strips of the blurred estimate
-    constexpr int NR4 = TH / 4;         // row strips of the output
-    constexpr int OUT_STRIPS = NR4 * TW / THREADS;
+    constexpr int NR4 = ITH / 4;        // row strips of the output
+    constexpr int OUT_STRIPS = NR4 * TW / ITHREADS;
     static_assert(TW == 64, "output strips map one wavefront to one tile row");
     static_assert(OUT_STRIPS >= 1, "every thread owns at least one output strip");
     const uint3 blk = xcd_block();
-    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    const int x0 = blk.x * TW, y0 = blk.y * ITH;
     // The guide window waits in registers while the estimate window, which
     // shares its LDS, is blurred. Border tiles stage both windows element by
     // element, half-sample mirrored; interior ones as aligned float4 rows
     // that include the columns between the aligned start and the window.
-    std::conditional_t<BORDER, Staged<GH, GW>, Staged4<GH, P / 4>> gw;
+    std::conditional_t<BORDER, Staged<GH, GW, ITHREADS>, Staged4<GH, P / 4, ITHREADS>> gw;
     if constexpr (BORDER) {
         gw.load_mirrored(guide, width, height, y0 - R, x0 - R, GH, GW);
-        stage<EH, EW, true>(s_a + Win::ED, EP, est_in, width, height, y0 - 2 * R, x0 - 2 * R);
+        stage<EH, EW, true, ITHREADS>(s_a + Win::ED, EP, est_in, width, height, y0 - 2 * R,
+                                      x0 - 2 * R);
     } else {
         gw.load(guide + (size_t)(y0 - R) * width + x0 - 4, width);
-        Staged4<EH, EP / 4> ew;
+        Staged4<EH, EP / 4, ITHREADS> ew;
         ew.load(est_in + (size_t)(y0 - 2 * R) * width + x0 - EL, width);
         ew.store(s_a, EP);
     }
@@ -468,11 +473,11 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     float e_out[OUT_STRIPS][4], g_out[OUT_STRIPS][4];
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
-        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
 #pragma unroll
         for (int i = 0; i < 4; ++i) e_out[q][i] = s_a[(r0 + i + 2 * R) * EP + c + EL];
     }
-    for_items<EH, NS1>([&](int r, int j) {
+    for_items<EH, NS1, ITHREADS>([&](int r, int j) {
         *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * EP + 4 * j, k);
     });
     __syncthreads();
@@ -483,14 +488,14 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
-        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+        const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
 #pragma unroll
         for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + 4];
     }
     __syncthreads();
     // The ratio at guide column c (LDS column c + GD) takes the estimate's
     // blur at LDS column c + ED of s_b, the same image column.
-    for_items<NR2, GW>([&](int strip, int c) {
+    for_items<NR2, GW, ITHREADS>([&](int strip, int c) {
         const int r0 = 4 * strip;
         float b[4];
         vstrip<R, P>(s_b + r0 * P + c + Win::ED, k, b);
@@ -505,12 +510,12 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     __syncthreads();
     if constexpr (BORDER) {
         // The ratio's blur reads up to r positions across the image edge.
-        mirror_window_rows<R, GH, GW>(s_a + GD, P, y0 - R, height);
+        mirror_window_rows<R, GH, GW, ITHREADS>(s_a + GD, P, y0 - R, height);
         __syncthreads();
-        mirror_window_cols<R, GH, GW>(s_a + GD, P, x0 - R, width);
+        mirror_window_cols<R, GH, GW, ITHREADS>(s_a + GD, P, x0 - R, width);
         __syncthreads();
     }
-    for_items<GH, NS3>([&](int r, int j) {
+    for_items<GH, NS3, ITHREADS>([&](int r, int j) {
         *reinterpret_cast<float4*>(s_b + r * P + 4 * j) = hstrip<R>(s_a + r * P + 4 * j, k);
     });
     __syncthreads();
@@ -518,7 +523,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     for (int q = 0; q < OUT_STRIPS; ++q) {
         // TW == 64: a wavefront's strips share their rows.
         const int c = threadIdx.x & 63;
-        const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
+        const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
         float b[4];
         vstrip<R, P>(s_b + r0 * P + c + GD, k, b);
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
@@ -540,22 +545,22 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
 }
 
 template <int R, bool APPLY>
-__global__ __launch_bounds__(THREADS) void cap_iterate(const float* __restrict__ est_in,
+__global__ __launch_bounds__(ITHREADS) void cap_iterate(const float* __restrict__ est_in,
                                                        const float* __restrict__ guide,
                                                        float* __restrict__ est_out, int width,
                                                        int height, Weights k, Apply ap,
                                                        bool aligned,
                                                        const unsigned* __restrict__ range_flag) {
-    using Win = Windows<R>;
+    using Win = Windows<R, ITH>;
     __shared__ __align__(16) float s_a[Win::A_FLOATS];
     __shared__ __align__(16) float s_b[Win::H_FLOATS];
     const uint3 blk = xcd_block();
-    const int x0 = blk.x * TW, y0 = blk.y * TH;
+    const int x0 = blk.x * TW, y0 = blk.y * ITH;
     // Interior: the aligned windows lie inside the image and the planes' rows
     // start 16-byte aligned (aligned: the width is a multiple of 4 and the
     // planes are 16-byte aligned).
     const bool interior = aligned && x0 - Win::EL >= 0 && x0 + TW + Win::EL <= width &&
-                          y0 - 2 * R >= 0 && y0 + TH + 2 * R <= height;
+                          y0 - 2 * R >= 0 && y0 + ITH + 2 * R <= height;
     // With every guide value in [floor, 2^64) (cap_prep's flag), every
     // estimate lies in [floor / 2, 2^65] (the clamp) and so does its blur,
     // whose weights sum to 1 within a few ulps: the ratio's operands are in
@@ -678,6 +683,7 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
                                      a.ceilings[0], a.ceilings[1], a.ceilings[2], a.guide,
                                      a.mask, a.stats));
     const dim3 tiles((a.width + TW - 1) / TW, (a.height + TH - 1) / TH);
+    const dim3 iter_tiles((a.width + TW - 1) / TW, (a.height + ITH - 1) / ITH);
     // Interior tiles move their windows as float4 when every row of the
     // planes they read starts 16-byte aligned.
     auto aligned16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
@@ -709,10 +715,10 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     auto iterate = [&](const float* in, float* out, auto apply) {
         constexpr bool APPLY = decltype(apply)::value;
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL((cap_iterate<1, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
-            case 2: hipLaunchKernelGGL((cap_iterate<2, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
-            case 3: hipLaunchKernelGGL((cap_iterate<3, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
-            default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), tiles, dim3(THREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            case 1: hipLaunchKernelGGL((cap_iterate<1, APPLY>), iter_tiles, dim3(ITHREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            case 2: hipLaunchKernelGGL((cap_iterate<2, APPLY>), iter_tiles, dim3(ITHREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            case 3: hipLaunchKernelGGL((cap_iterate<3, APPLY>), iter_tiles, dim3(ITHREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
+            default: hipLaunchKernelGGL((cap_iterate<4, APPLY>), iter_tiles, dim3(ITHREADS), 0, s, in, a.guide, out, a.width, a.height, k, ap, aligned, &a.stats->capture_range); break;
         }
     };
     // Reads the mask and the guide, writes the final mask and the ratio.

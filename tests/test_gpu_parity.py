@@ -193,14 +193,16 @@ def test_capture_bit_exact(gpu, oracle, size, sigma):
 
 
 @pytest.mark.parametrize("size", [(63, 17), (65, 15), (128, 16), (129, 33), (40, 70), (100, 9),
-                                  (136, 44)])
+                                  (136, 44), (136, 31), (136, 32), (136, 70), (140, 69)])
 @pytest.mark.parametrize("sigma", [0.3, 0.6, 0.7, 1.2])
 def test_capture_tile_edges_bit_exact(gpu, oracle, size, sigma):
-    # Widths and heights one below, at and one above the 64x16 output tile,
-    # and windows that reach across two image edges at once, for every
-    # radius: the edge tiles' mirrored windows and the repaired ratio rows
-    # and columns, through the prepared first iteration, a middle one and
-    # the applying last one.
+    # Widths and heights one below, at and one above the 64x16 tiles of the
+    # preparation and the first iteration and the 64x32 tiles of the later
+    # ones, heights whose middle tile row is interior or just not, and
+    # windows that reach across two image edges at once, for every radius:
+    # the edge tiles' mirrored windows and the repaired ratio rows and
+    # columns, through the prepared first iteration, a middle one and the
+    # applying last one.
     w, h = size
     spec = synth.SceneSpec(max(w, 8), max(h, 8), 7 + w * h, clipped_fraction=0.05)
     m = ((synth.mosaic(spec)[:h, :w].astype(F) - F(512)) / F(16383 - 512)).astype(F)
