@@ -166,8 +166,12 @@ __device__ __forceinline__ void stage(float* lds, int pitch, const float* __rest
 // strip reads aligned chunks. For a tile at (x0, y0), LDS column 0 of
 //   the estimate window (EH rows from y0 - 2r)   is image column x0 - EL,
 //   its horizontal blur                          is x0 - EL + r,
-//   the guide / ratio window (GH rows from y0 - r) is x0 - 4,
-//   the ratio's horizontal blur                  is x0 - 4 + r.
+//   the guide / ratio window (GH rows from y0 - r) is x0 - r,
+//   the ratio's horizontal blur                  is x0.
+// Interior guide and ratio windows load as float4 from the aligned column
+// x0 - 4 and land GS columns left of their LDS origin: the columns left of
+// the window fall in the previous row's padding (in front of the array for
+// the first row), those right of it in the row's own padding.
 template <int R, int TILE_H = TH>
 struct Windows {
     static constexpr int EW = TW + 4 * R, EH = TILE_H + 4 * R;   // estimate (2r halo)
@@ -176,16 +180,16 @@ struct Windows {
     static constexpr int EL = (2 * R + 3) / 4 * 4;          // estimate halo rounded up to 4
     static constexpr int EP = TW + 2 * EL;                  // estimate pitch
     static constexpr int ED = EL - 2 * R;                   // LDS column of image column x0 - 2r
-    static constexpr int GD = 4 - R;                        // LDS column of image column x0 - r
+    static constexpr int GS = 4 - R;                        // aligned load start, left of x0 - r
     static constexpr int P = 72;                            // guide / ratio and blur pitch
     static constexpr int NS1 = (TW + EL) / 4;               // estimate strips per row
-    static constexpr int NS3 = (TW + GD + 3) / 4;           // ratio strips per row
+    static constexpr int NS3 = TW / 4;                      // ratio strips per row
     static constexpr int E_FLOATS = EH * EP;
     static constexpr int H_FLOATS = (EH + 4) * P;
     static constexpr int G_FLOATS = GH * P + 4;
     // The estimate window, then the guide / ratio window, share one array.
     static constexpr int A_FLOATS = E_FLOATS > G_FLOATS ? E_FLOATS : G_FLOATS;
-    static_assert(GD + GW <= P && ED + EW <= EP, "windows fit their pitch");
+    static_assert(GW + GS <= P && ED + EW <= EP, "windows and their load padding fit the pitch");
     static_assert(4 * NS1 <= P && 4 * (NS1 - 1) + 4 * CHUNKS <= EP,
                   "estimate strips cover the blur the ratio needs and stay in a row");
     static_assert((GH - 1) * P + 4 * (NS3 - 1) + 4 * CHUNKS <= G_FLOATS,
@@ -216,14 +220,23 @@ struct Staged4 {
                 v[k] = *reinterpret_cast<const vf4*>(group + lane);
         }
     }
+    // Row r's chunk c goes to lds + r * pitch + 4 c - SHIFT (float4 stores
+    // when SHIFT keeps them aligned, single words otherwise).
+    template <int SHIFT = 0>
     __device__ __forceinline__ void store(float* lds, int pitch) const {
         const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
         const int dr = l / F4, dc = l - dr * F4;
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             const int g = w + WAVES * k, r = g * RPG + dr;
-            if (g < GROUPS && dr < RPG && r < ROWS)
-                *reinterpret_cast<vf4*>(lds + r * pitch + 4 * dc) = v[k];
+            if (!(g < GROUPS && dr < RPG && r < ROWS)) continue;
+            float* dst = lds + r * pitch + 4 * dc - SHIFT;
+            if constexpr (SHIFT % 4 == 0) {
+                *reinterpret_cast<vf4*>(dst) = v[k];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) dst[q] = v[k][q];
+            }
         }
     }
 };
@@ -446,7 +459,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
                                const Weights& k, const Apply& ap, float* s_a, float* s_b) {
     using Win = Windows<R, ITH>;
     constexpr int EW = Win::EW, EH = Win::EH, GW = Win::GW, GH = Win::GH, P = Win::P;
-    constexpr int EP = Win::EP, EL = Win::EL, GD = Win::GD, NS1 = Win::NS1, NS3 = Win::NS3;
+    constexpr int EP = Win::EP, EL = Win::EL, NS1 = Win::NS1, NS3 = Win::NS3;
     constexpr int NR2 = (GH + 3) / 4;   // row strips of the blurred estimate
     constexpr int NR4 = ITH / 4;        // row strips of the output
     constexpr int OUT_STRIPS = NR4 * TW / ITHREADS;
@@ -482,18 +495,18 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     });
     __syncthreads();
     if constexpr (BORDER)
-        gw.store(s_a + GD, P, GH, GW);
+        gw.store(s_a, P, GH, GW);
     else
-        gw.store(s_a, P);
+        gw.template store<Win::GS>(s_a, P);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + 4];
+        for (int i = 0; i < 4; ++i) g_out[q][i] = s_a[(r0 + i + R) * P + c + R];
     }
     __syncthreads();
-    // The ratio at guide column c (LDS column c + GD) takes the estimate's
+    // The ratio at guide column c takes the estimate's
     // blur at LDS column c + ED of s_b, the same image column.
     for_items<NR2, GW, ITHREADS>([&](int strip, int c) {
         const int r0 = 4 * strip;
@@ -502,7 +515,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (r0 + i >= GH) continue;
-            float* ratio = s_a + (r0 + i) * P + c + GD;
+            float* ratio = s_a + (r0 + i) * P + c;
             const float den = num_max(b[i], FLOOR);
             *ratio = IN_RANGE ? div_in_range(*ratio, den) : *ratio / den;
         }
@@ -510,9 +523,9 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     __syncthreads();
     if constexpr (BORDER) {
         // The ratio's blur reads up to r positions across the image edge.
-        mirror_window_rows<R, GH, GW, ITHREADS>(s_a + GD, P, y0 - R, height);
+        mirror_window_rows<R, GH, GW, ITHREADS>(s_a, P, y0 - R, height);
         __syncthreads();
-        mirror_window_cols<R, GH, GW, ITHREADS>(s_a + GD, P, x0 - R, width);
+        mirror_window_cols<R, GH, GW, ITHREADS>(s_a, P, x0 - R, width);
         __syncthreads();
     }
     for_items<GH, NS3, ITHREADS>([&](int r, int j) {
@@ -525,7 +538,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
         const int c = threadIdx.x & 63;
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
         float b[4];
-        vstrip<R, P>(s_b + r0 * P + c + GD, k, b);
+        vstrip<R, P>(s_b + r0 * P + c, k, b);
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -552,7 +565,9 @@ __global__ __launch_bounds__(ITHREADS) void cap_iterate(const float* __restrict_
                                                        bool aligned,
                                                        const unsigned* __restrict__ range_flag) {
     using Win = Windows<R, ITH>;
-    __shared__ __align__(16) float s_a[Win::A_FLOATS];
+    // Four floats in front of s_a take the first guide row's load padding.
+    __shared__ __align__(16) float s_a_mem[Win::A_FLOATS + 4];
+    float* const s_a = s_a_mem + 4;
     __shared__ __align__(16) float s_b[Win::H_FLOATS];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * ITH;
@@ -586,7 +601,7 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
                                      int width, int height, const Weights& k, const Apply& ap,
                                      float* s_a, float* s_b) {
     using Win = Windows<R>;
-    constexpr int GW = Win::GW, GH = Win::GH, P = Win::P, GD = Win::GD, NS3 = Win::NS3;
+    constexpr int GW = Win::GW, GH = Win::GH, P = Win::P, NS3 = Win::NS3;
     constexpr int OUT_STRIPS = (TH / 4) * TW / THREADS;
     static_assert(TW == 64 && OUT_STRIPS >= 1, "output strips map one wavefront to one tile row");
     const uint3 blk = xcd_block();
@@ -603,11 +618,11 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
         }
     }
     if constexpr (BORDER) {
-        stage<GH, GW, true>(s_a + GD, P, ratio, width, height, y0 - R, x0 - R);
+        stage<GH, GW, true>(s_a, P, ratio, width, height, y0 - R, x0 - R);
     } else {
         Staged4<GH, P / 4> rw;
         rw.load(ratio + (size_t)(y0 - R) * width + x0 - 4, width);
-        rw.store(s_a, P);
+        rw.template store<Win::GS>(s_a, P);
     }
     __syncthreads();
     for_items<GH, NS3>([&](int r, int j) {
@@ -618,7 +633,7 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
     for (int q = 0; q < OUT_STRIPS; ++q) {
         const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + q * (THREADS / 64));
         float b[4];
-        vstrip<R, P>(s_b + r0 * P + c + GD, k, b);
+        vstrip<R, P>(s_b + r0 * P + c, k, b);
         const size_t o = (size_t)(y0 + r0) * width + x0 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -640,7 +655,9 @@ __global__ __launch_bounds__(THREADS) void cap_iterate_first(const float* __rest
                                                              int width, int height, Weights k,
                                                              Apply ap, bool aligned) {
     using Win = Windows<R>;
-    __shared__ __align__(16) float s_a[Win::G_FLOATS];
+    // Four floats in front of s_a take the first ratio row's load padding.
+    __shared__ __align__(16) float s_a_mem[Win::G_FLOATS + 4];
+    float* const s_a = s_a_mem + 4;
     __shared__ __align__(16) float s_b[Win::G_FLOATS];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
