@@ -345,7 +345,12 @@ __device__ void prep_strips(const float* __restrict__ mask, const float* __restr
                             float* __restrict__ mask_out, float* __restrict__ ratio_out,
                             int width, int height, const Weights& k, float noise_protection,
                             unsigned* range_flag, float* s_in, float* s_h) {
-    constexpr int W = TW + 2 * R, H = TH + 2 * R, P = (W + 3) / 4 * 4;
+    // Window column 0 is image column x0 - r. Interior windows load as
+    // float4 rows from the aligned column x0 - 4 and land GS columns left of
+    // the window, in the previous row's padding (in front of s_in for the
+    // first row) and the row's own.
+    constexpr int W = TW + 2 * R, H = TH + 2 * R, P = 72, GS = 4 - R;
+    static_assert(W + GS <= P, "the aligned rows' padding stays in the pitch");
     static_assert(4 * (TW / 4 - 1) + Windows<R>::CHUNKS * 4 <= P, "strips stay in a row");
     static_assert(H * P <= (TH + 2 * MAXR) * (TW + 2 * MAXR), "the window fits s_in");
     static_assert(TH * TW == 4 * THREADS, "one vertical strip per thread");
@@ -354,14 +359,20 @@ __device__ void prep_strips(const float* __restrict__ mask, const float* __restr
     const int c = threadIdx.x & 63;
     const int r0 = 4 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     // Both windows are requested up front; the guide's waits in registers.
-    Staged<H, W> mw, gw;
-    if (BORDER) {
+    std::conditional_t<BORDER, Staged<H, W>, Staged4<H, P / 4>> mw, gw;
+    if constexpr (BORDER) {
         mw.load_mirrored(mask, width, height, y0 - R, x0 - R, H, W);
         gw.load_mirrored(guide, width, height, y0 - R, x0 - R, H, W);
     } else {
-        mw.load(mask + (size_t)(y0 - R) * width + x0 - R, width, H, W);
-        gw.load(guide + (size_t)(y0 - R) * width + x0 - R, width, H, W);
+        mw.load(mask + (size_t)(y0 - R) * width + x0 - 4, width);
+        gw.load(guide + (size_t)(y0 - R) * width + x0 - 4, width);
     }
+    auto put = [&](const auto& win) {
+        if constexpr (BORDER)
+            win.store(s_in, P, H, W);
+        else
+            win.template store<GS>(s_in, P);
+    };
     float m[4], bm[4], g[4], bg[4];
     // One blur of the window in s_in into b (this thread's 4 outputs); the
     // window's values at the outputs go to centre.
@@ -376,11 +387,11 @@ __device__ void prep_strips(const float* __restrict__ mask, const float* __restr
         __syncthreads();
         vstrip<R, TW>(s_h + r0 * TW + c, k, b);
     };
-    mw.store(s_in, P, H, W);
+    put(mw);
     blur(bm, m);
     // Every read of s_in precedes blur's second barrier, and the guide's
     // horizontal pass writes s_h only after its first one.
-    gw.store(s_in, P, H, W);
+    put(gw);
     blur(bg, g);
     bool wide = false;
 #pragma unroll
@@ -408,12 +419,17 @@ __global__ __launch_bounds__(THREADS) void cap_prep(const float* __restrict__ ma
                                                     float* __restrict__ mask_out,
                                                     float* __restrict__ ratio_out, int width,
                                                     int height, Weights k, float noise_protection,
-                                                    unsigned* range_flag) {
-    __shared__ __align__(16) float s_in[(TH + 2 * MAXR) * (TW + 2 * MAXR)];
+                                                    unsigned* range_flag, bool aligned) {
+    // Four floats in front of s_in take the first window row's load padding.
+    __shared__ __align__(16) float s_in_mem[(TH + 2 * MAXR) * (TW + 2 * MAXR) + 4];
+    float* const s_in = s_in_mem + 4;
     __shared__ __align__(16) float s_h[(TH + 2 * MAXR) * TW];
     const uint3 blk = xcd_block();
     const int x0 = blk.x * TW, y0 = blk.y * TH;
-    const bool interior = x0 - R >= 0 && x0 + TW + R <= width && y0 - R >= 0 && y0 + TH + R <= height;
+    // Interior: the aligned windows (image columns x0 - 4 to x0 + 67) lie
+    // inside the image and the planes' rows start 16-byte aligned.
+    const bool interior = aligned && x0 - 4 >= 0 && x0 + TW + 4 <= width && y0 - R >= 0 &&
+                          y0 + TH + R <= height;
     if (interior)
         prep_strips<R, false>(mask, guide, mask_out, ratio_out, width, height, k, noise_protection,
                               range_flag, s_in, s_h);
@@ -704,8 +720,8 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // Interior tiles move their windows as float4 when every row of the
     // planes they read starts 16-byte aligned.
     auto aligned16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
-    const bool aligned = a.width % 4 == 0 && aligned16(a.guide) && aligned16(a.est_a) &&
-                         aligned16(a.est_b);
+    const bool aligned = a.width % 4 == 0 && aligned16(a.guide) && aligned16(a.mask) &&
+                         aligned16(a.est_a) && aligned16(a.est_b);
     // The final mask goes to tmp and the first iteration's ratio to est_b,
     // which the first iteration reads before the second one overwrites it.
     float* mask = a.tmp;
@@ -714,10 +730,10 @@ hipError_t launch_capture(const CaptureArgs& a, hipStream_t s, Profiler* prof) {
     // The radius is a template parameter so the tap loops unroll.
     auto prep = [&] {
         switch (a.radius) {
-            case 1: hipLaunchKernelGGL(cap_prep<1>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
-            case 2: hipLaunchKernelGGL(cap_prep<2>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
-            case 3: hipLaunchKernelGGL(cap_prep<3>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
-            default: hipLaunchKernelGGL(cap_prep<4>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range); break;
+            case 1: hipLaunchKernelGGL(cap_prep<1>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range, aligned); break;
+            case 2: hipLaunchKernelGGL(cap_prep<2>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range, aligned); break;
+            case 3: hipLaunchKernelGGL(cap_prep<3>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range, aligned); break;
+            default: hipLaunchKernelGGL(cap_prep<4>, tiles, dim3(THREADS), 0, s, a.mask, a.guide, mask, ratio, a.width, a.height, k, a.noise_protection, &a.stats->capture_range, aligned); break;
         }
     };
     auto first = [&](float* out, auto apply) {
