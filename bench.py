@@ -892,8 +892,8 @@ def detail_bench(args):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
-                     "launches_per_step": dom_launches / sampled,
-                "sampled_steps": sampled,
+                     "launches_per_step": dom_launches / args.steps,
+                     "sampled_steps": args.steps,
                      "algorithmic_bytes_per_launch": int(dom_bytes),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4)},
