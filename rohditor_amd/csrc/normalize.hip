@@ -15,7 +15,7 @@ namespace rhd {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int PER_THREAD = 4;
+constexpr int PER_THREAD = 4, ROWS_IN_FLIGHT = 4;
 // Row blocks per launch: each block strides over height / ROW_BLOCKS rows.
 constexpr int ROW_BLOCKS = 256;
 
@@ -75,15 +75,12 @@ __global__ __launch_bounds__(BLOCK) void normalize_kernel(NormalizeArgs a, float
     __syncthreads();
     const int x0 = (blockIdx.x * BLOCK + threadIdx.x) * PER_THREAD;
     Counters k = {{0, 0, 0, 0, 0, 0}};
-    for (int y = blockIdx.y; y < a.height && x0 < a.width; y += gridDim.y) {
-        const int sy = a.crop_y + y;
-        const int row_t = (sy % a.table.period_y) * px;
-        int phase = (a.crop_x + x0) % px;
-        const uint64_t first = (uint64_t)sy * a.row_stride + a.crop_x;
+    const bool full = x0 + PER_THREAD <= a.width;
+    // The samples of ROWS_IN_FLIGHT rows are requested before the first of
+    // them is converted, so each thread keeps that many loads in flight.
+    auto load = [&](int y, uint16_t raw[PER_THREAD]) {
+        const uint64_t first = (uint64_t)(a.crop_y + y) * a.row_stride + a.crop_x;
         const uint16_t* src = a.mosaic + first;
-        float* dst = out + (size_t)y * a.width;
-        const bool full = x0 + PER_THREAD <= a.width;
-        uint16_t raw[PER_THREAD];
         if (full && (first & 3) == 0) {
             const ushort4 q = *reinterpret_cast<const ushort4*>(src + x0);
             raw[0] = q.x;
@@ -94,6 +91,12 @@ __global__ __launch_bounds__(BLOCK) void normalize_kernel(NormalizeArgs a, float
 #pragma unroll
             for (int i = 0; i < PER_THREAD; ++i) raw[i] = x0 + i < a.width ? src[x0 + i] : 0;
         }
+    };
+    auto convert = [&](int y, const uint16_t raw[PER_THREAD]) {
+        const int sy = a.crop_y + y;
+        const int row_t = (sy % a.table.period_y) * px;
+        int phase = (a.crop_x + x0) % px;
+        float* dst = out + (size_t)y * a.width;
         float v[PER_THREAD];
 #pragma unroll
         for (int i = 0; i < PER_THREAD; ++i) {
@@ -109,6 +112,16 @@ __global__ __launch_bounds__(BLOCK) void normalize_kernel(NormalizeArgs a, float
             for (int i = 0; i < PER_THREAD; ++i)
                 if (x0 + i < a.width) dst[x0 + i] = v[i];
         }
+    };
+    const int step = (int)gridDim.y;
+    for (int y0 = blockIdx.y; y0 < a.height && x0 < a.width; y0 += ROWS_IN_FLIGHT * step) {
+        uint16_t raw[ROWS_IN_FLIGHT][PER_THREAD];
+#pragma unroll
+        for (int u = 0; u < ROWS_IN_FLIGHT; ++u)
+            if (y0 + u * step < a.height) load(y0 + u * step, raw[u]);
+#pragma unroll
+        for (int u = 0; u < ROWS_IN_FLIGHT; ++u)
+            if (y0 + u * step < a.height) convert(y0 + u * step, raw[u]);
     }
     if (a.mode == 1) flush_counters(k, 6, stats->clip);
     if (a.mode == 2) flush_counters(k, 6, stats->detect);
