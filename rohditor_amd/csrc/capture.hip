@@ -448,8 +448,10 @@ struct Apply {
     float amount;
     size_t n;
 };
-__device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est, float guide) {
-    const float gain = 1.0f + ap.amount * ap.mask[i] * (est / guide - 1.0f);
+// `mask` is the mask value at i, read early by callers that can.
+__device__ __forceinline__ void apply_gain(const Apply& ap, size_t i, float est, float guide,
+                                           float mask) {
+    const float gain = 1.0f + ap.amount * mask * (est / guide - 1.0f);
     if (gain == 1.0f) return;
     const float r = ap.rgb[i], g = ap.rgb[ap.n + i], b = ap.rgb[2 * ap.n + i];
     if (is_finite(r * gain) && is_finite(g * gain) && is_finite(b * gain)) {
@@ -488,6 +490,21 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
     // element, half-sample mirrored; interior ones as aligned float4 rows
     // that include the columns between the aligned start and the window.
     std::conditional_t<BORDER, Staged<GH, GW, ITHREADS>, Staged4<GH, P / 4, ITHREADS>> gw;
+    // The applying iteration requests the mask at its outputs up front, so
+    // its latency hides behind the blurs.
+    float m_out[OUT_STRIPS][4];
+    if constexpr (APPLY) {
+#pragma unroll
+        for (int q = 0; q < OUT_STRIPS; ++q) {
+            const int c = threadIdx.x & 63, r0 = 4 * ((int)(threadIdx.x >> 6) + q * (ITHREADS / 64));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int y = y0 + r0 + i, x = x0 + c;
+                m_out[q][i] = !BORDER || (y < height && x < width)
+                                  ? ap.mask[(size_t)y * width + x] : 0.0f;
+            }
+        }
+    }
     if constexpr (BORDER) {
         gw.load_mirrored(guide, width, height, y0 - R, x0 - R, GH, GW);
         stage<EH, EW, true, ITHREADS>(s_a + Win::ED, EP, est_in, width, height, y0 - 2 * R,
@@ -566,7 +583,7 @@ __device__ void iterate_strips(const float* __restrict__ est_in, const float* __
             // input; the median of three is then the reference's clamp.
             const float est = __builtin_amdgcn_fmed3f(v, g * 0.5f, g * 2.0f);
             if (APPLY)
-                apply_gain(ap, o + (size_t)i * width, est, g);
+                apply_gain(ap, o + (size_t)i * width, est, g, m_out[q][i]);
             else
                 est_out[o + (size_t)i * width] = est;
         }
@@ -657,7 +674,7 @@ __device__ void iterate_first_strips(const float* __restrict__ ratio,
             const float g = g_out[q][i];
             const float est = __builtin_amdgcn_fmed3f(g * b[i], g * 0.5f, g * 2.0f);
             if (APPLY)
-                apply_gain(ap, o + (size_t)i * width, est, g);
+                apply_gain(ap, o + (size_t)i * width, est, g, ap.mask[o + (size_t)i * width]);
             else
                 est_out[o + (size_t)i * width] = est;
         }
@@ -694,7 +711,7 @@ __global__ __launch_bounds__(THREADS) void cap_apply(float* __restrict__ rgb, in
                                                      const float* __restrict__ est) {
     int i = blockIdx.x * THREADS + threadIdx.x;
     if (i >= n) return;
-    apply_gain(Apply{rgb, mask, amount, (size_t)n}, i, est[i], guide[i]);
+    apply_gain(Apply{rgb, mask, amount, (size_t)n}, i, est[i], guide[i], mask[i]);
 }
 
 }  // namespace
