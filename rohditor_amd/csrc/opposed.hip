@@ -417,6 +417,43 @@ __device__ __forceinline__ unsigned predicate(const Layout& L, int mode, int i) 
     return lab != 0 && (L.flag8[i] >> 3) == 0 ? 1u : 0u;
 }
 
+// The predicate bits of cells base .. base + SCAN_PER_THREAD - 1 (bit k for
+// cell base + k). A thread's labels and flag bytes are contiguous and 16-byte
+// aligned (the layout's arrays start on 256-byte boundaries), so a whole run
+// arrives in four uint4 label loads and one uint4 flag load.
+__device__ __forceinline__ unsigned predicate_bits(const Layout& L, int mode, int base) {
+    static_assert(SCAN_PER_THREAD == 16, "four uint4 of labels, one of flag bytes");
+    unsigned bits = 0;
+    if (base + SCAN_PER_THREAD <= L.cells) {
+        unsigned lab[SCAN_PER_THREAD];
+        const uint4* lp = reinterpret_cast<const uint4*>(L.label + base);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = lp[q];
+            lab[4 * q] = v.x;
+            lab[4 * q + 1] = v.y;
+            lab[4 * q + 2] = v.z;
+            lab[4 * q + 3] = v.w;
+        }
+        if (mode == ROOTS) {
+#pragma unroll
+            for (int k = 0; k < SCAN_PER_THREAD; ++k)
+                bits |= (lab[k] == (unsigned)(base + k) ? 1u : 0u) << k;
+        } else {
+            const uint4 f = *reinterpret_cast<const uint4*>(L.flag8 + base);
+            const unsigned fw[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+            for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+                const unsigned flag = (fw[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                bits |= (lab[k] != 0 && (flag >> 3) == 0 ? 1u : 0u) << k;
+            }
+        }
+        return bits;
+    }
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) bits |= predicate(L, mode, base + k) << k;
+    return bits;
+}
+
 // Block-wide exclusive scan of one value per thread; returns the block total.
 __device__ unsigned block_exclusive_scan(unsigned v, unsigned& exclusive) {
     __shared__ unsigned wave_totals[THREADS / 64];
@@ -440,8 +477,7 @@ __device__ unsigned block_exclusive_scan(unsigned v, unsigned& exclusive) {
 
 __global__ __launch_bounds__(THREADS) void opp_scan_count(Layout L, int mode) {
     const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-    unsigned n = 0;
-    for (int k = 0; k < SCAN_PER_THREAD; ++k) n += predicate(L, mode, base + k);
+    const unsigned n = __popc(predicate_bits(L, mode, base));
     unsigned excl;
     unsigned total = block_exclusive_scan(n, excl);
     if (threadIdx.x == 0) L.block_sums[blockIdx.x] = total;
@@ -464,12 +500,7 @@ __global__ __launch_bounds__(THREADS) void opp_scan_blocks(Layout L, int blocks,
 
 __global__ __launch_bounds__(THREADS) void opp_scan_emit(Layout L, int mode) {
     const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-    unsigned flags = 0, n = 0;
-    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-        unsigned p = predicate(L, mode, base + k);
-        flags |= p << k;
-        n += p;
-    }
+    const unsigned flags = predicate_bits(L, mode, base), n = __popc(flags);
     unsigned excl;
     block_exclusive_scan(n, excl);
     unsigned pos = L.block_sums[blockIdx.x] + excl;
