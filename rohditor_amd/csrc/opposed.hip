@@ -421,11 +421,12 @@ __device__ __forceinline__ unsigned predicate(const Layout& L, int mode, int i) 
 // cell base + k). A thread's labels and flag bytes are contiguous and 16-byte
 // aligned (the layout's arrays start on 256-byte boundaries), so a whole run
 // arrives in four uint4 label loads and one uint4 flag load.
-__device__ __forceinline__ unsigned predicate_bits(const Layout& L, int mode, int base) {
+// The cells' labels are left in lab.
+__device__ __forceinline__ unsigned predicate_bits(const Layout& L, int mode, int base,
+                                                   unsigned (&lab)[SCAN_PER_THREAD]) {
     static_assert(SCAN_PER_THREAD == 16, "four uint4 of labels, one of flag bytes");
     unsigned bits = 0;
     if (base + SCAN_PER_THREAD <= L.cells) {
-        unsigned lab[SCAN_PER_THREAD];
         const uint4* lp = reinterpret_cast<const uint4*>(L.label + base);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -450,7 +451,10 @@ __device__ __forceinline__ unsigned predicate_bits(const Layout& L, int mode, in
         }
         return bits;
     }
-    for (int k = 0; k < SCAN_PER_THREAD; ++k) bits |= predicate(L, mode, base + k) << k;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        lab[k] = base + k < L.cells ? L.label[base + k] : 0u;
+        bits |= predicate(L, mode, base + k) << k;
+    }
     return bits;
 }
 
@@ -477,7 +481,8 @@ __device__ unsigned block_exclusive_scan(unsigned v, unsigned& exclusive) {
 
 __global__ __launch_bounds__(THREADS) void opp_scan_count(Layout L, int mode) {
     const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-    const unsigned n = __popc(predicate_bits(L, mode, base));
+    unsigned lab[SCAN_PER_THREAD];
+    const unsigned n = __popc(predicate_bits(L, mode, base, lab));
     unsigned excl;
     unsigned total = block_exclusive_scan(n, excl);
     if (threadIdx.x == 0) L.block_sums[blockIdx.x] = total;
@@ -500,13 +505,15 @@ __global__ __launch_bounds__(THREADS) void opp_scan_blocks(Layout L, int blocks,
 
 __global__ __launch_bounds__(THREADS) void opp_scan_emit(Layout L, int mode) {
     const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-    const unsigned flags = predicate_bits(L, mode, base), n = __popc(flags);
+    unsigned lab[SCAN_PER_THREAD];
+    const unsigned flags = predicate_bits(L, mode, base, lab), n = __popc(flags);
     unsigned excl;
     block_exclusive_scan(n, excl);
     unsigned pos = L.block_sums[blockIdx.x] + excl;
     // Ring entries of one label come in runs; each run updates its region's
     // first/last index once, and only when it would change them (the values
-    // only move one way, so an observed value that is already better holds).
+    // only move one way, so an observed value that is already better holds;
+    // unconditional atomics contend on the large regions and are slower).
     unsigned run_lab = NONE, run_first = 0, run_last = 0;
     auto flush = [&]() {
         if (run_lab >= (unsigned)L.cap) return;
@@ -520,10 +527,9 @@ __global__ __launch_bounds__(THREADS) void opp_scan_emit(Layout L, int mode) {
             L.rank[i] = pos;
         } else {
             L.ring_cell[pos] = (unsigned)i;
-            const unsigned lab = L.label[i];
-            if (lab != run_lab) {
+            if (lab[k] != run_lab) {
                 flush();
-                run_lab = lab;
+                run_lab = lab[k];
                 run_first = pos;
             }
             run_last = pos;
